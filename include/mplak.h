@@ -1,0 +1,198 @@
+/*
+ * mplak.h -- C ABI of the gfx950 kernel library (libmplak.so).
+ *
+ * This is the layer under the drop-in `mpla.h` API: plain pointers, 64-bit
+ * sizes and a HIP stream handle, no C++ or torch types.  Every entry point
+ *   - is stream-ordered (enqueues work on `stream`, never synchronises),
+ *   - allocates nothing (workspace comes from the caller, sized by the
+ *     matching *_workspace_bytes query),
+ *   - returns an MPLAK_* status code instead of exiting.
+ * All array arguments are device pointers unless stated otherwise.
+ *
+ * Each function names the reference operation it replaces (paths relative to
+ * the MPLA source tree).
+ */
+#ifndef MPLAK_H
+#define MPLAK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void* mplak_stream_t; /* a hipStream_t; NULL is the default stream */
+
+/* Status codes */
+#define MPLAK_OK 0
+#define MPLAK_ERR_ARG 1         /* invalid size, pointer or enum value */
+#define MPLAK_ERR_HIP 2         /* a HIP runtime call or kernel launch failed */
+#define MPLAK_ERR_WORKSPACE 3   /* caller workspace smaller than required */
+#define MPLAK_ERR_UNSUPPORTED 4 /* valid request this build does not implement */
+
+/* Kernel functions k(r) of the matrix-free operator, r = |p_i - p_j| */
+#define MPLAK_KERNEL_GAUSSIAN 0 /* exp(-r^2 / (2 l^2)); l = 1/sqrt(2) is exp(-r^2) of src/kernel.cu:12-15 */
+#define MPLAK_KERNEL_MATERN32 1 /* (1 + s) exp(-s), s = sqrt(3) r / l; l = sqrt(3) is src/kernel.cu:7-10 */
+
+/* Arithmetic used for kernel-entry evaluation and the product */
+#define MPLAK_PREC_F64 0    /* entries and accumulation in FP64 */
+#define MPLAK_PREC_F32 1    /* entries in FP32, FP32 partial sums folded into FP64 */
+#define MPLAK_PREC_BF16 2   /* multi-RHS only: BF16 operands on MFMA, FP32 accumulation */
+#define MPLAK_PREC_BF16X3 3 /* multi-RHS only: split hi+lo BF16 operands, 3 MFMA products */
+
+/* Stopping rules of the fused CG iteration */
+#define MPLAK_STOP_RMS 0      /* sqrt(<r,r>/N) < eps      (src/mpla.cpp:843,868) */
+#define MPLAK_STOP_RELATIVE 1 /* sqrt(<r,r>)/||b|| < eps  (src/mpla.cpp:920,963-965) */
+
+/* Solver state flags (mplak_cg_state.done) */
+#define MPLAK_RUNNING 0
+#define MPLAK_CONVERGED 1
+#define MPLAK_ITER_LIMIT 2
+
+const char* mplak_status_string(int status);
+
+/* Library version as 10000*major + 100*minor + patch. */
+int mplak_version(void);
+
+/* ---------------------------------------------------------------------------
+ * Synthetic inputs and plain vector fills
+ * ------------------------------------------------------------------------ */
+
+/* out[i] = lo + (hi - lo) * U(seed, stream, first_index + i), i in [0, n).
+ * U is the SplitMix64 output number (first_index + i) of a generator keyed by
+ * (seed, stream), mapped to [0,1) with 53 bits; the same integer recipe runs
+ * on the host, so GPU and CPU inputs are bitwise identical.
+ * Replaces curandGenerateUniformDouble (src/benchmark_dgemv.cpp:66-73,
+ * src/benchmark_kernel_cg.cpp:75-103). */
+int mplak_fill_uniform_f64(double* out, int64_t n, uint64_t seed, uint64_t stream,
+                           uint64_t first_index, double lo, double hi, mplak_stream_t stream_);
+
+/* out[i] = value.  Replaces cudaMemset in mpla_vector_set_zero
+ * (src/mpla.cpp:812-815); a fill kernel is also safe inside HIP graphs. */
+int mplak_fill_f64(double* out, int64_t n, double value, mplak_stream_t stream);
+
+/* dst[i] = src[i] (device to device, kernel copy). */
+int mplak_copy_f64(double* dst, const double* src, int64_t n, mplak_stream_t stream);
+
+/* ---------------------------------------------------------------------------
+ * Dense FP64 level 1/2
+ * ------------------------------------------------------------------------ */
+
+/* Workspace for mplak_dgemv_f64 on an m x n block. */
+size_t mplak_dgemv_workspace_bytes(int64_t m, int64_t n);
+
+/* y = A x with A column-major (element (i,j) at A[i + j*lda]), beta = 0.
+ * Replaces cublasDgemv(OP_N, m, n, 1, A, lda, x, 1, 0, y, 1) of
+ * mpla_dgemv (src/mpla.cpp:729) and of the stored-dense plugin core
+ * (src/test_cg.cpp:38-43, src/test_library.cpp:37-42).
+ * The column sum is split over a fixed number of slabs that depends only on
+ * (m, n); partial sums are combined in slab order, so results are
+ * deterministic.  If `skip_if` is non-NULL and *skip_if != 0 when the kernels
+ * run, y is left untouched (used by solvers that have already converged). */
+int mplak_dgemv_f64(int64_t m, int64_t n, const double* A, int64_t lda, const double* x,
+                    double* y, void* work, size_t work_bytes, const int32_t* skip_if,
+                    mplak_stream_t stream);
+
+size_t mplak_dot_workspace_bytes(int64_t n);
+
+/* *result = sum_i x[i] y[i] (result is a device pointer), deterministic order.
+ * Replaces cublasDdot of mpla_ddot (src/mpla.cpp:757). */
+int mplak_ddot_f64(int64_t n, const double* x, const double* y, double* result, void* work,
+                   size_t work_bytes, mplak_stream_t stream);
+
+/* y += alpha x.  Replaces cublasDaxpy of mpla_daxpy (src/mpla.cpp:809). */
+int mplak_daxpy_f64(int64_t n, double alpha, const double* x, double* y, mplak_stream_t stream);
+
+/* w = a x + b y (w may alias x or y). */
+int mplak_dwaxpby_f64(int64_t n, double a, const double* x, double b, const double* y, double* w,
+                      mplak_stream_t stream);
+
+/* ---------------------------------------------------------------------------
+ * Matrix-free kernel-matrix operator
+ * ------------------------------------------------------------------------ */
+
+typedef struct mplak_kernel_desc {
+  const double* const* points; /* device array of `dim` device pointers: points[d][i] is
+                                  coordinate d of global point i (the layout of
+                                  kernel_matrix_data.points, src/kernel.h:3-8) */
+  int32_t dim;                 /* 1 .. 16 */
+  int32_t kind;                /* MPLAK_KERNEL_* */
+  double length_scale;         /* > 0 */
+  double nugget;               /* added to entries whose global row == global column */
+  double coincident_shift;     /* added to entries whose point distance is < 1e-14
+                                  (src/kernel.cu:45-48 uses 1e-8; 0 disables the test) */
+  int64_t row_offset, row_count; /* global row range of the local block */
+  int64_t col_offset, col_count; /* global column range of the local block */
+} mplak_kernel_desc;
+
+size_t mplak_kernel_matvec_workspace_bytes(const mplak_kernel_desc* desc);
+
+/* y[i] = sum_j K(row_offset+i, col_offset+j) x[j] with entries computed on the
+ * fly from the points and never stored.  `precision` is MPLAK_PREC_F64 or
+ * MPLAK_PREC_F32.  Replaces the distance-matrix / apply_kernel / cublasDgemv
+ * sequence of mpla_dgemv_core_kernel_cublas and
+ * mpla_dgemv_core_kernel_streamed_cublas (src/kernel.cu:17-50, 87-171).
+ * Deterministic; `skip_if` as for mplak_dgemv_f64. */
+int mplak_kernel_matvec(const mplak_kernel_desc* desc, int precision, const double* x, double* y,
+                        void* work, size_t work_bytes, const int32_t* skip_if,
+                        mplak_stream_t stream);
+
+/* b[i] = prod_d (sqrt(pi)/2) (erf(1 - p_d) - erf(-p_d)) for global point
+ * row_offset + i: the integral of exp(-|p - y|^2) over y in [0,1]^dim.
+ * Replaces gen_gaussian_kernel_rhs (src/kernel.cu:52-68). */
+int mplak_gaussian_kernel_rhs(const double* const* points, int32_t dim, int64_t row_offset,
+                              int64_t row_count, double* b, mplak_stream_t stream);
+
+/* ---------------------------------------------------------------------------
+ * Fused conjugate-gradient steps with device-resident scalars
+ *
+ * One CG iteration of src/mpla.cpp:844-879 (and :937-975) is
+ *     z = A d                         (caller's operator, skip_if = &in->done)
+ *     mplak_cg_iterate(...)           (3 kernels: <d,z>; x,r update + <r,r>;
+ *                                      stopping test + d update)
+ * The state is double-buffered: iteration j reads state[j%2] and writes
+ * state[(j+1)%2], so no launch both reads and writes a field.  Once `done`
+ * is set, later iterations only carry the state forward, which makes a
+ * captured chunk of iterations safe to replay past convergence.
+ * ------------------------------------------------------------------------ */
+
+typedef struct mplak_cg_state {
+  double rr;       /* <r,r> of the current residual */
+  double bnorm2;   /* <b,b> */
+  double rel;      /* value of the stopping quantity after the last update */
+  double eps;      /* tolerance */
+  double n_global; /* global vector length (RMS rule) */
+  double alpha;    /* last step length (diagnostic) */
+  double beta;     /* last direction update coefficient (diagnostic) */
+  int32_t k;       /* completed iterations */
+  int32_t done;    /* MPLAK_RUNNING / MPLAK_CONVERGED / MPLAK_ITER_LIMIT */
+  int32_t iter_max;
+  int32_t rule;    /* MPLAK_STOP_* */
+} mplak_cg_state;
+
+size_t mplak_cg_workspace_bytes(int64_t n);
+
+/* Start: r = b - z (z = A x0 computed by the caller), d = r, and state with
+ * rr = <r,r>, bnorm2 = <b,b>, k = 0, done set when the initial residual
+ * already meets the rule (src/mpla.cpp:829-843, 899-936).  `state` is a
+ * device pointer to one mplak_cg_state. */
+int mplak_cg_start(int64_t n, const double* b, const double* z, double* r, double* d,
+                   mplak_cg_state* state, int rule, double eps, int32_t iter_max,
+                   double n_global, void* work, size_t work_bytes, mplak_stream_t stream);
+
+/* One iteration after z = A d: alpha = rr/<d,z>, x += alpha d, r -= alpha z,
+ * rr' = <r,r>, stopping test, d = r + (rr'/rr) d.  Reads *in, writes *out. */
+int mplak_cg_iterate(int64_t n, double* x, double* r, double* d, const double* z,
+                     const mplak_cg_state* in, mplak_cg_state* out, void* work,
+                     size_t work_bytes, mplak_stream_t stream);
+
+/* Recompute rr = <r,r> into *state (used after loading a checkpointed r). */
+int mplak_cg_refresh_rr(int64_t n, const double* r, mplak_cg_state* state, void* work,
+                        size_t work_bytes, mplak_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPLAK_H */

@@ -1,0 +1,125 @@
+"""ctypes bindings of the native libraries built into ``mpla_amd/lib``.
+
+``libmplak.so`` is the gfx950 kernel library (C ABI of ``include/mplak.h``);
+``libmpla.so`` is the drop-in host library (``include/mpla.h`` and friends).
+Both are built in-tree by ``__graft_entry__.build()`` (``make -C
+mpla_amd/csrc``).  Loading fails with an explicit error when a library is
+missing: there is no fallback implementation of any operation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+LIB_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
+
+MPLAK_OK = 0
+MPLAK_KERNEL_GAUSSIAN = 0
+MPLAK_KERNEL_MATERN32 = 1
+MPLAK_PREC_F64 = 0
+MPLAK_PREC_F32 = 1
+MPLAK_PREC_BF16 = 2
+MPLAK_PREC_BF16X3 = 3
+MPLAK_STOP_RMS = 0
+MPLAK_STOP_RELATIVE = 1
+MPLAK_RUNNING = 0
+MPLAK_CONVERGED = 1
+MPLAK_ITER_LIMIT = 2
+
+
+class NativeLibraryMissing(RuntimeError):
+    """The in-tree native library has not been built."""
+
+
+class MplakError(RuntimeError):
+    """A kernel-library call returned a non-zero status."""
+
+
+class mplak_kernel_desc(C.Structure):
+    _fields_ = [
+        ("points", C.c_void_p),
+        ("dim", C.c_int32),
+        ("kind", C.c_int32),
+        ("length_scale", C.c_double),
+        ("nugget", C.c_double),
+        ("coincident_shift", C.c_double),
+        ("row_offset", C.c_int64),
+        ("row_count", C.c_int64),
+        ("col_offset", C.c_int64),
+        ("col_count", C.c_int64),
+    ]
+
+
+class mplak_cg_state(C.Structure):
+    _fields_ = [
+        ("rr", C.c_double),
+        ("bnorm2", C.c_double),
+        ("rel", C.c_double),
+        ("eps", C.c_double),
+        ("n_global", C.c_double),
+        ("alpha", C.c_double),
+        ("beta", C.c_double),
+        ("k", C.c_int32),
+        ("done", C.c_int32),
+        ("iter_max", C.c_int32),
+        ("rule", C.c_int32),
+    ]
+
+
+_i64, _u64, _i32, _dbl, _vp, _sz = C.c_int64, C.c_uint64, C.c_int32, C.c_double, C.c_void_p, C.c_size_t
+
+# name -> (restype, argtypes)
+MPLAK_SIGNATURES = {
+    "mplak_status_string": (C.c_char_p, [C.c_int]),
+    "mplak_version": (C.c_int, []),
+    "mplak_fill_uniform_f64": (C.c_int, [_vp, _i64, _u64, _u64, _u64, _dbl, _dbl, _vp]),
+    "mplak_fill_f64": (C.c_int, [_vp, _i64, _dbl, _vp]),
+    "mplak_copy_f64": (C.c_int, [_vp, _vp, _i64, _vp]),
+    "mplak_dgemv_workspace_bytes": (_sz, [_i64, _i64]),
+    "mplak_dgemv_f64": (C.c_int, [_i64, _i64, _vp, _i64, _vp, _vp, _vp, _sz, _vp, _vp]),
+    "mplak_dot_workspace_bytes": (_sz, [_i64]),
+    "mplak_ddot_f64": (C.c_int, [_i64, _vp, _vp, _vp, _vp, _sz, _vp]),
+    "mplak_daxpy_f64": (C.c_int, [_i64, _dbl, _vp, _vp, _vp]),
+    "mplak_dwaxpby_f64": (C.c_int, [_i64, _dbl, _vp, _dbl, _vp, _vp, _vp]),
+    "mplak_kernel_matvec_workspace_bytes": (_sz, [C.POINTER(mplak_kernel_desc)]),
+    "mplak_kernel_matvec": (C.c_int, [C.POINTER(mplak_kernel_desc), C.c_int, _vp, _vp, _vp, _sz, _vp, _vp]),
+    "mplak_gaussian_kernel_rhs": (C.c_int, [_vp, _i32, _i64, _i64, _vp, _vp]),
+    "mplak_cg_workspace_bytes": (_sz, [_i64]),
+    "mplak_cg_start": (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, C.c_int, _dbl, _i32, _dbl, _vp, _sz, _vp]),
+    "mplak_cg_iterate": (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
+    "mplak_cg_refresh_rr": (C.c_int, [_i64, _vp, _vp, _vp, _sz, _vp]),
+}
+
+_libs: dict[str, C.CDLL] = {}
+
+
+def lib_path(name: str) -> str:
+    return os.path.join(LIB_DIR, name)
+
+
+def _load(name: str, signatures: dict) -> C.CDLL:
+    if name in _libs:
+        return _libs[name]
+    path = lib_path(name)
+    if not os.path.exists(path):
+        raise NativeLibraryMissing(
+            f"{path} is not built; run `python -c 'import __graft_entry__ as g; g.build()'` "
+            "(there is no non-native fallback)")
+    lib = C.CDLL(path, mode=C.RTLD_GLOBAL)
+    for fname, (res, args) in signatures.items():
+        fn = getattr(lib, fname)
+        fn.restype = res
+        fn.argtypes = args
+    _libs[name] = lib
+    return lib
+
+
+def mplak() -> C.CDLL:
+    """The kernel library (raises NativeLibraryMissing if it is not built)."""
+    return _load("libmplak.so", MPLAK_SIGNATURES)
+
+
+def check(status: int, what: str = "mplak call") -> None:
+    if status != MPLAK_OK:
+        msg = mplak().mplak_status_string(status).decode()
+        raise MplakError(f"{what} failed with status {status} ({msg})")

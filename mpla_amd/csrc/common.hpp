@@ -1,0 +1,74 @@
+// Shared device/host helpers for the gfx950 kernel library (libmplak).
+//
+// Everything here is stream-ordered and allocation-free: the C-ABI entry
+// points receive every buffer they touch (inputs, outputs, workspace) from
+// the caller and only enqueue kernels on the caller's stream.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+#include "../../include/mplak.h"
+
+namespace mplak {
+
+constexpr int kWave = 64;          // CDNA wavefront width
+constexpr int kBlock = 256;        // default workgroup: 4 waves
+
+// Status helpers -----------------------------------------------------------
+
+inline int hip_status(hipError_t e) { return e == hipSuccess ? MPLAK_OK : MPLAK_ERR_HIP; }
+
+inline int launch_status() {
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? MPLAK_OK : MPLAK_ERR_HIP;
+}
+
+inline hipStream_t as_stream(mplak_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// Deterministic reductions -------------------------------------------------
+//
+// Every reduction below combines values in an order that depends only on the
+// launch shape, never on timing: wave reductions use a fixed xor-butterfly and
+// block reductions sum the per-wave results in wave order.  Repeated calls on
+// the same inputs therefore return bitwise-identical sums.
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+// Sum over a block of NT threads; the result is valid in every thread.
+// `scratch` must hold NT / kWave elements of T and is reused after a barrier.
+template <int NT, typename T>
+__device__ __forceinline__ T block_sum(T v, T* scratch) {
+  constexpr int NW = NT / kWave;
+  v = wave_sum(v);
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = threadIdx.x / kWave;
+  __syncthreads();
+  if (lane == 0) scratch[w] = v;
+  __syncthreads();
+  T s = scratch[0];
+#pragma unroll
+  for (int i = 1; i < NW; ++i) s += scratch[i];
+  return s;
+}
+
+// Sum of `count` partials laid out contiguously, evaluated by one block in a
+// fixed order (strided per-thread sums, then block_sum).  Every block that
+// calls this on the same data obtains the same bits.
+template <int NT>
+__device__ __forceinline__ double reduce_partials(const double* __restrict__ p, int count,
+                                                  double* scratch) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < count; i += NT) s += p[i];
+  return block_sum<NT>(s, scratch);
+}
+
+}  // namespace mplak

@@ -1,0 +1,310 @@
+// Matrix-free kernel-matrix products for gfx950.
+//
+// y_i = sum_j k(|p_i - p_j|) x_j (+ nugget x_i on the global diagonal,
+// + coincident_shift x_j where |p_i - p_j| < 1e-14), entries never stored.
+//
+// Tiling: a 256-thread workgroup owns a tile of 512 target points (two per
+// thread, held in registers) and one slab of source columns.  The slab is
+// streamed through LDS in chunks of 256 points (coordinates pre-scaled so the
+// kernel argument is a plain sum of squares) together with the matching x
+// values; every lane reads the same LDS word at a time (broadcast, no bank
+// conflicts).  Slab partial sums go to a workspace and a second kernel adds
+// them in slab order, so the result is deterministic for a given shape.
+//
+// Entry cost is VALU-bound: FP64 pays a software exp (~20 FP64 ops); FP32 uses
+// the hardware v_exp_f32 / v_sqrt_f32 and folds each chunk's FP32 partial sum
+// into an FP64 accumulator.
+
+#include <math.h>
+
+#include "common.hpp"
+
+namespace mplak {
+namespace {
+
+constexpr int kKmvTpt = 2;                     // target points per thread
+constexpr int kKmvTile = kBlock * kKmvTpt;     // 512 targets per workgroup
+constexpr int kKmvChunk = kBlock;              // sources per LDS chunk
+constexpr int kKmvMaxDim = 16;
+constexpr int kKmvTargetBlocks = 4096;
+constexpr double kLog2e = 1.4426950408889634074;
+
+struct KmvPlan {
+  int64_t row_tiles, slabs, slab_cols;
+};
+
+inline KmvPlan kmv_plan(int64_t rows, int64_t cols) {
+  KmvPlan p;
+  p.row_tiles = (rows + kKmvTile - 1) / kKmvTile;
+  int64_t ks = (kKmvTargetBlocks + p.row_tiles - 1) / p.row_tiles;
+  int64_t max_ks = (cols + kKmvChunk - 1) / kKmvChunk;  // at least one chunk per slab
+  if (ks > max_ks) ks = max_ks;
+  if (ks < 1) ks = 1;
+  if (ks > 65535) ks = 65535;
+  // slab width is a whole number of chunks
+  int64_t chunks = (cols + kKmvChunk - 1) / kKmvChunk;
+  int64_t chunks_per_slab = (chunks + ks - 1) / ks;
+  p.slab_cols = chunks_per_slab * kKmvChunk;
+  p.slabs = (cols + p.slab_cols - 1) / p.slab_cols;
+  return p;
+}
+
+// Coordinate scale so that the entry is a function of a plain sum of squares.
+//   Gaussian  FP64: s = |d|^2 / (2 l^2),  k = exp(-s)
+//   Gaussian  FP32: s = |d|^2 log2(e) / (2 l^2),  k = exp2(-s)
+//   Matern32  both: s = 3 |d|^2 / l^2, r = sqrt(s), k = (1 + r) exp(-r)
+inline double coord_scale(int kind, int precision, double l) {
+  if (kind == MPLAK_KERNEL_GAUSSIAN) {
+    double c = 1.0 / (sqrt(2.0) * l);
+    if (precision == MPLAK_PREC_F32) c *= sqrt(kLog2e);
+    return c;
+  }
+  return sqrt(3.0) / l;
+}
+
+template <typename T, int KIND>
+__device__ __forceinline__ T kernel_entry(T s);
+
+template <>
+__device__ __forceinline__ double kernel_entry<double, MPLAK_KERNEL_GAUSSIAN>(double s) {
+  return exp(-s);
+}
+template <>
+__device__ __forceinline__ double kernel_entry<double, MPLAK_KERNEL_MATERN32>(double s) {
+  const double r = sqrt(s);
+  const double e = exp(-r);
+  return fma(r, e, e);
+}
+template <>
+__device__ __forceinline__ float kernel_entry<float, MPLAK_KERNEL_GAUSSIAN>(float s) {
+  return __builtin_amdgcn_exp2f(-s);
+}
+template <>
+__device__ __forceinline__ float kernel_entry<float, MPLAK_KERNEL_MATERN32>(float s) {
+  const float r = __builtin_amdgcn_sqrtf(s);
+  const float e = __builtin_amdgcn_exp2f(-r * (float)kLog2e);
+  return fmaf(r, e, e);
+}
+
+// DIM > 0: compile-time dimension; DIM == 0: runtime dim <= kKmvMaxDim.
+template <typename T, int KIND, int DIM, bool COINC>
+__global__ __launch_bounds__(kBlock) void kmv_partial_kernel(
+    const double* const* __restrict__ points, int dim_rt, int64_t row_off, int64_t rows,
+    int64_t col_off, int64_t cols, int64_t slab_cols, double scale, double coinc_thr,
+    double coinc_shift, const double* __restrict__ x, double* __restrict__ out,
+    const int32_t* __restrict__ skip) {
+  if (skip != nullptr && *skip != 0) return;
+  constexpr int MD = DIM > 0 ? DIM : kKmvMaxDim;
+  const int dim = DIM > 0 ? DIM : dim_rt;
+  __shared__ T lp[MD][kKmvChunk];
+  __shared__ T lx[kKmvChunk];
+
+  const int tid = threadIdx.x;
+  const int64_t tile0 = (int64_t)blockIdx.x * kKmvTile;
+  const T sc = (T)scale;
+  const T thr = (T)coinc_thr;
+  const T shift = (T)coinc_shift;
+
+  // Target coordinates in registers (rows past the end are computed on a
+  // copy of the last row and never stored).
+  T pi[kKmvTpt][MD];
+#pragma unroll
+  for (int t = 0; t < kKmvTpt; ++t) {
+    int64_t i = tile0 + tid + t * kBlock;
+    if (i >= rows) i = rows - 1;
+#pragma unroll
+    for (int d = 0; d < MD; ++d) pi[t][d] = d < dim ? (T)(points[d][row_off + i] * scale) : T(0);
+  }
+
+  const int64_t c0 = (int64_t)blockIdx.y * slab_cols;
+  const int64_t c1 = (c0 + slab_cols < cols) ? c0 + slab_cols : cols;
+
+  double acc[kKmvTpt];
+#pragma unroll
+  for (int t = 0; t < kKmvTpt; ++t) acc[t] = 0.0;
+
+  for (int64_t cb = c0; cb < c1; cb += kKmvChunk) {
+    __syncthreads();
+    {
+      const int64_t j = cb + tid;
+      if (j < c1) {
+#pragma unroll
+        for (int d = 0; d < MD; ++d) lp[d][tid] = d < dim ? (T)(points[d][col_off + j] * scale) : T(0);
+        lx[tid] = (T)x[j];
+      } else {
+#pragma unroll
+        for (int d = 0; d < MD; ++d) lp[d][tid] = T(0);
+        lx[tid] = T(0);
+      }
+    }
+    __syncthreads();
+    T cacc[kKmvTpt];
+#pragma unroll
+    for (int t = 0; t < kKmvTpt; ++t) cacc[t] = T(0);
+#pragma unroll 4
+    for (int jj = 0; jj < kKmvChunk; ++jj) {
+      T pj[MD];
+#pragma unroll
+      for (int d = 0; d < MD; ++d) pj[d] = d < dim ? lp[d][jj] : T(0);
+      const T xj = lx[jj];
+#pragma unroll
+      for (int t = 0; t < kKmvTpt; ++t) {
+        T s = T(0);
+#pragma unroll
+        for (int d = 0; d < MD; ++d) {
+          if (d < dim) {
+            const T df = pi[t][d] - pj[d];
+            s = fma(df, df, s);
+          }
+        }
+        T k = kernel_entry<T, KIND>(s);
+        if (COINC) k += (s < thr) ? shift : T(0);
+        cacc[t] = fma(k, xj, cacc[t]);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < kKmvTpt; ++t) acc[t] += (double)cacc[t];
+  }
+  (void)sc;
+
+#pragma unroll
+  for (int t = 0; t < kKmvTpt; ++t) {
+    const int64_t i = tile0 + tid + t * kBlock;
+    if (i < rows) out[(int64_t)blockIdx.y * rows + i] = acc[t];
+  }
+}
+
+// y_i = sum over slabs (slab order) + nugget * x at the global diagonal.
+__global__ __launch_bounds__(kBlock) void kmv_finish_kernel(
+    int64_t rows, int64_t slabs, const double* __restrict__ part, int64_t row_off,
+    int64_t col_off, int64_t cols, double nugget, const double* __restrict__ x,
+    double* __restrict__ y, const int32_t* __restrict__ skip) {
+  if (skip != nullptr && *skip != 0) return;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < rows; i += stride) {
+    double s = part[i];
+    for (int64_t k = 1; k < slabs; ++k) s += part[k * rows + i];
+    const int64_t jd = row_off + i - col_off;
+    if (nugget != 0.0 && jd >= 0 && jd < cols) s = fma(nugget, x[jd], s);
+    y[i] = s;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void gaussian_rhs_kernel(const double* const* __restrict__ points,
+                                                              int dim, int64_t row_off, int64_t rows,
+                                                              double* __restrict__ b) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const double half_sqrt_pi = 0.88622692545275801365;  // sqrt(pi) / 2
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < rows; i += stride) {
+    double v = 1.0;
+    for (int d = 0; d < dim; ++d) {
+      const double p = points[d][row_off + i];
+      v = v * half_sqrt_pi * (erf(1.0 - p) - erf(-p));
+    }
+    b[i] = v;
+  }
+}
+
+template <typename T, int KIND, int DIM>
+void launch_partial(dim3 grid, hipStream_t st, bool coinc, const mplak_kernel_desc* d,
+                    int64_t slab_cols, double scale, double thr, const double* x, double* out,
+                    const int32_t* skip) {
+  if (coinc)
+    hipLaunchKernelGGL((kmv_partial_kernel<T, KIND, DIM, true>), grid, dim3(kBlock), 0, st,
+                       d->points, d->dim, d->row_offset, d->row_count, d->col_offset, d->col_count,
+                       slab_cols, scale, thr, d->coincident_shift, x, out, skip);
+  else
+    hipLaunchKernelGGL((kmv_partial_kernel<T, KIND, DIM, false>), grid, dim3(kBlock), 0, st,
+                       d->points, d->dim, d->row_offset, d->row_count, d->col_offset, d->col_count,
+                       slab_cols, scale, thr, d->coincident_shift, x, out, skip);
+}
+
+template <typename T, int KIND>
+void dispatch_dim(dim3 grid, hipStream_t st, bool coinc, const mplak_kernel_desc* d,
+                  int64_t slab_cols, double scale, double thr, const double* x, double* out,
+                  const int32_t* skip) {
+  switch (d->dim) {
+    case 1: launch_partial<T, KIND, 1>(grid, st, coinc, d, slab_cols, scale, thr, x, out, skip); break;
+    case 2: launch_partial<T, KIND, 2>(grid, st, coinc, d, slab_cols, scale, thr, x, out, skip); break;
+    case 3: launch_partial<T, KIND, 3>(grid, st, coinc, d, slab_cols, scale, thr, x, out, skip); break;
+    case 4: launch_partial<T, KIND, 4>(grid, st, coinc, d, slab_cols, scale, thr, x, out, skip); break;
+    default: launch_partial<T, KIND, 0>(grid, st, coinc, d, slab_cols, scale, thr, x, out, skip); break;
+  }
+}
+
+int check_desc(const mplak_kernel_desc* d) {
+  if (d == nullptr || d->points == nullptr) return MPLAK_ERR_ARG;
+  if (d->dim < 1 || d->dim > kKmvMaxDim) return MPLAK_ERR_ARG;
+  if (d->kind != MPLAK_KERNEL_GAUSSIAN && d->kind != MPLAK_KERNEL_MATERN32) return MPLAK_ERR_ARG;
+  if (!(d->length_scale > 0.0)) return MPLAK_ERR_ARG;
+  if (d->row_count < 0 || d->col_count < 0 || d->row_offset < 0 || d->col_offset < 0)
+    return MPLAK_ERR_ARG;
+  return MPLAK_OK;
+}
+
+}  // namespace
+}  // namespace mplak
+
+using namespace mplak;
+
+extern "C" {
+
+size_t mplak_kernel_matvec_workspace_bytes(const mplak_kernel_desc* d) {
+  if (d == nullptr || d->row_count <= 0 || d->col_count <= 0) return 0;
+  const KmvPlan p = kmv_plan(d->row_count, d->col_count);
+  return (size_t)p.slabs * (size_t)d->row_count * sizeof(double);
+}
+
+int mplak_kernel_matvec(const mplak_kernel_desc* d, int precision, const double* x, double* y,
+                        void* work, size_t work_bytes, const int32_t* skip_if, mplak_stream_t s) {
+  int st = check_desc(d);
+  if (st != MPLAK_OK) return st;
+  if (precision != MPLAK_PREC_F64 && precision != MPLAK_PREC_F32) return MPLAK_ERR_ARG;
+  if (d->row_count == 0) return MPLAK_OK;
+  if (y == nullptr) return MPLAK_ERR_ARG;
+  if (d->col_count == 0) return mplak_fill_f64(y, d->row_count, 0.0, s);
+  if (x == nullptr) return MPLAK_ERR_ARG;
+  const size_t need = mplak_kernel_matvec_workspace_bytes(d);
+  if (work == nullptr || work_bytes < need) return MPLAK_ERR_WORKSPACE;
+
+  const KmvPlan p = kmv_plan(d->row_count, d->col_count);
+  const double scale = coord_scale(d->kind, precision, d->length_scale);
+  const bool coinc = d->coincident_shift != 0.0;
+  const double thr = (1e-14 * scale) * (1e-14 * scale);
+  double* part = static_cast<double*>(work);
+  const dim3 grid((unsigned)p.row_tiles, (unsigned)p.slabs);
+  hipStream_t hs = as_stream(s);
+  if (precision == MPLAK_PREC_F64) {
+    if (d->kind == MPLAK_KERNEL_GAUSSIAN)
+      dispatch_dim<double, MPLAK_KERNEL_GAUSSIAN>(grid, hs, coinc, d, p.slab_cols, scale, thr, x, part, skip_if);
+    else
+      dispatch_dim<double, MPLAK_KERNEL_MATERN32>(grid, hs, coinc, d, p.slab_cols, scale, thr, x, part, skip_if);
+  } else {
+    if (d->kind == MPLAK_KERNEL_GAUSSIAN)
+      dispatch_dim<float, MPLAK_KERNEL_GAUSSIAN>(grid, hs, coinc, d, p.slab_cols, scale, thr, x, part, skip_if);
+    else
+      dispatch_dim<float, MPLAK_KERNEL_MATERN32>(grid, hs, coinc, d, p.slab_cols, scale, thr, x, part, skip_if);
+  }
+  st = launch_status();
+  if (st != MPLAK_OK) return st;
+  int64_t fb = (d->row_count + kBlock - 1) / kBlock;
+  if (fb > 4096) fb = 4096;
+  hipLaunchKernelGGL(kmv_finish_kernel, dim3((unsigned)fb), dim3(kBlock), 0, hs, d->row_count,
+                     p.slabs, part, d->row_offset, d->col_offset, d->col_count, d->nugget, x, y,
+                     skip_if);
+  return launch_status();
+}
+
+int mplak_gaussian_kernel_rhs(const double* const* points, int32_t dim, int64_t row_offset,
+                              int64_t row_count, double* b, mplak_stream_t s) {
+  if (points == nullptr || dim < 1 || row_offset < 0 || row_count < 0) return MPLAK_ERR_ARG;
+  if (row_count == 0) return MPLAK_OK;
+  if (b == nullptr) return MPLAK_ERR_ARG;
+  int64_t g = (row_count + kBlock - 1) / kBlock;
+  if (g > 4096) g = 4096;
+  hipLaunchKernelGGL(gaussian_rhs_kernel, dim3((unsigned)g), dim3(kBlock), 0, as_stream(s), points,
+                     dim, row_offset, row_count, b);
+  return launch_status();
+}
+
+}  // extern "C"

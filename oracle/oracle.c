@@ -1,0 +1,489 @@
+/*
+ * oracle.c -- CPU restatement of MPLA's solver hot path (TEST INFRASTRUCTURE).
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load this library; the product never does.  Every function cites the
+ * reference source it restates (paths relative to the MPLA source tree).
+ *
+ * How each part is pinned (checked against something other than itself,
+ * see tests/test_oracle.py):
+ *   rng            SplitMix64 published test vector (seed 1234567) and
+ *                  distribution moments.
+ *   block counts   hand-evaluated cases of src/mpla.cpp:157-209 and the
+ *                  invariants sum = M, |count_p - count_q| <= 1, prefix offsets.
+ *   dgemv / ddot   closed forms of src/test_library.cpp (A_ij = j*m+i,
+ *                  x_j = j, y_j = j+1), exact in both modes; numpy matmul.
+ *   kernel entry   closed forms: k(0) = 1 (+ nugget on i==j, + shift on
+ *                  coincident points), Gaussian/Matern values at known r,
+ *                  the reference's unscaled forms at l = 1/sqrt(2) and sqrt(3).
+ *   kernel matvec  brute force with numpy/scipy on small N; symmetry
+ *                  x^T K y = y^T K x; mode 0 vs mode 1 agreement.
+ *   gaussian rhs   the 1-D integral identity checked by scipy quadrature.
+ *   CG (both)      1-D Laplacian manufactured solution (src/test_cg.cpp:57-113),
+ *                  alpha*I converges in one iteration, k distinct eigenvalues
+ *                  converge in <= k iterations, scipy.sparse.linalg.cg.
+ *   BiCGSTAB       manufactured solution on D + R (nonsymmetric), true
+ *                  residual below eps at exit, numpy.linalg.solve.
+ *   vector files   committed byte fixture and round trip; length mismatch.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* SplitMix64 (Steele, Lea, Flood 2014).                                      */
+
+static const uint64_t GOLDEN = 0x9E3779B97F4A7C15ull;
+
+static uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+uint64_t oracle_rng_key(uint64_t seed, uint64_t stream) { return mix64(seed * GOLDEN + stream); }
+
+uint64_t oracle_rng_u64(uint64_t key, uint64_t i) { return mix64(key + (i + 1) * GOLDEN); }
+
+void oracle_fill_uniform(double* out, int64_t n, uint64_t seed, uint64_t stream,
+                         uint64_t first_index, double lo, double hi) {
+  const uint64_t key = oracle_rng_key(seed, stream);
+  const double span = hi - lo;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) {
+    const double u = (double)(oracle_rng_u64(key, first_index + (uint64_t)i) >> 11) * 0x1.0p-53;
+    out[i] = fma(span, u, lo);
+  }
+}
+
+void oracle_fill_normal(double* out, int64_t n, uint64_t seed, uint64_t stream,
+                        uint64_t first_index) {
+  const uint64_t key = oracle_rng_key(seed, stream);
+  const double two_pi = 6.283185307179586476925;
+  for (int64_t i = 0; i < n; ++i) {
+    const uint64_t k = 2 * (first_index + (uint64_t)i);
+    const double u1 = (double)((oracle_rng_u64(key, k) >> 11) + 1) * 0x1.0p-53;
+    const double u2 = (double)(oracle_rng_u64(key, k + 1) >> 11) * 0x1.0p-53;
+    out[i] = sqrt(-2.0 * log(u1)) * cos(two_pi * u2);
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Balanced block distribution: src/mpla.cpp:157-209 (matrix rows/columns),
+ * :338-367 (vectors over process rows), :390-439 (vectors over process
+ * columns) all use count_p = floor(M/P) + (p < M mod P), offsets by prefix
+ * sums, and exit when floor(M/P) == 0. */
+int oracle_block_counts(int64_t M, int P, int64_t* counts, int64_t* offsets) {
+  if (P <= 0) return -1;
+  const int64_t base = M / P;
+  const int64_t rem = M % P;
+  if (base == 0) return -1;
+  int64_t off = 0;
+  for (int p = 0; p < P; ++p) {
+    counts[p] = base + (p < rem ? 1 : 0);
+    offsets[p] = off;
+    off += counts[p];
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Dense level 1/2: cublasDgemv / cublasDdot / cublasDaxpy of
+ * src/mpla.cpp:729, :757, :809. */
+
+void oracle_dgemv(int64_t m, int64_t n, const double* A, int64_t lda, const double* x, double* y,
+                  int mode) {
+  if (mode == 0) {
+    /* row blocks per thread, columns streamed in order */
+#pragma omp parallel
+    {
+#ifdef _OPENMP
+      const int nt = omp_get_num_threads(), t = omp_get_thread_num();
+#else
+      const int nt = 1, t = 0;
+#endif
+      const int64_t chunk = (m + nt - 1) / nt;
+      const int64_t i0 = t * chunk;
+      const int64_t i1 = (i0 + chunk < m) ? i0 + chunk : m;
+      for (int64_t i = i0; i < i1; ++i) y[i] = 0.0;
+      for (int64_t j = 0; j < n; ++j) {
+        const double xj = x[j];
+        const double* a = A + j * lda;
+        for (int64_t i = i0; i < i1; ++i) y[i] += a[i] * xj;
+      }
+    }
+  } else {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < m; ++i) {
+      long double s = 0.0L;
+      for (int64_t j = 0; j < n; ++j) s += (long double)A[i + j * lda] * (long double)x[j];
+      y[i] = (double)s;
+    }
+  }
+}
+
+double oracle_ddot(int64_t n, const double* x, const double* y, int mode) {
+  if (mode == 0) {
+    double s = 0.0;
+    for (int64_t i = 0; i < n; ++i) s += x[i] * y[i];
+    return s;
+  }
+  long double s = 0.0L;
+  for (int64_t i = 0; i < n; ++i) s += (long double)x[i] * (long double)y[i];
+  return (double)s;
+}
+
+void oracle_daxpy(int64_t n, double alpha, const double* x, double* y) {
+  for (int64_t i = 0; i < n; ++i) y[i] = y[i] + alpha * x[i];
+}
+
+/* ------------------------------------------------------------------------ */
+/* Kernel entries.  src/kernel.cu:7-15 defines (1+r)exp(-r) and exp(-r*r) of
+ * the point distance r; :39-50 adds 1e-8 when r < 1e-14.  The length scale
+ * generalises them: Gaussian exp(-r^2/(2 l^2)) (l = 1/sqrt(2) gives exp(-r^2)),
+ * Matern-3/2 (1+s)exp(-s) with s = sqrt(3) r / l (l = sqrt(3) gives the
+ * reference's (1+r)exp(-r)).  The nugget on i == j follows the assembler's
+ * regularization rule (src/kernel_system_assembler.h:55-68). */
+
+double oracle_kernel_entry(const oracle_kernel* k, int64_t i, int64_t j) {
+  double r2 = 0.0;
+  for (int d = 0; d < k->dim; ++d) {
+    const double df = k->points[(int64_t)d * k->n_points + i] - k->points[(int64_t)d * k->n_points + j];
+    r2 += df * df;
+  }
+  const double r = sqrt(r2);
+  const double l = k->length_scale;
+  double v;
+  if (k->kind == ORACLE_KERNEL_GAUSSIAN) {
+    v = exp(-r2 / (2.0 * l * l));
+  } else {
+    const double s = sqrt(3.0) * r / l;
+    v = (1.0 + s) * exp(-s);
+  }
+  if (r < 1.0e-14) v += k->coincident_shift;
+  if (i == j) v += k->nugget;
+  return v;
+}
+
+void oracle_kernel_matvec(const oracle_kernel* k, int64_t row_off, int64_t rows, int64_t col_off,
+                          int64_t cols, const double* x, double* y, int mode) {
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t i = 0; i < rows; ++i) {
+    if (mode == 0) {
+      double s = 0.0;
+      for (int64_t j = 0; j < cols; ++j) s += oracle_kernel_entry(k, row_off + i, col_off + j) * x[j];
+      y[i] = s;
+    } else {
+      long double s = 0.0L;
+      for (int64_t j = 0; j < cols; ++j)
+        s += (long double)oracle_kernel_entry(k, row_off + i, col_off + j) * (long double)x[j];
+      y[i] = (double)s;
+    }
+  }
+}
+
+void oracle_gaussian_rhs(const oracle_kernel* k, int64_t row_off, int64_t rows, double* b) {
+  const double half_sqrt_pi = 0.88622692545275801365;
+  for (int64_t i = 0; i < rows; ++i) {
+    double v = 1.0;
+    for (int d = 0; d < k->dim; ++d) {
+      const double p = k->points[(int64_t)d * k->n_points + row_off + i];
+      v = v * half_sqrt_pi * (erf(1.0 - p) - erf(0.0 - p));
+    }
+    b[i] = v;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+
+void oracle_apply(const oracle_op* op, const double* x, double* y) {
+  if (op->type == 0)
+    oracle_dgemv(op->n, op->n, op->A, op->lda, x, y, op->mode);
+  else
+    oracle_kernel_matvec(&op->kernel, 0, op->n, 0, op->n, x, y, op->mode);
+}
+
+static double dot(int64_t n, const double* a, const double* b) { return oracle_ddot(n, a, b, 1); }
+
+static void set_zero(int64_t n, double* v) { memset(v, 0, sizeof(double) * (size_t)n); }
+
+static void ckpt_path(char* buf, size_t len, const char* dir, const char* name) {
+  snprintf(buf, len, "%s/%s", (dir && dir[0]) ? dir : ".", name);
+}
+
+static void save_named(const char* dir, const char* name, const double* v, int64_t n) {
+  char p[4096];
+  ckpt_path(p, sizeof p, dir, name);
+  oracle_save_vector(p, v, (int32_t)n);
+}
+
+static int load_named(const char* dir, const char* name, double* v, int64_t n) {
+  char p[4096];
+  ckpt_path(p, sizeof p, dir, name);
+  return oracle_load_vector(p, v, (int32_t)n);
+}
+
+/* src/mpla.cpp:817-885 */
+void oracle_cg_rms(const oracle_op* op, const double* b, double* x, int iter_max, double eps,
+                   oracle_solve_info* info) {
+  const int64_t n = op->n;
+  double* r = calloc((size_t)n, sizeof(double));
+  double* d = calloc((size_t)n, sizeof(double));
+  double* z = calloc((size_t)n, sizeof(double));
+  int matvecs = 0;
+  oracle_apply(op, x, z);
+  ++matvecs;
+  set_zero(n, r);
+  oracle_daxpy(n, 1.0, b, r);
+  oracle_daxpy(n, -1.0, z, r);
+  set_zero(n, d);
+  oracle_daxpy(n, 1.0, r, d);
+  double res = dot(n, r, r);
+  int k = 0, converged = 0;
+  double rel = sqrt(res / (double)n);
+  if (rel >= eps) {
+    for (k = 1; k < iter_max; ++k) {
+      oracle_apply(op, d, z);
+      ++matvecs;
+      const double t1 = dot(n, r, r);
+      double t2 = dot(n, d, z);
+      const double alpha = t1 / t2;
+      oracle_daxpy(n, alpha, d, x);
+      oracle_daxpy(n, -alpha, z, r);
+      t2 = dot(n, r, r);
+      const double beta = t2 / t1;
+      rel = sqrt(t2 / (double)n);
+      if (rel < eps) {
+        converged = 1;
+        break;
+      }
+      set_zero(n, z);
+      oracle_daxpy(n, beta, d, z);
+      oracle_daxpy(n, 1.0, r, z);
+      set_zero(n, d);
+      oracle_daxpy(n, 1.0, z, d);
+    }
+  } else {
+    converged = 1;
+  }
+  if (info) {
+    info->iterations = k;
+    info->converged = converged;
+    info->final_rel = rel;
+    info->matvecs = matvecs;
+  }
+  free(r);
+  free(d);
+  free(z);
+}
+
+/* src/mpla.cpp:887-1001 */
+void oracle_cg_relative(const oracle_op* op, const double* b, double* x, int iter_max, double eps,
+                        int ckpt_every, int restart, const char* dir, oracle_solve_info* info) {
+  const int64_t n = op->n;
+  double* r = calloc((size_t)n, sizeof(double));
+  double* d = calloc((size_t)n, sizeof(double));
+  double* z = calloc((size_t)n, sizeof(double));
+  int matvecs = 0;
+  const double b_norm = sqrt(dot(n, b, b));
+  oracle_apply(op, x, z);
+  ++matvecs;
+  set_zero(n, r);
+  oracle_daxpy(n, 1.0, b, r);
+  oracle_daxpy(n, -1.0, z, r);
+  set_zero(n, d);
+  oracle_daxpy(n, 1.0, r, d);
+  const double res = dot(n, r, r);
+  double rel = sqrt(res) / b_norm;
+  if (restart) {
+    /* loaded after the initial residual is formed (src/mpla.cpp:920-933) */
+    load_named(dir, "x.dat", x, n);
+    load_named(dir, "r.dat", r, n);
+    load_named(dir, "d.dat", d, n);
+    load_named(dir, "z.dat", z, n);
+  }
+  int k = 0, converged = 0;
+  if (rel >= eps) {
+    for (k = 1; k < iter_max; ++k) {
+      oracle_apply(op, d, z);
+      ++matvecs;
+      const double t1 = dot(n, r, r);
+      double t2 = dot(n, d, z);
+      const double alpha = t1 / t2;
+      oracle_daxpy(n, alpha, d, x);
+      oracle_daxpy(n, -alpha, z, r);
+      t2 = dot(n, r, r);
+      const double beta = t2 / t1;
+      rel = sqrt(t2) / b_norm;
+      if (rel < eps) {
+        converged = 1;
+        break;
+      }
+      set_zero(n, z);
+      oracle_daxpy(n, beta, d, z);
+      oracle_daxpy(n, 1.0, r, z);
+      set_zero(n, d);
+      oracle_daxpy(n, 1.0, z, d);
+      if (ckpt_every > 0 && k % ckpt_every == 0) {
+        save_named(dir, "x.dat", x, n);
+        save_named(dir, "r.dat", r, n);
+        save_named(dir, "d.dat", d, n);
+        save_named(dir, "z.dat", z, n);
+      }
+    }
+  } else {
+    converged = 1;
+  }
+  if (info) {
+    info->iterations = k;
+    info->converged = converged;
+    info->final_rel = rel;
+    info->matvecs = matvecs;
+  }
+  free(r);
+  free(d);
+  free(z);
+}
+
+/* src/mpla.cpp:1003-1183 */
+void oracle_bicgstab(const oracle_op* op, const double* b, double* x, int iter_max, double eps,
+                     int ckpt_every, int restart, const char* dir, oracle_solve_info* info) {
+  const int64_t n = op->n;
+  double* r = calloc((size_t)n, sizeof(double));
+  double* r_hat = calloc((size_t)n, sizeof(double));
+  double* p = calloc((size_t)n, sizeof(double));
+  double* z = calloc((size_t)n, sizeof(double));
+  double* zz = calloc((size_t)n, sizeof(double));
+  double* s = calloc((size_t)n, sizeof(double));
+  int matvecs = 0;
+  const double b_norm = sqrt(dot(n, b, b));
+  oracle_apply(op, x, z);
+  ++matvecs;
+  set_zero(n, r);
+  oracle_daxpy(n, 1.0, b, r);
+  oracle_daxpy(n, -1.0, z, r);
+  set_zero(n, r_hat);
+  oracle_daxpy(n, 1.0, r, r_hat);
+  set_zero(n, p);
+  oracle_daxpy(n, 1.0, r, p);
+  const double res = dot(n, r, r);
+  double rel = sqrt(res) / b_norm;
+  if (restart) {
+    load_named(dir, "x.dat", x, n);
+    load_named(dir, "r.dat", r, n);
+    load_named(dir, "r_hat.dat", r_hat, n);
+    load_named(dir, "z.dat", z, n);
+    load_named(dir, "zz.dat", zz, n);
+    load_named(dir, "s.dat", s, n);
+    load_named(dir, "p.dat", p, n);
+  }
+  int k = 0, converged = 0;
+  if (rel >= eps) {
+    for (k = 1; k < iter_max; ++k) {
+      double tmp1 = dot(n, r_hat, r);
+      oracle_apply(op, p, z);
+      ++matvecs;
+      double tmp2 = dot(n, z, r_hat);
+      const double alpha = tmp1 / tmp2;
+      const double r_i_r_hat = tmp1;
+      set_zero(n, s);
+      oracle_daxpy(n, 1.0, r, s);
+      oracle_daxpy(n, -alpha, z, s);
+      oracle_apply(op, s, zz);
+      ++matvecs;
+      tmp1 = dot(n, zz, s);
+      tmp2 = dot(n, zz, zz);
+      const double omega = tmp1 / tmp2;
+      oracle_daxpy(n, alpha, p, x);
+      oracle_daxpy(n, omega, s, x);
+      set_zero(n, r);
+      oracle_daxpy(n, 1.0, s, r);
+      oracle_daxpy(n, -omega, zz, r);
+      tmp1 = dot(n, r, r_hat);
+      const double beta = (alpha / omega) * (tmp1 / r_i_r_hat);
+      set_zero(n, zz);
+      oracle_daxpy(n, 1.0, p, zz);
+      oracle_daxpy(n, -omega, z, zz);
+      set_zero(n, p);
+      oracle_daxpy(n, 1.0, r, p);
+      oracle_daxpy(n, beta, zz, p);
+      oracle_apply(op, x, z);
+      ++matvecs;
+      set_zero(n, zz);
+      oracle_daxpy(n, 1.0, b, zz);
+      oracle_daxpy(n, -1.0, z, zz);
+      rel = sqrt(dot(n, zz, zz)) / b_norm;
+      if (rel < eps) {
+        converged = 1;
+        break;
+      }
+      if (ckpt_every > 0 && k % ckpt_every == 0) {
+        save_named(dir, "x.dat", x, n);
+        save_named(dir, "r.dat", r, n);
+        save_named(dir, "r_hat.dat", r_hat, n);
+        save_named(dir, "z.dat", z, n);
+        save_named(dir, "zz.dat", zz, n);
+        save_named(dir, "s.dat", s, n);
+        save_named(dir, "p.dat", p, n);
+      }
+    }
+  } else {
+    converged = 1;
+  }
+  if (info) {
+    info->iterations = k;
+    info->converged = converged;
+    info->final_rel = rel;
+    info->matvecs = matvecs;
+  }
+  free(r);
+  free(r_hat);
+  free(p);
+  free(z);
+  free(zz);
+  free(s);
+}
+
+/* ------------------------------------------------------------------------ */
+/* src/mpla.cpp:605-652 (save: int32 count then raw doubles, native endian)
+ * and :686-713 (load: exit(1) on a length mismatch). */
+
+int oracle_save_vector(const char* path, const double* x, int32_t n) {
+  FILE* f = fopen(path, "wb");
+  if (!f) return -1;
+  int ok = fwrite(&n, sizeof(int32_t), 1, f) == 1 &&
+           fwrite(x, sizeof(double), (size_t)n, f) == (size_t)n;
+  fclose(f);
+  return ok ? 0 : -1;
+}
+
+int oracle_load_vector(const char* path, double* x, int32_t n) {
+  FILE* f = fopen(path, "rb");
+  if (!f) return -1;
+  int32_t m = 0;
+  if (fread(&m, sizeof(int32_t), 1, f) != 1) {
+    fclose(f);
+    return -1;
+  }
+  if (m != n) {
+    fclose(f);
+    return -2;
+  }
+  size_t got = fread(x, sizeof(double), (size_t)n, f);
+  fclose(f);
+  return got == (size_t)n ? 0 : -1;
+}
+
+int oracle_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
