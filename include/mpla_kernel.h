@@ -1,0 +1,55 @@
+/*
+ * mpla_kernel.h -- kernel-interpolation systems beyond the reference plugin:
+ * length scale, Matern-3/2, nugget, FP32 entry evaluation and the
+ * mixed-precision solver.  No reference counterpart (the reference's
+ * Gaussian has no length scale and Matern is unused, src/kernel.cu:7-15).
+ */
+#ifndef mpla_kernel_ext_h__
+#define mpla_kernel_ext_h__
+
+#include "mpla.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPLA_KERNEL_GAUSSIAN 0 /* exp(-r^2 / (2 l^2)) */
+#define MPLA_KERNEL_MATERN32 1 /* (1 + s) exp(-s), s = sqrt(3) r / l */
+
+#define MPLA_PREC_F64 0
+#define MPLA_PREC_F32 1
+#define MPLA_PREC_BF16 2
+#define MPLA_PREC_BF16X3 3
+
+/* Payload of mpla_dgemv_core_kernel_fused (A->data, host struct).  The first
+ * three fields match struct kernel_matrix_data. */
+struct mpla_kernel_matrix_data
+{
+	double** points;             /* device array of dim device pointers (SoA) */
+	int dim;
+	int max_row_count_per_dgemv; /* ignored */
+	int kernel;                  /* MPLA_KERNEL_* */
+	int precision;               /* entry arithmetic of the core: MPLA_PREC_F64 or _F32 */
+	double length_scale;
+	double nugget;               /* added to K_ii (global row == global column) */
+	double coincident_shift;     /* added where the distance is < 1e-14 (0: never) */
+};
+
+/* b = K x with entries on the fly (K + nugget I, in the payload's precision). */
+void mpla_dgemv_core_kernel_fused(struct mpla_vector* b, struct mpla_generic_matrix* A, struct mpla_vector* x_redist, struct mpla_instance* instance);
+
+/* Solve (K + nugget I) x = b to ||b - A x||/||b|| < epsilon, with the
+ * residual, norms, dot products and updates in FP64 and the inner CG's
+ * operator in `inner_precision` (MPLA_PREC_F32; MPLA_PREC_F64 runs plain CG).
+ * Iterative refinement: r = b - A64 x (FP64 operator), inner CG on A_low e = r
+ * to a relative tolerance, x += e, until the true FP64 residual meets
+ * epsilon.  iter_max bounds the total inner iterations.  On exit
+ * instance->last_rel_res is the true FP64 relative residual of x and
+ * instance->last_iterations the total inner iterations. */
+void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_generic_matrix* A, struct mpla_vector* x, int iter_max, double epsilon, int inner_precision, struct mpla_instance* instance);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

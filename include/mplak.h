@@ -191,6 +191,64 @@ int mplak_cg_iterate(int64_t n, double* x, double* r, double* d, const double* z
 int mplak_cg_refresh_rr(int64_t n, const double* r, mplak_cg_state* state, void* work,
                         size_t work_bytes, mplak_stream_t stream);
 
+
+/* Copy one state record (device to device); used to return a replayed chunk
+ * with an odd number of iterations to buffer 0. */
+int mplak_cg_copy_state(const mplak_cg_state* src, mplak_cg_state* dst, mplak_stream_t stream);
+
+/* ---------------------------------------------------------------------------
+ * Fused BiCGSTAB steps (src/mpla.cpp:1003-1183), device-resident scalars,
+ * double-buffered state as for CG.  One iteration:
+ *     z  = A p ;  mplak_bicgstab_step_s     alpha = rho/<z,r_hat>, s = r - alpha z
+ *     zz = A s ;  mplak_bicgstab_step_x     omega = <zz,s>/<zz,zz>,
+ *                                           x += alpha p + omega s, r = s - omega zz,
+ *                                           rho' = <r,r_hat>, beta, p = r + beta (p - omega z)
+ *     z  = A x ;  mplak_bicgstab_step_true  zz = b - z, stopping test on ||zz||/||b||
+ * With true_residual = 0 the third product is skipped and step_x tests the
+ * recursive residual ||r||/||b|| and writes the state itself.
+ * ------------------------------------------------------------------------ */
+
+typedef struct mplak_bicgstab_state {
+  double rho;    /* <r, r_hat> of the current residual */
+  double bnorm2; /* <b,b> */
+  double rel;    /* stopping quantity after the last iteration */
+  double eps;
+  double alpha, omega, beta; /* last coefficients (diagnostic) */
+  int32_t k;
+  int32_t done;
+  int32_t iter_max;
+  int32_t true_residual; /* 1: stop on ||b - A x||/||b|| (reference), 0: on ||r||/||b|| */
+} mplak_bicgstab_state;
+
+size_t mplak_bicgstab_workspace_bytes(int64_t n);
+
+/* r = b - z (z = A x0), r_hat = r, p = r, rho = <r,r>, bnorm2, stopping test. */
+int mplak_bicgstab_start(int64_t n, const double* b, const double* z, double* r, double* r_hat,
+                         double* p, mplak_bicgstab_state* state, double eps, int32_t iter_max,
+                         int32_t true_residual, void* work, size_t work_bytes,
+                         mplak_stream_t stream);
+
+int mplak_bicgstab_step_s(int64_t n, const double* r, const double* z, double* s,
+                          const double* r_hat, const mplak_bicgstab_state* in, void* work,
+                          size_t work_bytes, mplak_stream_t stream);
+
+int mplak_bicgstab_step_x(int64_t n, double* x, double* r, const double* r_hat, double* p,
+                          const double* s, const double* z, const double* zz,
+                          const mplak_bicgstab_state* in, mplak_bicgstab_state* out, void* work,
+                          size_t work_bytes, mplak_stream_t stream);
+
+int mplak_bicgstab_step_true(int64_t n, const double* b, const double* z, double* zz,
+                             const mplak_bicgstab_state* in, mplak_bicgstab_state* out,
+                             void* work, size_t work_bytes, mplak_stream_t stream);
+
+/* rho = <r, r_hat> into *state (after loading checkpointed vectors). */
+int mplak_bicgstab_refresh_rho(int64_t n, const double* r, const double* r_hat,
+                               mplak_bicgstab_state* state, void* work, size_t work_bytes,
+                               mplak_stream_t stream);
+
+int mplak_bicgstab_copy_state(const mplak_bicgstab_state* src, mplak_bicgstab_state* dst,
+                              mplak_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -66,6 +66,22 @@ class mplak_cg_state(C.Structure):
     ]
 
 
+class mplak_bicgstab_state(C.Structure):
+    _fields_ = [
+        ("rho", C.c_double),
+        ("bnorm2", C.c_double),
+        ("rel", C.c_double),
+        ("eps", C.c_double),
+        ("alpha", C.c_double),
+        ("omega", C.c_double),
+        ("beta", C.c_double),
+        ("k", C.c_int32),
+        ("done", C.c_int32),
+        ("iter_max", C.c_int32),
+        ("true_residual", C.c_int32),
+    ]
+
+
 _i64, _u64, _i32, _dbl, _vp, _sz = C.c_int64, C.c_uint64, C.c_int32, C.c_double, C.c_void_p, C.c_size_t
 
 # name -> (restype, argtypes)
@@ -88,6 +104,14 @@ MPLAK_SIGNATURES = {
     "mplak_cg_start": (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, C.c_int, _dbl, _i32, _dbl, _vp, _sz, _vp]),
     "mplak_cg_iterate": (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
     "mplak_cg_refresh_rr": (C.c_int, [_i64, _vp, _vp, _vp, _sz, _vp]),
+    "mplak_cg_copy_state": (C.c_int, [_vp, _vp, _vp]),
+    "mplak_bicgstab_workspace_bytes": (_sz, [_i64]),
+    "mplak_bicgstab_start": (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _dbl, _i32, _i32, _vp, _sz, _vp]),
+    "mplak_bicgstab_step_s": (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
+    "mplak_bicgstab_step_x": (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
+    "mplak_bicgstab_step_true": (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
+    "mplak_bicgstab_refresh_rho": (C.c_int, [_i64, _vp, _vp, _vp, _vp, _sz, _vp]),
+    "mplak_bicgstab_copy_state": (C.c_int, [_vp, _vp, _vp]),
 }
 
 _libs: dict[str, C.CDLL] = {}

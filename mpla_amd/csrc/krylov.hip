@@ -144,7 +144,9 @@ __global__ __launch_bounds__(kBlock) void cg_direct_kernel(int64_t n, const doub
   const double rel = stop_value(s_in.rule, rr_new, s_in.bnorm2, s_in.n_global);
   const bool conv = rel < s_in.eps;
   const bool limit = k >= s_in.iter_max - 1;
-  if (!conv && !limit) {
+  // The reference breaks before the direction update only on convergence;
+  // on its last iteration it still updates d (and may checkpoint it).
+  if (!conv) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
       d[i] = fma(beta, d[i], r[i]);
@@ -159,6 +161,260 @@ __global__ __launch_bounds__(kBlock) void cg_direct_kernel(int64_t n, const doub
     s.done = conv ? MPLAK_CONVERGED : (limit ? MPLAK_ITER_LIMIT : MPLAK_RUNNING);
     *out = s;
   }
+}
+
+// ---------------------------------------------------------------------------
+// BiCGSTAB (src/mpla.cpp:1003-1183).  Per iteration, with the caller's
+// operator applications in between:
+//   z  = A p ; bicg_alpha : alpha = rho/<z,r_hat>, s = r - alpha z
+//   zz = A s ; bicg_omega : partials <zz,s>, <zz,zz>
+//              bicg_update: omega, x += alpha p + omega s, r = s - omega zz,
+//                           partial <r,r_hat> (and <r,r> for the cheap rule)
+//              bicg_direct: beta, p = r + beta (p - omega z)
+//                           [recursive rule: stopping test + state out]
+//   z  = A x ; bicg_true  : zz = b - z, partial <zz,zz>
+//              bicg_final : true relative residual, stopping test, state out
+// Partial-sum slots: 0 <z,r_hat>, 1 <zz,s>, 2 <zz,zz>, 3 <r,r_hat>,
+// 4 <r,r> or <b-Ax,b-Ax>; each slot holds one value per block.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ void bicg_state_out(const mplak_bicgstab_state& s_in, double rho_new,
+                                               double alpha, double omega, double beta, double rel,
+                                               mplak_bicgstab_state* out) {
+  mplak_bicgstab_state s = s_in;
+  s.k = s_in.k + 1;
+  s.rho = rho_new;
+  s.alpha = alpha;
+  s.omega = omega;
+  s.beta = beta;
+  s.rel = rel;
+  const bool conv = rel < s_in.eps;
+  const bool limit = s.k >= s_in.iter_max - 1;
+  s.done = conv ? MPLAK_CONVERGED : (limit ? MPLAK_ITER_LIMIT : MPLAK_RUNNING);
+  *out = s;
+}
+
+__global__ __launch_bounds__(kBlock) void bicg_start_kernel(int64_t n, const double* __restrict__ b,
+                                                            const double* __restrict__ z,
+                                                            double* __restrict__ r,
+                                                            double* __restrict__ r_hat,
+                                                            double* __restrict__ p,
+                                                            double* __restrict__ part) {
+  __shared__ double scratch[kBlock / kWave];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  double rr = 0.0, bb = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double bi = b[i];
+    const double ri = bi - z[i];
+    r[i] = ri;
+    r_hat[i] = ri;
+    p[i] = ri;
+    rr = fma(ri, ri, rr);
+    bb = fma(bi, bi, bb);
+  }
+  rr = block_sum<kBlock>(rr, scratch);
+  bb = block_sum<kBlock>(bb, scratch);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = rr;
+    part[gridDim.x + blockIdx.x] = bb;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void bicg_start_final_kernel(const double* __restrict__ part,
+                                                                  int g, mplak_bicgstab_state* st,
+                                                                  double eps, int iter_max,
+                                                                  int true_residual) {
+  __shared__ double scratch[kBlock / kWave];
+  const double rr = reduce_partials<kBlock>(part, g, scratch);
+  const double bb = reduce_partials<kBlock>(part + g, g, scratch);
+  if (threadIdx.x == 0) {
+    mplak_bicgstab_state s;
+    s.rho = rr;  // <r_hat, r> with r_hat = r
+    s.bnorm2 = bb;
+    s.rel = sqrt(rr) / sqrt(bb);
+    s.eps = eps;
+    s.alpha = s.omega = s.beta = 0.0;
+    s.k = 0;
+    s.iter_max = iter_max;
+    s.true_residual = true_residual;
+    if (!(s.rel >= eps))
+      s.done = MPLAK_CONVERGED;
+    else if (iter_max <= 1)
+      s.done = MPLAK_ITER_LIMIT;
+    else
+      s.done = MPLAK_RUNNING;
+    *st = s;
+  }
+}
+
+template <int NDOT>
+__device__ __forceinline__ void write_partials(double (&v)[NDOT], double* scratch, double* part,
+                                               const int* slots) {
+#pragma unroll
+  for (int q = 0; q < NDOT; ++q) {
+    const double s = block_sum<kBlock>(v[q], scratch);
+    if (threadIdx.x == 0) part[slots[q] * gridDim.x + blockIdx.x] = s;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void bicg_zr_kernel(int64_t n, const double* __restrict__ z,
+                                                         const double* __restrict__ r_hat,
+                                                         const mplak_bicgstab_state* __restrict__ in,
+                                                         double* __restrict__ part) {
+  if (in->done != MPLAK_RUNNING) return;
+  __shared__ double scratch[kBlock / kWave];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  double v[1] = {0.0};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    v[0] = fma(z[i], r_hat[i], v[0]);
+  const int slots[1] = {0};
+  write_partials<1>(v, scratch, part, slots);
+}
+
+__global__ __launch_bounds__(kBlock) void bicg_alpha_kernel(int64_t n, const double* __restrict__ r,
+                                                            const double* __restrict__ z,
+                                                            double* __restrict__ s,
+                                                            const mplak_bicgstab_state* __restrict__ in,
+                                                            const double* __restrict__ part) {
+  if (in->done != MPLAK_RUNNING) return;
+  __shared__ double scratch[kBlock / kWave];
+  const double alpha = in->rho / reduce_partials<kBlock>(part, gridDim.x, scratch);
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    s[i] = fma(-alpha, z[i], r[i]);
+}
+
+__global__ __launch_bounds__(kBlock) void bicg_omega_kernel(int64_t n, const double* __restrict__ zz,
+                                                            const double* __restrict__ s,
+                                                            const mplak_bicgstab_state* __restrict__ in,
+                                                            double* __restrict__ part) {
+  if (in->done != MPLAK_RUNNING) return;
+  __shared__ double scratch[kBlock / kWave];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  double v[2] = {0.0, 0.0};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double q = zz[i];
+    v[0] = fma(q, s[i], v[0]);
+    v[1] = fma(q, q, v[1]);
+  }
+  const int slots[2] = {1, 2};
+  write_partials<2>(v, scratch, part, slots);
+}
+
+__global__ __launch_bounds__(kBlock) void bicg_update_kernel(
+    int64_t n, double* __restrict__ x, double* __restrict__ r, const double* __restrict__ r_hat,
+    const double* __restrict__ p, const double* __restrict__ s, const double* __restrict__ zz,
+    const mplak_bicgstab_state* __restrict__ in, double* __restrict__ part) {
+  if (in->done != MPLAK_RUNNING) return;
+  __shared__ double scratch[kBlock / kWave];
+  const int g = gridDim.x;
+  const double alpha = in->rho / reduce_partials<kBlock>(part, g, scratch);
+  const double omega =
+      reduce_partials<kBlock>(part + g, g, scratch) / reduce_partials<kBlock>(part + 2 * g, g, scratch);
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  double v[2] = {0.0, 0.0};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double si = s[i];
+    x[i] = fma(omega, si, fma(alpha, p[i], x[i]));
+    const double ri = fma(-omega, zz[i], si);
+    r[i] = ri;
+    v[0] = fma(ri, r_hat[i], v[0]);
+    v[1] = fma(ri, ri, v[1]);
+  }
+  const int slots[2] = {3, 4};
+  write_partials<2>(v, scratch, part, slots);
+}
+
+__global__ __launch_bounds__(kBlock) void bicg_direct_kernel(
+    int64_t n, const double* __restrict__ r, double* __restrict__ p, const double* __restrict__ z,
+    const mplak_bicgstab_state* __restrict__ in, mplak_bicgstab_state* __restrict__ out,
+    const double* __restrict__ part) {
+  const mplak_bicgstab_state s_in = *in;
+  if (s_in.done != MPLAK_RUNNING) {
+    if (!s_in.true_residual && blockIdx.x == 0 && threadIdx.x == 0) *out = s_in;
+    return;
+  }
+  __shared__ double scratch[kBlock / kWave];
+  const int g = gridDim.x;
+  const double alpha = s_in.rho / reduce_partials<kBlock>(part, g, scratch);
+  const double omega =
+      reduce_partials<kBlock>(part + g, g, scratch) / reduce_partials<kBlock>(part + 2 * g, g, scratch);
+  const double rho_new = reduce_partials<kBlock>(part + 3 * g, g, scratch);
+  const double beta = (alpha / omega) * (rho_new / s_in.rho);
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    p[i] = fma(beta, fma(-omega, z[i], p[i]), r[i]);
+  if (!s_in.true_residual) {
+    const double rr = reduce_partials<kBlock>(part + 4 * g, g, scratch);
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+      bicg_state_out(s_in, rho_new, alpha, omega, beta, sqrt(rr) / sqrt(s_in.bnorm2), out);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void bicg_true_kernel(int64_t n, const double* __restrict__ b,
+                                                           const double* __restrict__ z,
+                                                           double* __restrict__ zz,
+                                                           const mplak_bicgstab_state* __restrict__ in,
+                                                           double* __restrict__ part) {
+  if (in->done != MPLAK_RUNNING) return;
+  __shared__ double scratch[kBlock / kWave];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  double v[1] = {0.0};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double q = b[i] - z[i];
+    zz[i] = q;
+    v[0] = fma(q, q, v[0]);
+  }
+  const int slots[1] = {4};
+  write_partials<1>(v, scratch, part, slots);
+}
+
+__global__ __launch_bounds__(kBlock) void bicg_final_kernel(int g,
+                                                            const mplak_bicgstab_state* __restrict__ in,
+                                                            mplak_bicgstab_state* __restrict__ out,
+                                                            const double* __restrict__ part) {
+  const mplak_bicgstab_state s_in = *in;
+  if (s_in.done != MPLAK_RUNNING) {
+    if (threadIdx.x == 0) *out = s_in;
+    return;
+  }
+  __shared__ double scratch[kBlock / kWave];
+  const double alpha = s_in.rho / reduce_partials<kBlock>(part, g, scratch);
+  const double omega =
+      reduce_partials<kBlock>(part + g, g, scratch) / reduce_partials<kBlock>(part + 2 * g, g, scratch);
+  const double rho_new = reduce_partials<kBlock>(part + 3 * g, g, scratch);
+  const double beta = (alpha / omega) * (rho_new / s_in.rho);
+  const double tt = reduce_partials<kBlock>(part + 4 * g, g, scratch);
+  if (threadIdx.x == 0)
+    bicg_state_out(s_in, rho_new, alpha, omega, beta, sqrt(tt) / sqrt(s_in.bnorm2), out);
+}
+
+__global__ __launch_bounds__(kBlock) void dot2_partial_kernel(int64_t n, const double* __restrict__ a,
+                                                              const double* __restrict__ b,
+                                                              double* __restrict__ part) {
+  __shared__ double scratch[kBlock / kWave];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    s = fma(a[i], b[i], s);
+  s = block_sum<kBlock>(s, scratch);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kBlock) void refresh_rho_final_kernel(const double* __restrict__ part,
+                                                                   int g, mplak_bicgstab_state* st) {
+  __shared__ double scratch[kBlock / kWave];
+  const double v = reduce_partials<kBlock>(part, g, scratch);
+  if (threadIdx.x == 0) st->rho = v;
+}
+
+__global__ void copy_state_kernel(const mplak_cg_state* __restrict__ src, mplak_cg_state* __restrict__ dst) {
+  if (threadIdx.x == 0) *dst = *src;
+}
+
+__global__ void copy_bicg_state_kernel(const mplak_bicgstab_state* __restrict__ src,
+                                       mplak_bicgstab_state* __restrict__ dst) {
+  if (threadIdx.x == 0) *dst = *src;
 }
 
 __global__ __launch_bounds__(kBlock) void dot_into_kernel(int64_t n, const double* __restrict__ a,
@@ -236,6 +492,109 @@ int mplak_cg_refresh_rr(int64_t n, const double* r, mplak_cg_state* state, void*
   int st = launch_status();
   if (st != MPLAK_OK) return st;
   hipLaunchKernelGGL(refresh_rr_final_kernel, dim3(1), dim3(kBlock), 0, hs, part, g, state);
+  return launch_status();
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int mplak_cg_copy_state(const mplak_cg_state* src, mplak_cg_state* dst, mplak_stream_t s) {
+  if (!src || !dst) return MPLAK_ERR_ARG;
+  hipLaunchKernelGGL(copy_state_kernel, dim3(1), dim3(kWave), 0, as_stream(s), src, dst);
+  return launch_status();
+}
+
+size_t mplak_bicgstab_workspace_bytes(int64_t n) {
+  return (size_t)5 * vec_blocks(n > 0 ? n : 1) * sizeof(double);
+}
+
+int mplak_bicgstab_start(int64_t n, const double* b, const double* z, double* r, double* r_hat,
+                         double* p, mplak_bicgstab_state* state, double eps, int32_t iter_max,
+                         int32_t true_residual, void* work, size_t work_bytes, mplak_stream_t s) {
+  if (n <= 0 || !b || !z || !r || !r_hat || !p || !state) return MPLAK_ERR_ARG;
+  if (work == nullptr || work_bytes < mplak_bicgstab_workspace_bytes(n)) return MPLAK_ERR_WORKSPACE;
+  const int g = vec_blocks(n);
+  double* part = static_cast<double*>(work);
+  hipStream_t hs = as_stream(s);
+  hipLaunchKernelGGL(bicg_start_kernel, dim3(g), dim3(kBlock), 0, hs, n, b, z, r, r_hat, p, part);
+  int st = launch_status();
+  if (st != MPLAK_OK) return st;
+  hipLaunchKernelGGL(bicg_start_final_kernel, dim3(1), dim3(kBlock), 0, hs, part, g, state, eps,
+                     (int)iter_max, (int)(true_residual != 0));
+  return launch_status();
+}
+
+int mplak_bicgstab_step_s(int64_t n, const double* r, const double* z, double* s_vec,
+                          const double* r_hat, const mplak_bicgstab_state* in, void* work,
+                          size_t work_bytes, mplak_stream_t s) {
+  if (n <= 0 || !r || !z || !s_vec || !r_hat || !in) return MPLAK_ERR_ARG;
+  if (work == nullptr || work_bytes < mplak_bicgstab_workspace_bytes(n)) return MPLAK_ERR_WORKSPACE;
+  const int g = vec_blocks(n);
+  double* part = static_cast<double*>(work);
+  hipStream_t hs = as_stream(s);
+  hipLaunchKernelGGL(bicg_zr_kernel, dim3(g), dim3(kBlock), 0, hs, n, z, r_hat, in, part);
+  int st = launch_status();
+  if (st != MPLAK_OK) return st;
+  hipLaunchKernelGGL(bicg_alpha_kernel, dim3(g), dim3(kBlock), 0, hs, n, r, z, s_vec, in, part);
+  return launch_status();
+}
+
+int mplak_bicgstab_step_x(int64_t n, double* x, double* r, const double* r_hat, double* p,
+                          const double* s_vec, const double* z, const double* zz,
+                          const mplak_bicgstab_state* in, mplak_bicgstab_state* out, void* work,
+                          size_t work_bytes, mplak_stream_t s) {
+  if (n <= 0 || !x || !r || !r_hat || !p || !s_vec || !z || !zz || !in || !out || in == out)
+    return MPLAK_ERR_ARG;
+  if (work == nullptr || work_bytes < mplak_bicgstab_workspace_bytes(n)) return MPLAK_ERR_WORKSPACE;
+  const int g = vec_blocks(n);
+  double* part = static_cast<double*>(work);
+  hipStream_t hs = as_stream(s);
+  hipLaunchKernelGGL(bicg_omega_kernel, dim3(g), dim3(kBlock), 0, hs, n, zz, s_vec, in, part);
+  int st = launch_status();
+  if (st != MPLAK_OK) return st;
+  hipLaunchKernelGGL(bicg_update_kernel, dim3(g), dim3(kBlock), 0, hs, n, x, r, r_hat, p, s_vec, zz, in,
+                     part);
+  st = launch_status();
+  if (st != MPLAK_OK) return st;
+  hipLaunchKernelGGL(bicg_direct_kernel, dim3(g), dim3(kBlock), 0, hs, n, r, p, z, in, out, part);
+  return launch_status();
+}
+
+int mplak_bicgstab_step_true(int64_t n, const double* b, const double* z, double* zz,
+                             const mplak_bicgstab_state* in, mplak_bicgstab_state* out, void* work,
+                             size_t work_bytes, mplak_stream_t s) {
+  if (n <= 0 || !b || !z || !zz || !in || !out || in == out) return MPLAK_ERR_ARG;
+  if (work == nullptr || work_bytes < mplak_bicgstab_workspace_bytes(n)) return MPLAK_ERR_WORKSPACE;
+  const int g = vec_blocks(n);
+  double* part = static_cast<double*>(work);
+  hipStream_t hs = as_stream(s);
+  hipLaunchKernelGGL(bicg_true_kernel, dim3(g), dim3(kBlock), 0, hs, n, b, z, zz, in, part);
+  int st = launch_status();
+  if (st != MPLAK_OK) return st;
+  hipLaunchKernelGGL(bicg_final_kernel, dim3(1), dim3(kBlock), 0, hs, g, in, out, part);
+  return launch_status();
+}
+
+int mplak_bicgstab_refresh_rho(int64_t n, const double* r, const double* r_hat,
+                               mplak_bicgstab_state* state, void* work, size_t work_bytes,
+                               mplak_stream_t s) {
+  if (n <= 0 || !r || !r_hat || !state) return MPLAK_ERR_ARG;
+  if (work == nullptr || work_bytes < mplak_bicgstab_workspace_bytes(n)) return MPLAK_ERR_WORKSPACE;
+  const int g = vec_blocks(n);
+  double* part = static_cast<double*>(work);
+  hipStream_t hs = as_stream(s);
+  hipLaunchKernelGGL(dot2_partial_kernel, dim3(g), dim3(kBlock), 0, hs, n, r, r_hat, part);
+  int st = launch_status();
+  if (st != MPLAK_OK) return st;
+  hipLaunchKernelGGL(refresh_rho_final_kernel, dim3(1), dim3(kBlock), 0, hs, part, g, state);
+  return launch_status();
+}
+
+int mplak_bicgstab_copy_state(const mplak_bicgstab_state* src, mplak_bicgstab_state* dst,
+                              mplak_stream_t s) {
+  if (!src || !dst) return MPLAK_ERR_ARG;
+  hipLaunchKernelGGL(copy_bicg_state_kernel, dim3(1), dim3(kWave), 0, as_stream(s), src, dst);
   return launch_status();
 }
 

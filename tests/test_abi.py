@@ -12,9 +12,15 @@ INCLUDE = os.path.join(ROOT, "include")
 # header -> library that must export its functions
 HEADERS = {
     "mplak.h": "libmplak.so",
+    "mpla.h": "libmpla.so",
+    "kernel.h": "libmpla.so",
+    "mpla_kernel.h": "libmpla.so",
+    "generic_system_adapter.h": "libmpla.so",
+    "kernel_system_assembler.h": "libmpla.so",
 }
 
-DECL = re.compile(r"^\s*(?:extern\s+)?[A-Za-z_][\w\s\*]*?[\s\*]((?:mplak|mpla)_\w+)\s*\(", re.M)
+DECL = re.compile(r"^\s*(?:extern\s+)?[A-Za-z_][\w\s\*]*?[\s\*]((?:mplak|mpla)_\w+|info|checkCUDAError|"
+                  r"(?:create|destroy)_gaussian_kernel_system_assembler_object)\s*\(", re.M)
 
 
 def declared(header):
@@ -37,7 +43,7 @@ def test_library_exports_header(header, libname):
     assert os.path.exists(path), f"{path} not built"
     lib = C.CDLL(path)
     names = declared(header)
-    assert len(names) > 5
+    assert len(names) >= 2
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, f"{libname} lacks {missing}"
 
@@ -55,3 +61,63 @@ def test_status_strings_without_gpu():
     assert lib.mplak_version() >= 100
     # argument validation happens before any device work
     assert lib.mplak_daxpy_f64(-1, 1.0, None, None, None) == 1
+
+
+def test_python_bindings_cover_headers():
+    from mpla_amd import mpla
+    names = set()
+    for h in ("mpla.h", "kernel.h", "mpla_kernel.h", "generic_system_adapter.h", "kernel_system_assembler.h"):
+        names |= set(declared(h))
+    assert sorted(names) == sorted(mpla.MPLA_SIGNATURES)
+
+
+STRUCTS = {
+    "mpla_instance": ["comm_storage", "proc_count", "proc_rows", "proc_cols", "cur_proc_coord", "cur_proc_row",
+                      "cur_proc_col", "cur_proc_rank", "is_parent", "stream", "runtime", "skip_flag",
+                      "last_iterations", "last_converged", "last_rel_res", "last_matvecs", "verbosity"],
+    "mpla_matrix": None,
+    "mpla_generic_matrix": None,
+    "mpla_vector": None,
+    "kernel_matrix_data": None,
+    "mpla_kernel_matrix_data": None,
+}
+
+
+def test_struct_layout_matches_headers(tmp_path):
+    """Compile the C headers with gcc and compare sizeof/offsetof with ctypes."""
+    import subprocess
+    from mpla_amd import mpla
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "mpla.h"', '#include "kernel.h"',
+             '#include "mpla_kernel.h"', "int main(void) {"]
+    for sname in STRUCTS:
+        cls = getattr(mpla, sname)
+        lines.append(f'printf("{sname} size %zu\\n", sizeof(struct {sname}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'printf("{sname}.{fname} %zu\\n", offsetof(struct {sname}, {fname}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", INCLUDE, str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = dict(line.rsplit(" ", 1) for line in out if line)
+    for sname in STRUCTS:
+        cls = getattr(mpla, sname)
+        assert int(got[f"{sname} size"]) == C.sizeof(cls), sname
+        for fname, _ in cls._fields_:
+            assert int(got[f"{sname}.{fname}"]) == getattr(cls, fname).offset, (sname, fname)
+
+
+def test_block_distribution_host_only():
+    """The product's distribution arithmetic (no GPU needed) against hand cases
+    of src/mpla.cpp:157-209."""
+    from mpla_amd import mpla
+    lib = mpla.lib()
+    for M, P, counts in [(10, 3, [4, 3, 3]), (131072, 4, [32768] * 4), (7, 7, [1] * 7), (65537, 2, [32769, 32768])]:
+        c = (C.c_int * P)()
+        o = (C.c_int * P)()
+        assert lib.mpla_block_distribution(M, P, c, o) == 0
+        assert list(c) == counts
+        assert list(o) == [sum(counts[:p]) for p in range(P)]
+    c = (C.c_int * 3)()
+    assert lib.mpla_block_distribution(2, 3, c, c) == -1
