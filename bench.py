@@ -1,0 +1,289 @@
+#!/usr/bin/env python3
+"""Benchmark of the solver hot path on one MI355X.
+
+Workload (the headline metric's configuration): matrix-free Gaussian kernel
+system (K + 1.0 I) x = b, N = 65536 points uniform in [0,1]^3, length scale
+0.05, b ~ U[-1,1], x0 = 0, CG to relative residual 1e-8, entries never stored.
+A step is one complete solve through the drop-in API
+(mpla_kernel_conjugate_gradient_mixed: FP32 entry evaluation inside FP64
+iterative refinement); the plain FP64 solve (the reference algorithm) is timed
+beside it.  Inputs are generated on the device by the counter-based RNG and
+stay resident in HBM; x is reset to 0 inside each timed step (as the
+reference's benchmark_kernel_cg does).
+
+Also reported: p50/p99 solve latency, the dominant kernel's roofline (its
+average launch time measured with HIP events on the library's stream), dense
+dgemv HBM bandwidth at N = 65536 (a 34 GB matrix), and the CPU oracle's FP64
+rate on a bounded sample, timed on this host's cores.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N = 65536
+DIM = 3
+ELL = 0.05
+NUGGET = 1.0
+EPS = 1e-8
+SEED = 1235
+ITER_MAX = 20001
+FLOP_PER_ENTRY = 3 * DIM + 4  # d subs, d FMA, 1 scale, 1 exp, 1 FMA accumulate
+PEAK_FP32 = 157.3e12
+PEAK_FP64 = 78.6e12
+PEAK_HBM = 8.0e12
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def percentile(xs, q):
+    xs = sorted(xs)
+    if not xs:
+        return float("nan")
+    k = (len(xs) - 1) * q / 100.0
+    lo, hi = math.floor(k), math.ceil(k)
+    return xs[lo] + (xs[hi] - xs[lo]) * (k - lo)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--cpu-matvecs", type=int, default=2)
+    ap.add_argument("--skip-dgemv", action="store_true")
+    ap.add_argument("--skip-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path runs on a 1x1 "
+              "process grid and multi-GPU runs are not part of this build.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from mpla_amd import _native, mpla
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    K = _native.mplak()
+    L = mpla.lib()
+    inst = mpla.mpla_instance()
+    L.mpla_init_instance(C.byref(inst), mpla.MPI_COMM_WORLD)
+    inst.verbosity = 0
+    stream_ptr = L.mpla_instance_stream(C.byref(inst))
+    ext = torch.cuda.ExternalStream(stream_ptr)
+
+    # ---- inputs, generated in HBM ------------------------------------------
+    P = torch.empty(DIM, N, dtype=torch.float64, device="cuda")
+    _native.check(K.mplak_fill_uniform_f64(P.data_ptr(), DIM * N, SEED, 0, 0, 0.0, 1.0, stream_ptr))
+    table = torch.tensor([P[d].data_ptr() for d in range(DIM)], dtype=torch.int64, device="cuda")
+    b = mpla.mpla_vector()
+    x = mpla.mpla_vector()
+    L.mpla_init_vector(C.byref(b), C.byref(inst), N)
+    L.mpla_init_vector(C.byref(x), C.byref(inst), N)
+    _native.check(K.mplak_fill_uniform_f64(b.data, N, SEED, 1, 0, -1.0, 1.0, stream_ptr))
+    torch.cuda.synchronize()
+    kd = mpla.mpla_kernel_matrix_data(table.data_ptr(), DIM, 0, mpla.MPLA_KERNEL_GAUSSIAN, mpla.MPLA_PREC_F64,
+                                      ELL, NUGGET, 0.0)
+    A = mpla.mpla_generic_matrix()
+    L.mpla_init_generic_matrix(C.byref(A), C.byref(inst), N, N)
+    A.data = C.addressof(kd)
+
+    def reset_x():
+        _native.check(K.mplak_fill_f64(x.data, N, 0.0, stream_ptr))
+
+    def solve_mixed():
+        reset_x()
+        L.mpla_kernel_conjugate_gradient_mixed(C.byref(b), C.byref(A), C.byref(x), ITER_MAX, EPS,
+                                               mpla.MPLA_PREC_F32, C.byref(inst))
+        return inst.last_iterations, inst.last_rel_res, inst.last_converged, inst.last_matvecs
+
+    def solve_plain():
+        reset_x()
+        L.mpla_generic_conjugate_gradient_with_checkpoint_restart(
+            C.byref(b), C.byref(A), C.byref(x), ITER_MAX, EPS, mpla.core("mpla_dgemv_core_kernel_fused"), 0, False,
+            C.byref(inst))
+        return inst.last_iterations, inst.last_rel_res, inst.last_converged, inst.last_matvecs
+
+    def read_x():
+        out = np.empty(N)
+        L.mpla_copy_distributed_vector_to_cpu(out.ctypes.data, C.byref(x), C.byref(inst))
+        return out
+
+    # ---- mixed precision (the product path) ---------------------------------
+    log(f"warmup: {args.warmup} mixed solves")
+    for _ in range(args.warmup):
+        solve_mixed()
+    torch.cuda.synchronize()
+    lat = []
+    infos = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        infos.append(solve_mixed())
+        lat.append(time.perf_counter() - ts)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    x_mixed = read_x()
+    it_mixed, rel_mixed, conv_mixed, mv_mixed = infos[-1]
+    log(f"mixed: {args.steps} solves in {wall:.3f} s, inner iterations {it_mixed}, true rel residual {rel_mixed:.3e}")
+
+    # ---- plain FP64 (reference algorithm on the GPU) ------------------------
+    solve_plain()  # warm the FP64 path
+    torch.cuda.synchronize()
+    plat = []
+    for _ in range(args.plain_steps):
+        ts = time.perf_counter()
+        pinfo = solve_plain()
+        plat.append(time.perf_counter() - ts)
+    x_plain = read_x()
+    it_plain, rel_plain, conv_plain, mv_plain = pinfo
+    plain_sps = args.plain_steps / sum(plat)
+    log(f"plain FP64: {args.plain_steps} solves, {1 / plain_sps:.3f} s each, iterations {it_plain}")
+    err_vs_fp64 = float(np.linalg.norm(x_mixed - x_plain) / np.linalg.norm(x_plain))
+
+    # ---- dominant kernel: FP32 matrix-free product, HIP events on the stream --
+    desc = _native.mplak_kernel_desc()
+    desc.points, desc.dim, desc.kind = table.data_ptr(), DIM, _native.MPLAK_KERNEL_GAUSSIAN
+    desc.length_scale, desc.nugget, desc.coincident_shift = ELL, NUGGET, 0.0
+    desc.row_offset, desc.row_count, desc.col_offset, desc.col_count = 0, N, 0, N
+    wb = K.mplak_kernel_matvec_workspace_bytes(C.byref(desc))
+    work = torch.empty(wb // 8 + 1, dtype=torch.float64, device="cuda")
+    yv = torch.empty(N, dtype=torch.float64, device="cuda")
+    xv = torch.empty(N, dtype=torch.float64, device="cuda")
+    _native.check(K.mplak_fill_uniform_f64(xv.data_ptr(), N, SEED, 2, 0, -1.0, 1.0, stream_ptr))
+    torch.cuda.synchronize()
+
+    def time_kmv(prec, reps):
+        for _ in range(2):
+            _native.check(K.mplak_kernel_matvec(C.byref(desc), prec, xv.data_ptr(), yv.data_ptr(), work.data_ptr(),
+                                                wb, None, stream_ptr))
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(ext)
+        for _ in range(reps):
+            _native.check(K.mplak_kernel_matvec(C.byref(desc), prec, xv.data_ptr(), yv.data_ptr(), work.data_ptr(),
+                                                wb, None, stream_ptr))
+        e1.record(ext)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / reps * 1e-3
+
+    t32 = time_kmv(_native.MPLAK_PREC_F32, 20)
+    t64 = time_kmv(_native.MPLAK_PREC_F64, 5)
+    entries = float(N) * N
+    ach32 = FLOP_PER_ENTRY * entries / t32
+    ach64 = FLOP_PER_ENTRY * entries / t64
+    log(f"kernel matvec: FP32 {t32 * 1e3:.3f} ms ({entries / t32:.3e} entries/s), "
+        f"FP64 {t64 * 1e3:.3f} ms ({entries / t64:.3e} entries/s)")
+    del work
+
+    # ---- dense dgemv at N = 65536 (34 GB, far above the 256 MB MALL) ---------
+    dgemv = None
+    if not args.skip_dgemv:
+        n = 65536
+        Ad = torch.empty(n * n, dtype=torch.float64, device="cuda")
+        xd = torch.empty(n, dtype=torch.float64, device="cuda")
+        yd = torch.empty(n, dtype=torch.float64, device="cuda")
+        _native.check(K.mplak_fill_uniform_f64(Ad.data_ptr(), n * n, 1234, 1, 0, -1.0, 1.0, stream_ptr))
+        _native.check(K.mplak_fill_uniform_f64(xd.data_ptr(), n, 1234, 2, 0, -1.0, 1.0, stream_ptr))
+        gwb = K.mplak_dgemv_workspace_bytes(n, n)
+        gw = torch.empty(gwb // 8 + 1, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+
+        def gemv():
+            _native.check(K.mplak_dgemv_f64(n, n, Ad.data_ptr(), n, xd.data_ptr(), yd.data_ptr(), gw.data_ptr(),
+                                            gwb, None, stream_ptr))
+
+        gemv()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        reps = 20
+        e0.record(ext)
+        for _ in range(reps):
+            gemv()
+        e1.record(ext)
+        e1.synchronize()
+        tg = e0.elapsed_time(e1) / reps * 1e-3
+        gbytes = 8.0 * n * n + 16.0 * n
+        dgemv = {"n": n, "ms": tg * 1e3, "gbps": gbytes / tg / 1e9, "frac_hbm": gbytes / tg / PEAK_HBM,
+                 "algorithmic_bytes": gbytes}
+        log(f"dgemv n={n}: {tg * 1e3:.3f} ms, {gbytes / tg / 1e9:.1f} GB/s")
+        del Ad, xd, yd, gw
+        torch.cuda.empty_cache()
+
+    # ---- CPU oracle on a bounded sample --------------------------------------
+    cpu = None
+    if not args.skip_cpu:
+        from oracle import binding as oracle
+        pts = P.cpu().numpy()
+        k = oracle.Kernel(pts, oracle.GAUSSIAN, ELL, nugget=NUGGET)
+        xh = xv.cpu().numpy()
+        tc = time.perf_counter()
+        for _ in range(args.cpu_matvecs):
+            k.matvec(xh, mode=0)
+        tcpu = (time.perf_counter() - tc) / args.cpu_matvecs
+        solve_cpu = tcpu * (1 + it_plain)
+        cpu = {"value": 1.0 / solve_cpu, "unit": "solves/s", "cores": oracle.threads(), "kind": "port",
+               "sample": f"{args.cpu_matvecs} FP64 matrix-free products of the N={N} Gaussian operator "
+                         f"({tcpu:.2f} s each, OpenMP), extrapolated to the {1 + it_plain} products of one plain "
+                         f"FP64 CG solve; entries/s {entries / tcpu:.3e}"}
+        log(f"cpu oracle: {tcpu:.2f} s per product on {oracle.threads()} threads")
+
+    value = args.steps / wall
+    out = {
+        "metric": "kernel-CG solves/sec + p50/p99 latency @1e-8 rel residual; dgemv HBM GB/s",
+        "value": value,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-entries+f64-refinement",
+        "data": "synthetic (counter-based RNG, seed 1235), resident in HBM",
+        "config": {"workload": "matrix-free Gaussian kernel CG: N=65536 points U[0,1]^3, l=0.05, (K+1.0I)x=b, "
+                               "b~U[-1,1], x0=0, rel residual 1e-8; mixed FP32 entries + FP64 refinement",
+                   "n": N, "dim": DIM, "length_scale": ELL, "nugget": NUGGET, "tolerance": EPS},
+        "latency_ms": {"p50": percentile(lat, 50) * 1e3, "p99": percentile(lat, 99) * 1e3},
+        "mixed": {"inner_iterations": it_mixed, "true_rel_residual": rel_mixed, "converged": bool(conv_mixed),
+                  "products": mv_mixed, "rel_error_vs_fp64_solution": err_vs_fp64},
+        "plain_fp64": {"solves_per_s": plain_sps, "iterations": it_plain, "rel_residual": rel_plain,
+                       "converged": bool(conv_plain),
+                       "latency_ms": {"p50": percentile(plat, 50) * 1e3, "p99": percentile(plat, 99) * 1e3}},
+        "speedup_mixed_vs_plain": value / plain_sps,
+        "roofline": {"bound": "valu", "achieved": ach32 / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
+                     "frac": ach32 / PEAK_FP32, "traffic": None,
+                     "kernel": "kmv_partial_kernel<float, GAUSSIAN, 3> + finish",
+                     "launch_ms": t32 * 1e3, "entries_per_s": entries / t32,
+                     "flop_per_entry": FLOP_PER_ENTRY},
+        "roofline_fp64_kernel": {"bound": "valu", "achieved": ach64 / 1e12, "peak": PEAK_FP64 / 1e12,
+                                 "unit": "TFLOP/s", "frac": ach64 / PEAK_FP64, "launch_ms": t64 * 1e3,
+                                 "entries_per_s": entries / t64},
+        "dgemv": dgemv,
+        "cpu_baseline": cpu,
+    }
+    if dgemv is not None:
+        out["roofline_dgemv"] = {"bound": "hbm", "achieved": dgemv["gbps"], "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                                 "frac": dgemv["frac_hbm"], "traffic": None}
+    L.mpla_free_vector(C.byref(b), C.byref(inst))
+    L.mpla_free_vector(C.byref(x), C.byref(inst))
+    L.mpla_free_generic_matrix(C.byref(A), C.byref(inst))
+    L.mpla_destroy_instance(C.byref(inst))
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
