@@ -145,6 +145,27 @@ int mplak_gaussian_kernel_rhs(const double* const* points, int32_t dim, int64_t 
                               int64_t row_count, double* b, mplak_stream_t stream);
 
 /* ---------------------------------------------------------------------------
+ * Multi-right-hand-side product (no reference counterpart)
+ * ------------------------------------------------------------------------ */
+
+size_t mplak_kernel_mrhs_workspace_bytes(const mplak_kernel_desc* desc, int precision, int64_t nrhs);
+
+/* Y[:, r] = K V[:, r] for r < nrhs, V and Y column-major (leading dimensions
+ * ldv >= col_count, ldy >= row_count).  precision:
+ *   MPLAK_PREC_F64    FP64 entries and accumulation (VALU)
+ *   MPLAK_PREC_F32    FP32 entries; MFMA v_mfma_f32_32x32x2_f32 when nrhs >= 16
+ *                     and dim <= 4, VALU otherwise
+ *   MPLAK_PREC_BF16   FP32 entries rounded to BF16, V in BF16, MFMA with FP32
+ *                     accumulation (dim <= 4)
+ *   MPLAK_PREC_BF16X3 hi+lo BF16 split of entries and V, three MFMA products
+ * Each slab of sources accumulates in FP32 (FP64 for MPLAK_PREC_F64); slab
+ * partials are added in FP64 in a fixed order; the nugget term is exact
+ * FP64.  Deterministic; skip_if as for mplak_dgemv_f64. */
+int mplak_kernel_mrhs(const mplak_kernel_desc* desc, int precision, int64_t nrhs, const double* V, int64_t ldv,
+                      double* Y, int64_t ldy, void* work, size_t work_bytes, const int32_t* skip_if,
+                      mplak_stream_t stream);
+
+/* ---------------------------------------------------------------------------
  * Fused conjugate-gradient steps with device-resident scalars
  *
  * One CG iteration of src/mpla.cpp:844-879 (and :937-975) is

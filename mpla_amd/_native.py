@@ -100,6 +100,9 @@ MPLAK_SIGNATURES = {
     "mplak_kernel_matvec_workspace_bytes": (_sz, [C.POINTER(mplak_kernel_desc)]),
     "mplak_kernel_matvec": (C.c_int, [C.POINTER(mplak_kernel_desc), C.c_int, _vp, _vp, _vp, _sz, _vp, _vp]),
     "mplak_gaussian_kernel_rhs": (C.c_int, [_vp, _i32, _i64, _i64, _vp, _vp]),
+    "mplak_kernel_mrhs_workspace_bytes": (_sz, [C.POINTER(mplak_kernel_desc), C.c_int, _i64]),
+    "mplak_kernel_mrhs": (C.c_int, [C.POINTER(mplak_kernel_desc), C.c_int, _i64, _vp, _i64, _vp, _i64, _vp, _sz, _vp,
+                                    _vp]),
     "mplak_cg_workspace_bytes": (_sz, [_i64]),
     "mplak_cg_start": (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, C.c_int, _dbl, _i32, _dbl, _vp, _sz, _vp]),
     "mplak_cg_iterate": (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
