@@ -48,6 +48,17 @@ void mpla_dgemv_core_kernel_fused(struct mpla_vector* b, struct mpla_generic_mat
  * instance->last_iterations the total inner iterations. */
 void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_generic_matrix* A, struct mpla_vector* x, int iter_max, double epsilon, int inner_precision, struct mpla_instance* instance);
 
+/* Solve (K + nugget I) X = B for nrhs right-hand sides: B and X are device
+ * arrays, column-major N x nrhs with leading dimension A->cur_proc_row_count.
+ * Batched CG (per-column scalars and stopping tests, one shared multi-RHS
+ * product per iteration) inside FP64 iterative refinement; inner_precision is
+ * MPLA_PREC_BF16, MPLA_PREC_BF16X3 or MPLA_PREC_F32 for the shared product
+ * (raised automatically if refinement stagnates), or MPLA_PREC_F64 for plain
+ * batched FP64 CG.  iter_max bounds the shared products.  rel_res (host array
+ * of nrhs, may be NULL) receives every column's true FP64 relative residual;
+ * instance->last_rel_res is the largest of them. */
+void mpla_kernel_block_conjugate_gradient_mixed(double* B, double* X, int nrhs, struct mpla_generic_matrix* A, int iter_max, double epsilon, int inner_precision, double* rel_res, struct mpla_instance* instance);
+
 #ifdef __cplusplus
 }
 #endif

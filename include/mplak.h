@@ -218,6 +218,38 @@ int mplak_cg_refresh_rr(int64_t n, const double* r, mplak_cg_state* state, void*
 int mplak_cg_copy_state(const mplak_cg_state* src, mplak_cg_state* dst, mplak_stream_t stream);
 
 /* ---------------------------------------------------------------------------
+ * Batched CG over nrhs right-hand sides (no reference counterpart)
+ *
+ * Column c of the N x nrhs column-major blocks X, R, D, Z (leading dimension
+ * n) runs the CG recurrence of src/mpla.cpp:937-975 with its own scalars in
+ * states[c]; all columns share one multi-RHS product Z = A D per iteration
+ * (skip_if = &group->all_done).  States are double-buffered per iteration as
+ * for mplak_cg_iterate (in = states + p*nrhs, out = states + (1-p)*nrhs).
+ * ------------------------------------------------------------------------ */
+
+typedef struct mplak_bcg_group {
+  int32_t all_done; /* 1 once no column is running */
+  int32_t active;   /* columns still running */
+  int32_t pad[2];
+} mplak_bcg_group;
+
+size_t mplak_bcg_workspace_bytes(int64_t n, int64_t nrhs);
+
+/* R = B - Z, D = R, per-column states (rule, eps, iter_max), group flags. */
+int mplak_bcg_start(int64_t n, int64_t nrhs, const double* B, const double* Z, double* R, double* D,
+                    mplak_cg_state* states, mplak_bcg_group* group, int rule, double eps, int32_t iter_max,
+                    double n_global, void* work, size_t work_bytes, mplak_stream_t stream);
+
+/* One batched iteration after Z = A D; updates `group` from `out`. */
+int mplak_bcg_iterate(int64_t n, int64_t nrhs, double* X, double* R, double* D, const double* Z,
+                      const mplak_cg_state* in, mplak_cg_state* out, mplak_bcg_group* group, void* work,
+                      size_t work_bytes, mplak_stream_t stream);
+
+/* Copy nrhs state records and recompute the group flags from dst (group may be NULL). */
+int mplak_bcg_copy_states(const mplak_cg_state* src, mplak_cg_state* dst, int64_t nrhs, mplak_bcg_group* group,
+                          mplak_stream_t stream);
+
+/* ---------------------------------------------------------------------------
  * Fused BiCGSTAB steps (src/mpla.cpp:1003-1183), device-resident scalars,
  * double-buffered state as for CG.  One iteration:
  *     z  = A p ;  mplak_bicgstab_step_s     alpha = rho/<z,r_hat>, s = r - alpha z

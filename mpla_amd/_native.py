@@ -66,6 +66,10 @@ class mplak_cg_state(C.Structure):
     ]
 
 
+class mplak_bcg_group(C.Structure):
+    _fields_ = [("all_done", C.c_int32), ("active", C.c_int32), ("pad", C.c_int32 * 2)]
+
+
 class mplak_bicgstab_state(C.Structure):
     _fields_ = [
         ("rho", C.c_double),
@@ -108,6 +112,10 @@ MPLAK_SIGNATURES = {
     "mplak_cg_iterate": (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
     "mplak_cg_refresh_rr": (C.c_int, [_i64, _vp, _vp, _vp, _sz, _vp]),
     "mplak_cg_copy_state": (C.c_int, [_vp, _vp, _vp]),
+    "mplak_bcg_workspace_bytes": (_sz, [_i64, _i64]),
+    "mplak_bcg_start": (C.c_int, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int, _dbl, _i32, _dbl, _vp, _sz, _vp]),
+    "mplak_bcg_iterate": (C.c_int, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
+    "mplak_bcg_copy_states": (C.c_int, [_vp, _vp, _i64, _vp, _vp]),
     "mplak_bicgstab_workspace_bytes": (_sz, [_i64]),
     "mplak_bicgstab_start": (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _dbl, _i32, _i32, _vp, _sz, _vp]),
     "mplak_bicgstab_step_s": (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),

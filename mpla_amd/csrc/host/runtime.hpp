@@ -29,7 +29,7 @@ struct Runtime {
   void* pinned = nullptr;  // kPinnedBytes of page-locked host memory
 };
 
-constexpr size_t kPinnedBytes = 4096;
+constexpr size_t kPinnedBytes = 65536;
 
 Runtime* runtime(mpla_instance* inst);
 

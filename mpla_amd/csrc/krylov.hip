@@ -417,6 +417,160 @@ __global__ void copy_bicg_state_kernel(const mplak_bicgstab_state* __restrict__ 
   if (threadIdx.x == 0) *dst = *src;
 }
 
+
+// ---------------------------------------------------------------------------
+// Batched CG over nrhs columns (no reference counterpart): every column runs
+// the recurrence of src/mpla.cpp:937-975 with its own scalars, all columns
+// share one multi-RHS operator launch per iteration.  Grids are
+// (vector blocks, columns); partial sums of column c live at part[c*G + g].
+// A finished column is frozen (its kernels return at once); bcg_group sets
+// all_done once every column has stopped, which lets the shared product skip.
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void bcg_start_kernel(int64_t n, const double* __restrict__ B,
+                                                           const double* __restrict__ Z, double* __restrict__ R,
+                                                           double* __restrict__ D, double* __restrict__ part) {
+  __shared__ double scratch[kBlock / kWave];
+  const int64_t c = blockIdx.y;
+  const int G = gridDim.x;
+  const int64_t off = c * n;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  double rr = 0.0, bb = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double bi = B[off + i];
+    const double ri = bi - Z[off + i];
+    R[off + i] = ri;
+    D[off + i] = ri;
+    rr = fma(ri, ri, rr);
+    bb = fma(bi, bi, bb);
+  }
+  rr = block_sum<kBlock>(rr, scratch);
+  bb = block_sum<kBlock>(bb, scratch);
+  if (threadIdx.x == 0) {
+    part[(2 * c) * G + blockIdx.x] = rr;
+    part[(2 * c + 1) * G + blockIdx.x] = bb;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void bcg_start_final_kernel(const double* __restrict__ part, int G,
+                                                                 mplak_cg_state* st, int rule, double eps,
+                                                                 int iter_max, double n_global) {
+  __shared__ double scratch[kBlock / kWave];
+  const int c = blockIdx.x;
+  const double rr = reduce_partials<kBlock>(part + (2 * c) * G, G, scratch);
+  const double bb = reduce_partials<kBlock>(part + (2 * c + 1) * G, G, scratch);
+  if (threadIdx.x == 0) {
+    mplak_cg_state s;
+    s.rr = rr;
+    s.bnorm2 = bb;
+    s.eps = eps;
+    s.n_global = n_global;
+    s.rel = stop_value(rule, rr, bb, n_global);
+    s.alpha = s.beta = 0.0;
+    s.k = 0;
+    s.iter_max = iter_max;
+    s.rule = rule;
+    s.done = !(s.rel >= eps) ? MPLAK_CONVERGED : (iter_max <= 1 ? MPLAK_ITER_LIMIT : MPLAK_RUNNING);
+    st[c] = s;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void bcg_group_kernel(const mplak_cg_state* __restrict__ st, int nrhs,
+                                                           mplak_bcg_group* __restrict__ group) {
+  __shared__ int scratch[kBlock / kWave];
+  int running = 0;
+  for (int c = threadIdx.x; c < nrhs; c += kBlock) running += st[c].done == MPLAK_RUNNING ? 1 : 0;
+  running = block_sum<kBlock>(running, scratch);
+  if (threadIdx.x == 0) {
+    group->active = running;
+    group->all_done = running == 0 ? 1 : 0;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void bcg_dz_kernel(int64_t n, const double* __restrict__ D,
+                                                        const double* __restrict__ Z,
+                                                        const mplak_cg_state* __restrict__ in,
+                                                        double* __restrict__ part) {
+  const int64_t c = blockIdx.y;
+  if (in[c].done != MPLAK_RUNNING) return;
+  __shared__ double scratch[kBlock / kWave];
+  const int64_t off = c * n;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    s = fma(D[off + i], Z[off + i], s);
+  s = block_sum<kBlock>(s, scratch);
+  if (threadIdx.x == 0) part[c * gridDim.x + blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kBlock) void bcg_update_kernel(int64_t n, double* __restrict__ X,
+                                                            double* __restrict__ R, const double* __restrict__ D,
+                                                            const double* __restrict__ Z,
+                                                            const mplak_cg_state* __restrict__ in, int64_t nrhs,
+                                                            double* __restrict__ part) {
+  const int64_t c = blockIdx.y;
+  if (in[c].done != MPLAK_RUNNING) return;
+  __shared__ double scratch[kBlock / kWave];
+  const int G = gridDim.x;
+  const double dz = reduce_partials<kBlock>(part + c * G, G, scratch);
+  const double alpha = in[c].rr / dz;
+  const int64_t off = c * n;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  double rr = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    X[off + i] = fma(alpha, D[off + i], X[off + i]);
+    const double ri = fma(-alpha, Z[off + i], R[off + i]);
+    R[off + i] = ri;
+    rr = fma(ri, ri, rr);
+  }
+  rr = block_sum<kBlock>(rr, scratch);
+  if (threadIdx.x == 0) part[(nrhs + c) * G + blockIdx.x] = rr;
+}
+
+__global__ __launch_bounds__(kBlock) void bcg_direct_kernel(int64_t n, const double* __restrict__ R,
+                                                            double* __restrict__ D,
+                                                            const mplak_cg_state* __restrict__ in,
+                                                            mplak_cg_state* __restrict__ out, int64_t nrhs,
+                                                            const double* __restrict__ part) {
+  const int64_t c = blockIdx.y;
+  const mplak_cg_state s_in = in[c];
+  if (s_in.done != MPLAK_RUNNING) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[c] = s_in;
+    return;
+  }
+  __shared__ double scratch[kBlock / kWave];
+  const int G = gridDim.x;
+  const double dz = reduce_partials<kBlock>(part + c * G, G, scratch);
+  const double rr_new = reduce_partials<kBlock>(part + (nrhs + c) * G, G, scratch);
+  const double beta = rr_new / s_in.rr;
+  const int k = s_in.k + 1;
+  const double rel = stop_value(s_in.rule, rr_new, s_in.bnorm2, s_in.n_global);
+  const bool conv = rel < s_in.eps;
+  const bool limit = k >= s_in.iter_max - 1;
+  if (!conv) {
+    const int64_t off = c * n;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+      D[off + i] = fma(beta, D[off + i], R[off + i]);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    mplak_cg_state s = s_in;
+    s.rr = rr_new;
+    s.rel = rel;
+    s.alpha = s_in.rr / dz;
+    s.beta = beta;
+    s.k = k;
+    s.done = conv ? MPLAK_CONVERGED : (limit ? MPLAK_ITER_LIMIT : MPLAK_RUNNING);
+    out[c] = s;
+  }
+}
+
+__global__ void copy_states_kernel(const mplak_cg_state* __restrict__ src, mplak_cg_state* __restrict__ dst,
+                                   int count) {
+  for (int c = threadIdx.x; c < count; c += blockDim.x) dst[c] = src[c];
+}
+
+
 __global__ __launch_bounds__(kBlock) void dot_into_kernel(int64_t n, const double* __restrict__ a,
                                                           double* __restrict__ part) {
   __shared__ double scratch[kBlock / kWave];
@@ -595,6 +749,74 @@ int mplak_bicgstab_copy_state(const mplak_bicgstab_state* src, mplak_bicgstab_st
                               mplak_stream_t s) {
   if (!src || !dst) return MPLAK_ERR_ARG;
   hipLaunchKernelGGL(copy_bicg_state_kernel, dim3(1), dim3(kWave), 0, as_stream(s), src, dst);
+  return launch_status();
+}
+
+}  // extern "C"
+
+extern "C" {
+
+size_t mplak_bcg_workspace_bytes(int64_t n, int64_t nrhs) {
+  return (size_t)2 * vec_blocks(n > 0 ? n : 1) * (size_t)(nrhs > 0 ? nrhs : 1) * sizeof(double);
+}
+
+static int bcg_grid(int64_t n, int64_t nrhs, dim3* grid) {
+  if (nrhs <= 0 || nrhs > 65535) return MPLAK_ERR_ARG;
+  *grid = dim3(vec_blocks(n), (unsigned)nrhs);
+  return MPLAK_OK;
+}
+
+int mplak_bcg_start(int64_t n, int64_t nrhs, const double* B, const double* Z, double* R, double* D,
+                    mplak_cg_state* states, mplak_bcg_group* group, int rule, double eps, int32_t iter_max,
+                    double n_global, void* work, size_t work_bytes, mplak_stream_t s) {
+  dim3 grid;
+  if (n <= 0 || !B || !Z || !R || !D || !states || !group || bcg_grid(n, nrhs, &grid) != MPLAK_OK)
+    return MPLAK_ERR_ARG;
+  if (rule != MPLAK_STOP_RMS && rule != MPLAK_STOP_RELATIVE) return MPLAK_ERR_ARG;
+  if (work == nullptr || work_bytes < mplak_bcg_workspace_bytes(n, nrhs)) return MPLAK_ERR_WORKSPACE;
+  double* part = static_cast<double*>(work);
+  hipStream_t hs = as_stream(s);
+  hipLaunchKernelGGL(bcg_start_kernel, grid, dim3(kBlock), 0, hs, n, B, Z, R, D, part);
+  int st = launch_status();
+  if (st != MPLAK_OK) return st;
+  hipLaunchKernelGGL(bcg_start_final_kernel, dim3((unsigned)nrhs), dim3(kBlock), 0, hs, part, (int)grid.x, states,
+                     rule, eps, (int)iter_max, n_global);
+  st = launch_status();
+  if (st != MPLAK_OK) return st;
+  hipLaunchKernelGGL(bcg_group_kernel, dim3(1), dim3(kBlock), 0, hs, states, (int)nrhs, group);
+  return launch_status();
+}
+
+int mplak_bcg_iterate(int64_t n, int64_t nrhs, double* X, double* R, double* D, const double* Z,
+                      const mplak_cg_state* in, mplak_cg_state* out, mplak_bcg_group* group, void* work,
+                      size_t work_bytes, mplak_stream_t s) {
+  dim3 grid;
+  if (n <= 0 || !X || !R || !D || !Z || !in || !out || !group || in == out || bcg_grid(n, nrhs, &grid) != MPLAK_OK)
+    return MPLAK_ERR_ARG;
+  if (work == nullptr || work_bytes < mplak_bcg_workspace_bytes(n, nrhs)) return MPLAK_ERR_WORKSPACE;
+  double* part = static_cast<double*>(work);
+  hipStream_t hs = as_stream(s);
+  hipLaunchKernelGGL(bcg_dz_kernel, grid, dim3(kBlock), 0, hs, n, D, Z, in, part);
+  int st = launch_status();
+  if (st != MPLAK_OK) return st;
+  hipLaunchKernelGGL(bcg_update_kernel, grid, dim3(kBlock), 0, hs, n, X, R, D, Z, in, nrhs, part);
+  st = launch_status();
+  if (st != MPLAK_OK) return st;
+  hipLaunchKernelGGL(bcg_direct_kernel, grid, dim3(kBlock), 0, hs, n, R, D, in, out, nrhs, part);
+  st = launch_status();
+  if (st != MPLAK_OK) return st;
+  hipLaunchKernelGGL(bcg_group_kernel, dim3(1), dim3(kBlock), 0, hs, out, (int)nrhs, group);
+  return launch_status();
+}
+
+int mplak_bcg_copy_states(const mplak_cg_state* src, mplak_cg_state* dst, int64_t nrhs, mplak_bcg_group* group,
+                          mplak_stream_t s) {
+  if (!src || !dst || nrhs <= 0 || nrhs > 65535) return MPLAK_ERR_ARG;
+  hipStream_t hs = as_stream(s);
+  hipLaunchKernelGGL(copy_states_kernel, dim3(1), dim3(kBlock), 0, hs, src, dst, (int)nrhs);
+  int st = launch_status();
+  if (st != MPLAK_OK || group == nullptr) return st;
+  hipLaunchKernelGGL(bcg_group_kernel, dim3(1), dim3(kBlock), 0, hs, dst, (int)nrhs, group);
   return launch_status();
 }
 

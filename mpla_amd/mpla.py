@@ -142,6 +142,7 @@ MPLA_SIGNATURES = {
     # mpla_kernel.h
     "mpla_dgemv_core_kernel_fused": (None, [_V, _G, _V, _I]),
     "mpla_kernel_conjugate_gradient_mixed": (None, [_V, _G, _V, _int, _dbl, _int, _I]),
+    "mpla_kernel_block_conjugate_gradient_mixed": (None, [_vp, _vp, _int, _G, _int, _dbl, _int, _vp, _I]),
     # generic_system_adapter.h / kernel_system_assembler.h
     "mpla_set_generic_system_rhs": (None, [_V, _G, _I]),
     "mpla_dgemv_core_generic_system_matrix_cublas": (None, [_V, _G, _V, _I]),
