@@ -250,6 +250,90 @@ int mplak_bcg_copy_states(const mplak_cg_state* src, mplak_cg_state* dst, int64_
                           mplak_stream_t stream);
 
 /* ---------------------------------------------------------------------------
+ * Ragged batches: many independent kernel systems in one launch (serving;
+ * no reference counterpart)
+ *
+ * A batch is a device array of system records plus a device array of work
+ * items; workgroup b of a launch processes items[b].  Each system's work split
+ * (tiles, slabs, vector blocks) depends only on its own n, nrhs and
+ * precision, so its arithmetic is identical alone and in any batch.
+ * ------------------------------------------------------------------------ */
+
+typedef struct mplak_ragged_system {
+  const double* points;  /* SoA: coordinate d of point i at points[d * n + i] (dim <= 4) */
+  const double* V;       /* product input, n x nrhs column-major (leading dimension n) */
+  double* Y;             /* product output, n x nrhs column-major */
+  double* part;          /* slab partials, mplak_ragged_plan() bytes */
+  const int32_t* skip;   /* the system's product is skipped when non-NULL and *skip != 0 */
+  int64_t n;             /* points (the system is n x n) */
+  int64_t slab_cols;     /* planned */
+  double length_scale;
+  double nugget;         /* added on the diagonal */
+  double scale;          /* planned: coordinate scale of the entry formula */
+  int32_t dim;
+  int32_t kind;          /* MPLAK_KERNEL_* */
+  int32_t nrhs;
+  int32_t precision;     /* MPLAK_PREC_F64 or MPLAK_PREC_F32 */
+  int32_t rt, ctiles, tiles, slabs; /* planned */
+} mplak_ragged_system;
+
+typedef struct mplak_ragged_item {
+  int32_t sys;   /* index into the system array */
+  int32_t tile;  /* product: target tile of 256 rows; finish: block of 256 entries; CG: vector block */
+  int32_t slab;  /* product: source slab; CG: column */
+  int32_t ctile; /* product: column tile of rt right-hand sides */
+} mplak_ragged_item;
+
+/* Host only: fills the planned fields of *s from (n, nrhs, precision, kind,
+ * length_scale) and returns the bytes the system's `part` buffer needs. */
+size_t mplak_ragged_plan(mplak_ragged_system* s);
+
+/* Host only: number of product / finish work items of a planned system; when
+ * `out` is non-NULL also writes them (sys field = sys_index). */
+int64_t mplak_ragged_product_items(const mplak_ragged_system* s, mplak_ragged_item* out, int32_t sys_index);
+int64_t mplak_ragged_finish_items(const mplak_ragged_system* s, mplak_ragged_item* out, int32_t sys_index);
+
+/* Product partials for `count` items whose systems share rt and precision
+ * (systems and items are device arrays); then mplak_ragged_finish adds the
+ * slabs in order and the nugget term: Y = (K + nugget I) V per system. */
+int mplak_ragged_product(const mplak_ragged_system* systems, const mplak_ragged_item* items, int64_t count, int rt,
+                         int precision, mplak_stream_t stream);
+int mplak_ragged_finish(const mplak_ragged_system* systems, const mplak_ragged_item* items, int64_t count,
+                        mplak_stream_t stream);
+
+typedef struct mplak_ragged_cg {
+  double* X;             /* the iterate CG updates (the correction in refinement) */
+  double* R;             /* residual, n x nrhs */
+  double* D;             /* direction, n x nrhs */
+  const double* Z;       /* operator applied to D (to the solution for the start step) */
+  const double* B;       /* right-hand sides for the start step */
+  double* Xout;          /* MPLAK_RAGGED_AXPY adds X into Xout */
+  mplak_cg_state* states; /* 2 x nrhs records, double-buffered by iteration parity */
+  mplak_bcg_group* group; /* all_done / active of the system */
+  double* part;          /* mplak_ragged_cg_part_bytes(n, nrhs) */
+  int64_t n;
+  int32_t nrhs;
+  int32_t G;             /* vector blocks per column: mplak_ragged_cg_blocks(n) */
+  double eps;            /* tolerance written by MPLAK_RAGGED_START_FINAL */
+} mplak_ragged_cg;
+
+#define MPLAK_RAGGED_START 0       /* R = B - Z, D = R, partial <r,r>, <b,b>; items (sys, block, column) */
+#define MPLAK_RAGGED_START_FINAL 1 /* states[column] of buffer 0 with rel = ||r||/||b||; items (sys, -, column) */
+#define MPLAK_RAGGED_GROUP 2       /* group flags from buffer `parity`; items (sys, -, -) */
+#define MPLAK_RAGGED_DZ 3          /* the three kernels of one CG iteration reading buffer `parity` */
+#define MPLAK_RAGGED_UPDATE 4
+#define MPLAK_RAGGED_DIRECT 5
+#define MPLAK_RAGGED_AXPY 6        /* Xout += X */
+#define MPLAK_RAGGED_FILL 7        /* X = value */
+
+int32_t mplak_ragged_cg_blocks(int64_t n);
+size_t mplak_ragged_cg_part_bytes(int64_t n, int32_t nrhs);
+/* Host only: the (sys, vector block, column) items of one system. */
+int64_t mplak_ragged_cg_items(const mplak_ragged_cg* s, mplak_ragged_item* out, int32_t sys_index);
+int mplak_ragged_cg_op(int op, const mplak_ragged_cg* systems, const mplak_ragged_item* items, int64_t count,
+                       int parity, double value, mplak_stream_t stream);
+
+/* ---------------------------------------------------------------------------
  * Fused BiCGSTAB steps (src/mpla.cpp:1003-1183), device-resident scalars,
  * double-buffered state as for CG.  One iteration:
  *     z  = A p ;  mplak_bicgstab_step_s     alpha = rho/<z,r_hat>, s = r - alpha z

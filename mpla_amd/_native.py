@@ -116,6 +116,15 @@ MPLAK_SIGNATURES = {
     "mplak_bcg_start": (C.c_int, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int, _dbl, _i32, _dbl, _vp, _sz, _vp]),
     "mplak_bcg_iterate": (C.c_int, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
     "mplak_bcg_copy_states": (C.c_int, [_vp, _vp, _i64, _vp, _vp]),
+    "mplak_ragged_plan": (_sz, [_vp]),
+    "mplak_ragged_product_items": (_i64, [_vp, _vp, _i32]),
+    "mplak_ragged_finish_items": (_i64, [_vp, _vp, _i32]),
+    "mplak_ragged_product": (C.c_int, [_vp, _vp, _i64, C.c_int, C.c_int, _vp]),
+    "mplak_ragged_finish": (C.c_int, [_vp, _vp, _i64, _vp]),
+    "mplak_ragged_cg_blocks": (_i32, [_i64]),
+    "mplak_ragged_cg_part_bytes": (_sz, [_i64, _i32]),
+    "mplak_ragged_cg_items": (_i64, [_vp, _vp, _i32]),
+    "mplak_ragged_cg_op": (C.c_int, [C.c_int, _vp, _vp, _i64, C.c_int, _dbl, _vp]),
     "mplak_bicgstab_workspace_bytes": (_sz, [_i64]),
     "mplak_bicgstab_start": (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _dbl, _i32, _i32, _vp, _sz, _vp]),
     "mplak_bicgstab_step_s": (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),

@@ -1,0 +1,416 @@
+// Ragged batched kernels for serving many independent kernel-interpolation
+// solves at once (no reference counterpart).
+//
+// A batch is a device array of systems (one per request: its own points, N,
+// kernel, right-hand sides and buffers) plus a device table of work items.
+// One launch covers every system in the batch: workgroup b processes item b,
+// whose fields name the system and the tile of work.  Each system's tiles,
+// slabs and vector blocks depend only on that system's own (n, nrhs,
+// precision), never on what else is in the batch, so a system's arithmetic,
+// and therefore its result, is bitwise the same alone and in any mix.
+//
+// Products: the VALU scheme of the multi-RHS product (one target per thread,
+// RT columns per thread, sources through LDS, FP64 slab partials) with the
+// kernel function, dimension (<= 4) and scale read from the system record.
+// CG: the batched recurrences of the multi-RHS solver with per-column
+// scalars in a state pool, items (system, column, vector block).
+
+#include <math.h>
+
+#include "common.hpp"
+
+namespace mplak {
+namespace {
+
+constexpr int kChunk = 256;
+constexpr int kRagTargetBlocks = 2048;
+constexpr int kMaxRaggedDim = 4;
+constexpr double kLog2e = 1.4426950408889634074;
+
+inline int vec_blocks_r(int64_t n) {
+  int64_t b = (n + kBlock * 8 - 1) / (kBlock * 8);
+  if (b < 1) b = 1;
+  if (b > 1024) b = 1024;
+  return (int)b;
+}
+
+inline int rt_class(int64_t R) { return R <= 1 ? 1 : (R <= 2 ? 2 : (R <= 4 ? 4 : 8)); }
+
+template <typename T>
+__device__ __forceinline__ T rag_entry(int kind, T s);
+template <>
+__device__ __forceinline__ double rag_entry<double>(int kind, double s) {
+  if (kind == MPLAK_KERNEL_GAUSSIAN) return exp(-s);
+  const double r = sqrt(s);
+  const double e = exp(-r);
+  return fma(r, e, e);
+}
+template <>
+__device__ __forceinline__ float rag_entry<float>(int kind, float s) {
+  if (kind == MPLAK_KERNEL_GAUSSIAN) return __builtin_amdgcn_exp2f(-s);
+  const float r = __builtin_amdgcn_sqrtf(s);
+  const float e = __builtin_amdgcn_exp2f(-r * (float)kLog2e);
+  return fmaf(r, e, e);
+}
+
+template <typename T, int RT>
+__global__ __launch_bounds__(kBlock) void ragged_product_kernel(const mplak_ragged_system* __restrict__ systems,
+                                                                const mplak_ragged_item* __restrict__ items) {
+  const mplak_ragged_item it = items[blockIdx.x];
+  const mplak_ragged_system S = systems[it.sys];
+  if (S.skip != nullptr && *S.skip != 0) return;
+  __shared__ T lp[kMaxRaggedDim][kChunk];
+  __shared__ T lv[RT][kChunk];
+  const int tid = threadIdx.x;
+  const int64_t n = S.n;
+  const int dim = S.dim;
+  const int64_t R = S.nrhs;
+  int64_t i = (int64_t)it.tile * kBlock + tid;
+  const bool active = i < n;
+  if (!active) i = n - 1;
+  const int64_t r0 = (int64_t)it.ctile * RT;
+  T pi[kMaxRaggedDim];
+#pragma unroll
+  for (int d = 0; d < kMaxRaggedDim; ++d) pi[d] = d < dim ? (T)(S.points[d * n + i] * S.scale) : T(0);
+  const int64_t c0 = (int64_t)it.slab * S.slab_cols;
+  const int64_t c1 = (c0 + S.slab_cols < n) ? c0 + S.slab_cols : n;
+  double acc[RT];
+#pragma unroll
+  for (int q = 0; q < RT; ++q) acc[q] = 0.0;
+  for (int64_t cb = c0; cb < c1; cb += kChunk) {
+    __syncthreads();
+    const int64_t j = cb + tid;
+#pragma unroll
+    for (int d = 0; d < kMaxRaggedDim; ++d) lp[d][tid] = (j < c1 && d < dim) ? (T)(S.points[d * n + j] * S.scale) : T(0);
+#pragma unroll
+    for (int q = 0; q < RT; ++q) lv[q][tid] = (j < c1 && r0 + q < R) ? (T)S.V[(r0 + q) * n + j] : T(0);
+    __syncthreads();
+    T cacc[RT];
+#pragma unroll
+    for (int q = 0; q < RT; ++q) cacc[q] = T(0);
+    for (int jj = 0; jj < kChunk; ++jj) {
+      T s = T(0);
+#pragma unroll
+      for (int d = 0; d < kMaxRaggedDim; ++d) {
+        const T df = pi[d] - lp[d][jj];
+        s = fma(df, df, s);
+      }
+      const T k = rag_entry<T>(S.kind, s);
+#pragma unroll
+      for (int q = 0; q < RT; ++q) cacc[q] = fma(k, lv[q][jj], cacc[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < RT; ++q) acc[q] += (double)cacc[q];
+  }
+  if (active) {
+#pragma unroll
+    for (int q = 0; q < RT; ++q)
+      if (r0 + q < R) S.part[((int64_t)it.slab * R + r0 + q) * n + i] = acc[q];
+  }
+}
+
+// items: (system, block) over n * nrhs entries of Y, 256 per block
+__global__ __launch_bounds__(kBlock) void ragged_finish_kernel(const mplak_ragged_system* __restrict__ systems,
+                                                               const mplak_ragged_item* __restrict__ items) {
+  const mplak_ragged_item it = items[blockIdx.x];
+  const mplak_ragged_system S = systems[it.sys];
+  if (S.skip != nullptr && *S.skip != 0) return;
+  const int64_t g = (int64_t)it.tile * kBlock + threadIdx.x;
+  const int64_t n = S.n, R = S.nrhs;
+  if (g >= n * R) return;
+  const int64_t i = g % n, r = g / n;
+  double s = S.part[r * n + i];
+  for (int64_t k = 1; k < S.slabs; ++k) s += S.part[(k * R + r) * n + i];
+  if (S.nugget != 0.0) s = fma(S.nugget, S.V[r * n + i], s);
+  S.Y[r * n + i] = s;
+}
+
+// ---- ragged CG --------------------------------------------------------------
+
+__device__ __forceinline__ mplak_cg_state* cg_in(const mplak_ragged_cg& S, int parity) {
+  return S.states + (int64_t)parity * S.nrhs;
+}
+
+// items: (system, column, vector block)
+__global__ __launch_bounds__(kBlock) void ragged_start_kernel(const mplak_ragged_cg* __restrict__ systems,
+                                                              const mplak_ragged_item* __restrict__ items) {
+  __shared__ double scratch[kBlock / kWave];
+  const mplak_ragged_item it = items[blockIdx.x];
+  const mplak_ragged_cg S = systems[it.sys];
+  const int64_t n = S.n, off = (int64_t)it.slab * n;
+  const int G = S.G;
+  double rr = 0.0, bb = 0.0;
+  for (int64_t i = (int64_t)it.tile * kBlock + threadIdx.x; i < n; i += (int64_t)G * kBlock) {
+    const double bi = S.B[off + i];
+    const double ri = bi - S.Z[off + i];
+    S.R[off + i] = ri;
+    S.D[off + i] = ri;
+    rr = fma(ri, ri, rr);
+    bb = fma(bi, bi, bb);
+  }
+  rr = block_sum<kBlock>(rr, scratch);
+  bb = block_sum<kBlock>(bb, scratch);
+  if (threadIdx.x == 0) {
+    S.part[(2 * it.slab) * G + it.tile] = rr;
+    S.part[(2 * it.slab + 1) * G + it.tile] = bb;
+  }
+}
+
+// items: (system, column) in fields (sys, slab); writes states[column] of buffer 0
+__global__ __launch_bounds__(kBlock) void ragged_start_final_kernel(const mplak_ragged_cg* __restrict__ systems,
+                                                                    const mplak_ragged_item* __restrict__ items) {
+  __shared__ double scratch[kBlock / kWave];
+  const mplak_ragged_item it = items[blockIdx.x];
+  const mplak_ragged_cg S = systems[it.sys];
+  const int G = S.G, c = it.slab;
+  const double rr = reduce_partials<kBlock>(S.part + (2 * c) * G, G, scratch);
+  const double bb = reduce_partials<kBlock>(S.part + (2 * c + 1) * G, G, scratch);
+  if (threadIdx.x == 0) {
+    mplak_cg_state s;
+    s.rr = rr;
+    s.bnorm2 = bb;
+    s.eps = S.eps;
+    s.n_global = (double)S.n;
+    s.rel = sqrt(rr) / sqrt(bb);
+    s.alpha = s.beta = 0.0;
+    s.k = 0;
+    s.iter_max = 2;
+    s.rule = MPLAK_STOP_RELATIVE;
+    s.done = !(s.rel >= S.eps) ? MPLAK_CONVERGED : MPLAK_RUNNING;
+    S.states[c] = s;
+  }
+}
+
+// one workgroup per system (items: (system, -)); `parity` selects the buffer to read
+__global__ __launch_bounds__(kBlock) void ragged_group_kernel(const mplak_ragged_cg* __restrict__ systems,
+                                                              const mplak_ragged_item* __restrict__ items, int parity) {
+  __shared__ int scratch[kBlock / kWave];
+  const mplak_ragged_item it = items[blockIdx.x];
+  const mplak_ragged_cg S = systems[it.sys];
+  const mplak_cg_state* st = cg_in(S, parity);
+  int running = 0;
+  for (int c = threadIdx.x; c < S.nrhs; c += kBlock) running += st[c].done == MPLAK_RUNNING ? 1 : 0;
+  running = block_sum<kBlock>(running, scratch);
+  if (threadIdx.x == 0) {
+    S.group->active = running;
+    S.group->all_done = running == 0 ? 1 : 0;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void ragged_dz_kernel(const mplak_ragged_cg* __restrict__ systems,
+                                                           const mplak_ragged_item* __restrict__ items, int parity) {
+  const mplak_ragged_item it = items[blockIdx.x];
+  const mplak_ragged_cg S = systems[it.sys];
+  const int c = it.slab;
+  if (cg_in(S, parity)[c].done != MPLAK_RUNNING) return;
+  __shared__ double scratch[kBlock / kWave];
+  const int64_t n = S.n, off = (int64_t)c * n;
+  const int G = S.G;
+  double s = 0.0;
+  for (int64_t i = (int64_t)it.tile * kBlock + threadIdx.x; i < n; i += (int64_t)G * kBlock)
+    s = fma(S.D[off + i], S.Z[off + i], s);
+  s = block_sum<kBlock>(s, scratch);
+  if (threadIdx.x == 0) S.part[c * G + it.tile] = s;
+}
+
+__global__ __launch_bounds__(kBlock) void ragged_update_kernel(const mplak_ragged_cg* __restrict__ systems,
+                                                               const mplak_ragged_item* __restrict__ items,
+                                                               int parity) {
+  const mplak_ragged_item it = items[blockIdx.x];
+  const mplak_ragged_cg S = systems[it.sys];
+  const int c = it.slab;
+  const mplak_cg_state* in = cg_in(S, parity);
+  if (in[c].done != MPLAK_RUNNING) return;
+  __shared__ double scratch[kBlock / kWave];
+  const int G = S.G;
+  const double alpha = in[c].rr / reduce_partials<kBlock>(S.part + c * G, G, scratch);
+  const int64_t n = S.n, off = (int64_t)c * n;
+  double rr = 0.0;
+  for (int64_t i = (int64_t)it.tile * kBlock + threadIdx.x; i < n; i += (int64_t)G * kBlock) {
+    S.X[off + i] = fma(alpha, S.D[off + i], S.X[off + i]);
+    const double ri = fma(-alpha, S.Z[off + i], S.R[off + i]);
+    S.R[off + i] = ri;
+    rr = fma(ri, ri, rr);
+  }
+  rr = block_sum<kBlock>(rr, scratch);
+  if (threadIdx.x == 0) S.part[(S.nrhs + c) * G + it.tile] = rr;
+}
+
+__global__ __launch_bounds__(kBlock) void ragged_direct_kernel(const mplak_ragged_cg* __restrict__ systems,
+                                                               const mplak_ragged_item* __restrict__ items,
+                                                               int parity) {
+  const mplak_ragged_item it = items[blockIdx.x];
+  const mplak_ragged_cg S = systems[it.sys];
+  const int c = it.slab;
+  const mplak_cg_state s_in = cg_in(S, parity)[c];
+  mplak_cg_state* out = cg_in(S, 1 - parity);
+  if (s_in.done != MPLAK_RUNNING) {
+    if (it.tile == 0 && threadIdx.x == 0) out[c] = s_in;
+    return;
+  }
+  __shared__ double scratch[kBlock / kWave];
+  const int G = S.G;
+  const double dz = reduce_partials<kBlock>(S.part + c * G, G, scratch);
+  const double rr_new = reduce_partials<kBlock>(S.part + (S.nrhs + c) * G, G, scratch);
+  const double beta = rr_new / s_in.rr;
+  const int k = s_in.k + 1;
+  const double rel = sqrt(rr_new) / sqrt(s_in.bnorm2);
+  const bool conv = rel < s_in.eps;
+  const bool limit = k >= s_in.iter_max - 1;
+  if (!conv) {
+    const int64_t n = S.n, off = (int64_t)c * n;
+    for (int64_t i = (int64_t)it.tile * kBlock + threadIdx.x; i < n; i += (int64_t)G * kBlock)
+      S.D[off + i] = fma(beta, S.D[off + i], S.R[off + i]);
+  }
+  if (it.tile == 0 && threadIdx.x == 0) {
+    mplak_cg_state s = s_in;
+    s.rr = rr_new;
+    s.rel = rel;
+    s.alpha = s_in.rr / dz;
+    s.beta = beta;
+    s.k = k;
+    s.done = conv ? MPLAK_CONVERGED : (limit ? MPLAK_ITER_LIMIT : MPLAK_RUNNING);
+    out[c] = s;
+  }
+}
+
+// items: (system, column, vector block): X += E (E stored in Z's companion buffer `E`)
+__global__ __launch_bounds__(kBlock) void ragged_axpy_kernel(const mplak_ragged_cg* __restrict__ systems,
+                                                             const mplak_ragged_item* __restrict__ items) {
+  const mplak_ragged_item it = items[blockIdx.x];
+  const mplak_ragged_cg S = systems[it.sys];
+  const int64_t n = S.n, off = (int64_t)it.slab * n;
+  for (int64_t i = (int64_t)it.tile * kBlock + threadIdx.x; i < n; i += (int64_t)S.G * kBlock)
+    S.Xout[off + i] += S.X[off + i];
+}
+
+__global__ __launch_bounds__(kBlock) void ragged_fill_kernel(const mplak_ragged_cg* __restrict__ systems,
+                                                             const mplak_ragged_item* __restrict__ items,
+                                                             double value) {
+  const mplak_ragged_item it = items[blockIdx.x];
+  const mplak_ragged_cg S = systems[it.sys];
+  const int64_t n = S.n, off = (int64_t)it.slab * n;
+  for (int64_t i = (int64_t)it.tile * kBlock + threadIdx.x; i < n; i += (int64_t)S.G * kBlock) S.X[off + i] = value;
+}
+
+template <typename T>
+void launch_product(int rt, const mplak_ragged_system* sys, const mplak_ragged_item* items, int64_t count,
+                    hipStream_t st) {
+  switch (rt) {
+    case 1: hipLaunchKernelGGL((ragged_product_kernel<T, 1>), dim3((unsigned)count), dim3(kBlock), 0, st, sys, items); break;
+    case 2: hipLaunchKernelGGL((ragged_product_kernel<T, 2>), dim3((unsigned)count), dim3(kBlock), 0, st, sys, items); break;
+    case 4: hipLaunchKernelGGL((ragged_product_kernel<T, 4>), dim3((unsigned)count), dim3(kBlock), 0, st, sys, items); break;
+    default: hipLaunchKernelGGL((ragged_product_kernel<T, 8>), dim3((unsigned)count), dim3(kBlock), 0, st, sys, items); break;
+  }
+}
+
+inline double coord_scale(int kind, bool f32, double l) {
+  if (kind == MPLAK_KERNEL_GAUSSIAN) {
+    double c = 1.0 / (sqrt(2.0) * l);
+    if (f32) c *= sqrt(kLog2e);
+    return c;
+  }
+  return sqrt(3.0) / l;
+}
+
+}  // namespace
+}  // namespace mplak
+
+using namespace mplak;
+
+extern "C" {
+
+size_t mplak_ragged_plan(mplak_ragged_system* s) {
+  if (s == nullptr || s->n <= 0 || s->nrhs <= 0 || s->dim < 1 || s->dim > kMaxRaggedDim) return 0;
+  s->rt = rt_class(s->nrhs);
+  s->ctiles = (s->nrhs + s->rt - 1) / s->rt;
+  s->tiles = (int32_t)((s->n + kBlock - 1) / kBlock);
+  int64_t ks = (kRagTargetBlocks + (int64_t)s->tiles * s->ctiles - 1) / ((int64_t)s->tiles * s->ctiles);
+  const int64_t chunks = (s->n + kChunk - 1) / kChunk;
+  if (ks > chunks) ks = chunks;
+  if (ks < 1) ks = 1;
+  const int64_t per = (chunks + ks - 1) / ks;
+  s->slab_cols = per * kChunk;
+  s->slabs = (int32_t)((s->n + s->slab_cols - 1) / s->slab_cols);
+  s->scale = coord_scale(s->kind, s->precision != MPLAK_PREC_F64, s->length_scale);
+  return (size_t)s->slabs * (size_t)s->nrhs * (size_t)s->n * sizeof(double);
+}
+
+int64_t mplak_ragged_product_items(const mplak_ragged_system* s, mplak_ragged_item* out, int32_t sys_index) {
+  if (s == nullptr || s->tiles <= 0) return 0;
+  const int64_t count = (int64_t)s->tiles * s->slabs * s->ctiles;
+  if (out) {
+    int64_t k = 0;
+    for (int32_t c = 0; c < s->ctiles; ++c)
+      for (int32_t sl = 0; sl < s->slabs; ++sl)
+        for (int32_t t = 0; t < s->tiles; ++t) out[k++] = mplak_ragged_item{sys_index, t, sl, c};
+  }
+  return count;
+}
+
+int64_t mplak_ragged_finish_items(const mplak_ragged_system* s, mplak_ragged_item* out, int32_t sys_index) {
+  if (s == nullptr || s->n <= 0) return 0;
+  const int64_t count = (s->n * s->nrhs + kBlock - 1) / kBlock;
+  if (out)
+    for (int64_t b = 0; b < count; ++b) out[b] = mplak_ragged_item{sys_index, (int32_t)b, 0, 0};
+  return count;
+}
+
+int mplak_ragged_product(const mplak_ragged_system* systems, const mplak_ragged_item* items, int64_t count, int rt,
+                         int precision, mplak_stream_t s) {
+  if (count <= 0) return MPLAK_OK;
+  if (!systems || !items || count > 0x7fffffff) return MPLAK_ERR_ARG;
+  hipStream_t hs = as_stream(s);
+  if (precision == MPLAK_PREC_F64)
+    launch_product<double>(rt, systems, items, count, hs);
+  else
+    launch_product<float>(rt, systems, items, count, hs);
+  return launch_status();
+}
+
+int mplak_ragged_finish(const mplak_ragged_system* systems, const mplak_ragged_item* items, int64_t count,
+                        mplak_stream_t s) {
+  if (count <= 0) return MPLAK_OK;
+  if (!systems || !items) return MPLAK_ERR_ARG;
+  hipLaunchKernelGGL(ragged_finish_kernel, dim3((unsigned)count), dim3(kBlock), 0, as_stream(s), systems, items);
+  return launch_status();
+}
+
+int32_t mplak_ragged_cg_blocks(int64_t n) { return vec_blocks_r(n); }
+
+size_t mplak_ragged_cg_part_bytes(int64_t n, int32_t nrhs) {
+  return (size_t)2 * vec_blocks_r(n) * (size_t)nrhs * sizeof(double);
+}
+
+int64_t mplak_ragged_cg_items(const mplak_ragged_cg* s, mplak_ragged_item* out, int32_t sys_index) {
+  if (s == nullptr || s->n <= 0) return 0;
+  const int64_t count = (int64_t)s->G * s->nrhs;
+  if (out) {
+    int64_t k = 0;
+    for (int32_t c = 0; c < s->nrhs; ++c)
+      for (int32_t b = 0; b < s->G; ++b) out[k++] = mplak_ragged_item{sys_index, b, c, 0};
+  }
+  return count;
+}
+
+int mplak_ragged_cg_op(int op, const mplak_ragged_cg* systems, const mplak_ragged_item* items, int64_t count,
+                       int parity, double value, mplak_stream_t s) {
+  if (count <= 0) return MPLAK_OK;
+  if (!systems || !items) return MPLAK_ERR_ARG;
+  hipStream_t hs = as_stream(s);
+  const dim3 g((unsigned)count), b(kBlock);
+  switch (op) {
+    case MPLAK_RAGGED_START: hipLaunchKernelGGL(ragged_start_kernel, g, b, 0, hs, systems, items); break;
+    case MPLAK_RAGGED_START_FINAL: hipLaunchKernelGGL(ragged_start_final_kernel, g, b, 0, hs, systems, items); break;
+    case MPLAK_RAGGED_GROUP: hipLaunchKernelGGL(ragged_group_kernel, g, b, 0, hs, systems, items, parity); break;
+    case MPLAK_RAGGED_DZ: hipLaunchKernelGGL(ragged_dz_kernel, g, b, 0, hs, systems, items, parity); break;
+    case MPLAK_RAGGED_UPDATE: hipLaunchKernelGGL(ragged_update_kernel, g, b, 0, hs, systems, items, parity); break;
+    case MPLAK_RAGGED_DIRECT: hipLaunchKernelGGL(ragged_direct_kernel, g, b, 0, hs, systems, items, parity); break;
+    case MPLAK_RAGGED_AXPY: hipLaunchKernelGGL(ragged_axpy_kernel, g, b, 0, hs, systems, items); break;
+    case MPLAK_RAGGED_FILL: hipLaunchKernelGGL(ragged_fill_kernel, g, b, 0, hs, systems, items, value); break;
+    default: return MPLAK_ERR_ARG;
+  }
+  return launch_status();
+}
+
+}  // extern "C"

@@ -161,3 +161,66 @@ def lib() -> C.CDLL:
 def core(name: str) -> int:
     """Address of an exported plugin core, for the solver/generic-dgemv calls."""
     return C.cast(getattr(lib(), name), C.c_void_p).value
+
+
+# ---- mpla_server.h -----------------------------------------------------------
+
+MPLA_SERVE_MIXED_BATCHED = 0
+MPLA_SERVE_FP64_SERIAL = 1
+MPLA_REQUEST_PENDING = 0
+MPLA_REQUEST_CONVERGED = 1
+MPLA_REQUEST_ITER_LIMIT = 2
+MPLA_REQUEST_REJECTED = -1
+
+
+class mpla_server_config(C.Structure):
+    _fields_ = [
+        ("hbm_budget_bytes", C.c_size_t),
+        ("max_active", C.c_int),
+        ("mode", C.c_int),
+        ("iterations_per_step", C.c_int),
+        ("inner_tol_floor", C.c_double),
+        ("max_iterations", C.c_int),
+    ]
+
+
+class mpla_solve_request(C.Structure):
+    _fields_ = [
+        ("points", C.c_void_p),
+        ("rhs", C.c_void_p),
+        ("solution", C.c_void_p),
+        ("n", C.c_int64),
+        ("dim", C.c_int),
+        ("kind", C.c_int),
+        ("length_scale", C.c_double),
+        ("nugget", C.c_double),
+        ("nrhs", C.c_int),
+        ("tolerance", C.c_double),
+        ("arrival", C.c_double),
+    ]
+
+
+class mpla_solve_result(C.Structure):
+    _fields_ = [
+        ("status", C.c_int),
+        ("iterations", C.c_int),
+        ("refinement_steps", C.c_int),
+        ("max_rel_residual", C.c_double),
+        ("t_arrival", C.c_double),
+        ("t_admit", C.c_double),
+        ("t_done", C.c_double),
+    ]
+
+
+MPLA_SIGNATURES.update({
+    "mpla_server_default_config": (None, [_P(mpla_server_config)]),
+    "mpla_server_create": (_vp, [_P(mpla_server_config)]),
+    "mpla_server_destroy": (None, [_vp]),
+    "mpla_server_request_bytes": (C.c_size_t, [C.c_int64, _int, _int]),
+    "mpla_server_submit": (_int, [_vp, _P(mpla_solve_request)]),
+    "mpla_server_step": (_int, [_vp]),
+    "mpla_server_result": (_int, [_vp, _int, _P(mpla_solve_result)]),
+    "mpla_server_run_trace": (_int, [_vp, _P(mpla_solve_request), _int, _P(mpla_solve_result)]),
+    "mpla_server_clock": (C.c_double, [_vp]),
+    "mpla_server_peak_bytes": (C.c_size_t, [_vp]),
+})

@@ -17,6 +17,7 @@ HEADERS = {
     "mpla_kernel.h": "libmpla.so",
     "generic_system_adapter.h": "libmpla.so",
     "kernel_system_assembler.h": "libmpla.so",
+    "mpla_server.h": "libmpla.so",
 }
 
 DECL = re.compile(r"^\s*(?:extern\s+)?[A-Za-z_][\w\s\*]*?[\s\*]((?:mplak|mpla)_\w+|info|checkCUDAError|"
@@ -66,7 +67,8 @@ def test_status_strings_without_gpu():
 def test_python_bindings_cover_headers():
     from mpla_amd import mpla
     names = set()
-    for h in ("mpla.h", "kernel.h", "mpla_kernel.h", "generic_system_adapter.h", "kernel_system_assembler.h"):
+    for h in ("mpla.h", "kernel.h", "mpla_kernel.h", "generic_system_adapter.h", "kernel_system_assembler.h",
+              "mpla_server.h"):
         names |= set(declared(h))
     assert sorted(names) == sorted(mpla.MPLA_SIGNATURES)
 
@@ -80,6 +82,9 @@ STRUCTS = {
     "mpla_vector": None,
     "kernel_matrix_data": None,
     "mpla_kernel_matrix_data": None,
+    "mpla_server_config": None,
+    "mpla_solve_request": None,
+    "mpla_solve_result": None,
 }
 
 
@@ -88,12 +93,13 @@ def test_struct_layout_matches_headers(tmp_path):
     import subprocess
     from mpla_amd import mpla
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "mpla.h"', '#include "kernel.h"',
-             '#include "mpla_kernel.h"', "int main(void) {"]
+             '#include "mpla_kernel.h"', '#include "mpla_server.h"', "int main(void) {"]
     for sname in STRUCTS:
         cls = getattr(mpla, sname)
-        lines.append(f'printf("{sname} size %zu\\n", sizeof(struct {sname}));')
+        tag = "" if sname.startswith(("mpla_server", "mpla_solve")) else "struct "
+        lines.append(f'printf("{sname} size %zu\\n", sizeof({tag}{sname}));')
         for fname, _ in cls._fields_:
-            lines.append(f'printf("{sname}.{fname} %zu\\n", offsetof(struct {sname}, {fname}));')
+            lines.append(f'printf("{sname}.{fname} %zu\\n", offsetof({tag}{sname}, {fname}));')
     lines.append("return 0; }")
     src = tmp_path / "layout.c"
     src.write_text("\n".join(lines))
@@ -121,3 +127,17 @@ def test_block_distribution_host_only():
         assert list(o) == [sum(counts[:p]) for p in range(P)]
     c = (C.c_int * 3)()
     assert lib.mpla_block_distribution(2, 3, c, c) == -1
+
+
+def test_server_request_bytes_host_only():
+    """Admission arithmetic is host-only: device block = points + 6 vectors +
+    slab partials + CG partials, rounded to 1 MiB."""
+    from mpla_amd import mpla
+    lib = mpla.lib()
+    small = lib.mpla_server_request_bytes(1024, 2, 1)
+    big = lib.mpla_server_request_bytes(65536, 3, 8)
+    assert small % (1 << 20) == 0 and small >= (1 << 20)
+    assert big >= 8 * (3 * 65536 + 6 * 65536 * 8)
+    assert big < 64 * (1 << 20)  # ~25 MB of vectors plus partials
+    assert lib.mpla_server_request_bytes(65536, 3, 4) < big
+    assert lib.mpla_server_request_bytes(0, 3, 1) == 0
