@@ -55,6 +55,248 @@ def percentile(xs, q):
     return xs[lo] + (xs[hi] - xs[lo]) * (k - lo)
 
 
+def _event_ms(ext, fn, reps):
+    import torch
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(ext)
+    for _ in range(reps):
+        fn()
+    e1.record(ext)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def bench_c1(K, L, inst, stream_ptr, ext):
+    """Dense FP64 CG: A = B B^T/N + I, N = 4096, b ~ U[-1,1], x0 = 0, relative 1e-10,
+    max 2000 iterations; and 100 dgemv y = A x (the 128 MiB matrix stays in the
+    256 MB Infinity Cache, so this is cache bandwidth, not HBM)."""
+    import numpy as np
+    import torch
+    from mpla_amd import _native, mpla
+    n = 4096
+    Bm = torch.empty(n * n, dtype=torch.float64, device="cuda")
+    _native.check(K.mplak_fill_uniform_f64(Bm.data_ptr(), n * n, 1235, 10, 0, -1.0, 1.0, stream_ptr))
+    torch.cuda.synchronize()
+    Bm = Bm.view(n, n).t().contiguous()  # B[i, j] = element i + j*n of the generated column-major array
+    A = mpla.mpla_matrix()
+    L.mpla_init_matrix(C.byref(A), C.byref(inst), n, n)
+    # A = B B^T / N + I, symmetrised exactly so its row- and column-major layouts coincide (input setup, untimed)
+    M = Bm @ Bm.t() / n
+    Ad = (0.5 * (M + M.t()) + torch.eye(n, dtype=torch.float64, device="cuda")).contiguous()
+    del M
+    torch.cuda.synchronize()
+    assert K.mplak_copy_f64(A.data, Ad.data_ptr(), n * n, stream_ptr) == 0
+    b = mpla.mpla_vector()
+    x = mpla.mpla_vector()
+    y = mpla.mpla_vector()
+    for v in (b, x, y):
+        L.mpla_init_vector(C.byref(v), C.byref(inst), n)
+    _native.check(K.mplak_fill_uniform_f64(b.data, n, 1235, 11, 0, -1.0, 1.0, stream_ptr))
+    core = mpla.core("mpla_generic_dgemv_core_dense")
+    Ag = C.cast(C.pointer(A), C.POINTER(mpla.mpla_generic_matrix))
+
+    def solve():
+        _native.check(K.mplak_fill_f64(x.data, n, 0.0, stream_ptr))
+        L.mpla_generic_conjugate_gradient_with_checkpoint_restart(C.byref(b), Ag, C.byref(x), 2001, 1e-10, core, 0,
+                                                                  False, C.byref(inst))
+    solve()
+    torch.cuda.synchronize()
+    reps = 20
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        solve()
+    t = (time.perf_counter() - t0) / reps
+    it, rel = inst.last_iterations, inst.last_rel_res
+    gemv_ms = _event_ms(ext, lambda: L.mpla_dgemv(C.byref(y), C.byref(A), C.byref(x), C.byref(inst)), 100)
+    for v in (b, x, y):
+        L.mpla_free_vector(C.byref(v), C.byref(inst))
+    L.mpla_free_matrix(C.byref(A), C.byref(inst))
+    del Ad, Bm
+    torch.cuda.empty_cache()
+    log(f"C1: dense CG {t * 1e3:.3f} ms/solve ({it} iterations, rel {rel:.2e}); dgemv {gemv_ms * 1e3:.1f} us")
+    return {"solves_per_s": 1.0 / t, "ms_per_solve": t * 1e3, "iterations": it, "rel_residual": rel,
+            "dgemv_us": gemv_ms * 1e3, "dgemv_gbps_cache": (8.0 * n * n + 16 * n) / (gemv_ms * 1e-3) / 1e9}
+
+
+def bench_c4(K, L, inst, stream_ptr, ext):
+    """dgemv at N = 131072 (137 GB matrix) and BiCGSTAB on A = D + R, N = 32768,
+    R ~ U[-1,1]/sqrt(N), D = diag(2 + U[0,1]), relative 1e-10 (reference
+    3-product iteration with the true-residual test)."""
+    import torch
+    from mpla_amd import _native, mpla
+    out = {}
+    n = 131072
+    try:
+        Ad = torch.empty(n * n, dtype=torch.float64, device="cuda")
+    except RuntimeError as exc:  # pragma: no cover - device without 140 GB free
+        out["dgemv_131072"] = f"skipped: {exc}"
+        Ad = None
+    if Ad is not None:
+        xd = torch.empty(n, dtype=torch.float64, device="cuda")
+        yd = torch.empty(n, dtype=torch.float64, device="cuda")
+        _native.check(K.mplak_fill_uniform_f64(Ad.data_ptr(), n * n, 1234, 1, 0, -1.0, 1.0, stream_ptr))
+        _native.check(K.mplak_fill_uniform_f64(xd.data_ptr(), n, 1234, 2, 0, -1.0, 1.0, stream_ptr))
+        gwb = K.mplak_dgemv_workspace_bytes(n, n)
+        gw = torch.empty(gwb // 8 + 1, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        f = lambda: _native.check(K.mplak_dgemv_f64(n, n, Ad.data_ptr(), n, xd.data_ptr(), yd.data_ptr(),
+                                                    gw.data_ptr(), gwb, None, stream_ptr))
+        f()
+        ms = _event_ms(ext, f, 10)
+        gb = 8.0 * n * n + 16.0 * n
+        out["dgemv_131072"] = {"ms": ms, "gbps": gb / (ms * 1e-3) / 1e9, "frac_hbm": gb / (ms * 1e-3) / PEAK_HBM}
+        log(f"C4: dgemv n={n}: {ms:.3f} ms, {gb / (ms * 1e-3) / 1e9:.1f} GB/s")
+        del Ad, xd, yd, gw
+        torch.cuda.empty_cache()
+    n = 32768
+    A = mpla.mpla_matrix()
+    L.mpla_init_matrix(C.byref(A), C.byref(inst), n, n)
+    _native.check(K.mplak_fill_uniform_f64(A.data, n * n, 1235, 20, 0, -1.0, 1.0, stream_ptr))
+    torch.cuda.synchronize()
+    # scale R by 1/sqrt(N) and add the diagonal (input setup, untimed)
+    Dg = torch.empty(n, dtype=torch.float64, device="cuda")
+    _native.check(K.mplak_fill_uniform_f64(Dg.data_ptr(), n, 1235, 21, 0, 2.0, 3.0, stream_ptr))
+    Mat = torch.empty(n * n, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    assert K.mplak_copy_f64(Mat.data_ptr(), A.data, n * n, stream_ptr) == 0
+    torch.cuda.synchronize()
+    Mv = Mat.view(n, n)
+    Mv.mul_(1.0 / math.sqrt(n))
+    Mv.diagonal().add_(Dg)
+    torch.cuda.synchronize()
+    assert K.mplak_copy_f64(A.data, Mat.data_ptr(), n * n, stream_ptr) == 0
+    torch.cuda.synchronize()
+    del Mat, Mv, Dg
+    torch.cuda.empty_cache()
+    b = mpla.mpla_vector()
+    x = mpla.mpla_vector()
+    for v in (b, x):
+        L.mpla_init_vector(C.byref(v), C.byref(inst), n)
+    _native.check(K.mplak_fill_uniform_f64(b.data, n, 1235, 22, 0, -1.0, 1.0, stream_ptr))
+    core = mpla.core("mpla_generic_dgemv_core_dense")
+    Ag = C.cast(C.pointer(A), C.POINTER(mpla.mpla_generic_matrix))
+
+    def solve():
+        _native.check(K.mplak_fill_f64(x.data, n, 0.0, stream_ptr))
+        L.mpla_generic_BiCGSTAB_with_checkpoint_restart(C.byref(b), Ag, C.byref(x), 2001, 1e-10, core, 0, False,
+                                                        C.byref(inst))
+    solve()
+    torch.cuda.synchronize()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        solve()
+    t = (time.perf_counter() - t0) / reps
+    it, rel, mv = inst.last_iterations, inst.last_rel_res, inst.last_matvecs
+    gb = mv * 8.0 * n * n
+    out["bicgstab_32768"] = {"ms_per_solve": t * 1e3, "solves_per_s": 1.0 / t, "iterations": it,
+                             "rel_residual": rel, "products": mv, "effective_gbps": gb / t / 1e9}
+    log(f"C4: BiCGSTAB n={n}: {t * 1e3:.2f} ms/solve, {it} iterations, rel {rel:.2e}, {gb / t / 1e9:.0f} GB/s eff")
+    for v in (b, x):
+        L.mpla_free_vector(C.byref(v), C.byref(inst))
+    L.mpla_free_matrix(C.byref(A), C.byref(inst))
+    return out
+
+
+def bench_c3(K, L, inst, stream_ptr, args):
+    """Multi-RHS Matern-3/2 system: N = 131072 points U[0,1]^3, l = 0.1, nugget
+    0.5, 64 right-hand sides ~ N(0,1), block CG with the shared K*V product in
+    the given precision inside FP64 refinement to 1e-8."""
+    import numpy as np
+    import torch
+    from mpla_amd import _native, inputs, mpla
+    n, R = 131072, 64
+    P = torch.empty(3, n, dtype=torch.float64, device="cuda")
+    _native.check(K.mplak_fill_uniform_f64(P.data_ptr(), 3 * n, 1235, 30, 0, 0.0, 1.0, stream_ptr))
+    table = torch.tensor([P[d].data_ptr() for d in range(3)], dtype=torch.int64, device="cuda")
+    Bh = inputs.normal(n * R, 1235, 31)
+    Bd = torch.as_tensor(Bh, device="cuda")
+    Xd = torch.zeros(n * R, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    kd = mpla.mpla_kernel_matrix_data(table.data_ptr(), 3, 0, mpla.MPLA_KERNEL_MATERN32, mpla.MPLA_PREC_F64, 0.1, 0.5,
+                                      0.0)
+    A = mpla.mpla_generic_matrix()
+    L.mpla_init_generic_matrix(C.byref(A), C.byref(inst), n, n)
+    A.data = C.addressof(kd)
+    out = {}
+    names = {"bf16": mpla.MPLA_PREC_BF16, "bf16x3": mpla.MPLA_PREC_BF16X3, "f32": mpla.MPLA_PREC_F32,
+             "f64": mpla.MPLA_PREC_F64}
+    sols = {}
+    for name in [p for p in args.c3_precisions.split(",") if p]:
+        Xd.zero_()
+        torch.cuda.synchronize()
+        rel = (C.c_double * R)()
+        t0 = time.perf_counter()
+        L.mpla_kernel_block_conjugate_gradient_mixed(Bd.data_ptr(), Xd.data_ptr(), R, C.byref(A), 20000, 1e-8,
+                                                     names[name], rel, C.byref(inst))
+        t = time.perf_counter() - t0
+        sols[name] = Xd.cpu().numpy().reshape((n, R), order="F")
+        out[name] = {"seconds": t, "rhs_solves_per_s": R / t, "shared_products": inst.last_iterations,
+                     "max_true_rel_residual": max(rel), "converged": bool(inst.last_converged)}
+        log(f"C3 {name}: {t:.2f} s for {R} RHS, {inst.last_iterations} shared products, max rel {max(rel):.2e}")
+    if args.c3_oracle_columns and sols:
+        x = mpla.mpla_vector()
+        bv = mpla.mpla_vector()
+        L.mpla_init_vector(C.byref(x), C.byref(inst), n)
+        L.mpla_init_vector(C.byref(bv), C.byref(inst), n)
+        errs = {k: [] for k in sols}
+        for c in (0, 21, 42, 63):
+            _native.check(K.mplak_copy_f64(bv.data, Bd.data_ptr() + 8 * c * n, n, stream_ptr))
+            _native.check(K.mplak_fill_f64(x.data, n, 0.0, stream_ptr))
+            L.mpla_kernel_conjugate_gradient_mixed(C.byref(bv), C.byref(A), C.byref(x), 20000, 1e-12,
+                                                   mpla.MPLA_PREC_F64, C.byref(inst))
+            xo = np.empty(n)
+            L.mpla_copy_distributed_vector_to_cpu(xo.ctypes.data, C.byref(x), C.byref(inst))
+            for k, X in sols.items():
+                errs[k].append(float(np.linalg.norm(X[:, c] - xo) / np.linalg.norm(xo)))
+        for k in sols:
+            out[k]["rel_error_vs_fp64_columns_0_21_42_63"] = errs[k]
+        L.mpla_free_vector(C.byref(x), C.byref(inst))
+        L.mpla_free_vector(C.byref(bv), C.byref(inst))
+    L.mpla_free_generic_matrix(C.byref(A), C.byref(inst))
+    del P, Bd, Xd
+    torch.cuda.empty_cache()
+    return out
+
+
+def bench_c5(args):
+    """Served stream: Poisson arrivals of kernel-interpolation solves (N
+    log-uniform in [1024, 65536], d in {2,3}, Gaussian l=0.05 or Matern-3/2
+    l=0.1, nugget U[0.5,1], 1-8 RHS, tolerance 1e-8, HBM budget 200 GB).  The
+    arrival rate is the plain FP64 one-at-a-time path's measured capacity
+    (all requests offered at t=0), so the plain path is offered ~1x its
+    capacity."""
+    from mpla_amd import mpla, serving
+    count = args.c5_requests
+    reqs = serving.request_stream(count, rate=1.0, seed=1235)
+    arrivals = [r["arrival"] for r in reqs]
+    # capacity of the plain path: all requests offered at once
+    with serving.Server(mode=mpla.MPLA_SERVE_FP64_SERIAL) as srv:
+        t0 = time.perf_counter()
+        res_sat, _ = srv.run_trace([dict(r, arrival=0.0) for r in reqs])
+        cap = count / (time.perf_counter() - t0)
+    rate = cap
+    for r, a in zip(reqs, arrivals):
+        r["arrival"] = a / rate  # rescale unit-rate arrivals to `rate`
+    out = {"requests": count, "arrival_rate_per_s": rate, "plain_capacity_solves_per_s": cap,
+           "mean_n": float(sum(r["n"] for r in reqs) / count)}
+    for name, mode in (("mixed_batched", mpla.MPLA_SERVE_MIXED_BATCHED), ("fp64_serial", mpla.MPLA_SERVE_FP64_SERIAL)):
+        with serving.Server(mode=mode) as srv:
+            t0 = time.perf_counter()
+            res, _ = srv.run_trace(reqs)
+            span = time.perf_counter() - t0
+        ok = sum(1 for x in res if x.status == mpla.MPLA_REQUEST_CONVERGED)
+        st = serving.latency_stats(res)
+        out[name] = {"solves_per_s": ok / span, "makespan_s": span, "converged": ok,
+                     "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st.get("mean_ms"),
+                     "max_true_rel_residual": max(x.max_rel_residual for x in res),
+                     "iterations": sum(x.iterations for x in res)}
+        log(f"C5 {name}: {ok}/{count} converged, {ok / span:.2f} solves/s, p50 {st['p50_ms']:.1f} ms, "
+            f"p99 {st['p99_ms']:.1f} ms")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -64,6 +306,11 @@ def main():
     ap.add_argument("--cpu-matvecs", type=int, default=2)
     ap.add_argument("--skip-dgemv", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-configs", action="store_true", help="only the headline workload")
+    ap.add_argument("--c5-requests", type=int, default=48)
+    ap.add_argument("--c3-precisions", default="bf16x3,bf16")
+    ap.add_argument("--c3-oracle-columns", action="store_true",
+                    help="also solve columns 0,21,42,63 with plain FP64 CG for the error report")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path runs on a 1x1 "
@@ -240,6 +487,13 @@ def main():
                          f"FP64 CG solve; entries/s {entries / tcpu:.3e}"}
         log(f"cpu oracle: {tcpu:.2f} s per product on {oracle.threads()} threads")
 
+    configs = {}
+    if not args.skip_configs:
+        configs["c1_dense_cg"] = bench_c1(K, L, inst, stream_ptr, ext)
+        configs["c4_dense"] = bench_c4(K, L, inst, stream_ptr, ext)
+        configs["c3_block"] = bench_c3(K, L, inst, stream_ptr, args)
+        configs["c5_served"] = bench_c5(args)
+
     value = args.steps / wall
     out = {
         "metric": "kernel-CG solves/sec + p50/p99 latency @1e-8 rel residual; dgemv HBM GB/s",
@@ -274,6 +528,7 @@ def main():
                                  "entries_per_s": entries / t64},
         "dgemv": dgemv,
         "cpu_baseline": cpu,
+        "configs": configs,
     }
     if dgemv is not None:
         out["roofline_dgemv"] = {"bound": "hbm", "achieved": dgemv["gbps"], "peak": PEAK_HBM / 1e9, "unit": "GB/s",
