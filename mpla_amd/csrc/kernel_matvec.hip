@@ -16,6 +16,7 @@
 // into an FP64 accumulator.
 
 #include <math.h>
+#include <stdlib.h>
 
 #include "common.hpp"
 
@@ -232,6 +233,226 @@ void dispatch_dim(dim3 grid, hipStream_t st, bool coinc, const mplak_kernel_desc
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// Symmetric product for a square block whose rows and columns are the same
+// points (row range == column range): K_ij = K_ji bitwise, so every
+// off-diagonal entry is evaluated once and used for both y_i += K_ij x_j and
+// y_j += K_ij x_i.
+//
+// Workgroup p owns the tile pair (I, J), I <= J, tiles of 512 points
+// (256 threads x 2 targets).  The I-side sum is the plain loop over the 512
+// sources of J.  For the J-side, wave w handles its 128 targets against the
+// sources in 8 groups of 64: at step s lane l takes source (l + s) mod 64 of
+// the group (from LDS, conflict-free) and adds its two entries times x_i to a
+// register t that then moves one lane down the wave (DPP wave_rol:1), so t
+// always belongs to the source the lane works on next; after 64 steps lane l
+// holds the whole wave's sum for source l.  The four waves' sums are added in
+// wave order.  Diagonal pairs (I == J) use the plain loop only.  Partials
+// go to part_i[p][512] and part_j[p][512]; the finish kernel adds, for every
+// row, the partials of its 512-tile in partner order.  All orders are fixed,
+// so the product is deterministic.
+//
+// In FP32 each group of 64 sources is stored twice in a row in LDS, so the
+// rotated read (l + s) mod 64 becomes group[l + s], an immediate offset from
+// one per-lane address (no index arithmetic in the loop).  In FP64 the
+// exponential dominates and the group is stored once.
+// ---------------------------------------------------------------------------
+
+constexpr int kSymTile = 2 * kBlock;  // 512 points per tile
+constexpr int kSymMinN = 16384;       // below this the plain kernel fills the GPU better
+
+// MPLAK_KMV_SYMMETRIC=0 turns the symmetric product off, =1 uses it for every
+// eligible block regardless of size; unset: blocks of at least kSymMinN points.
+inline int sym_mode() {
+  const char* v = getenv("MPLAK_KMV_SYMMETRIC");
+  if (v == nullptr || *v == '\0') return -1;
+  return atoi(v) != 0 ? 1 : 0;
+}
+
+template <typename T>
+struct Quad;
+template <>
+struct Quad<float> {
+  typedef float4 type;
+};
+template <>
+struct Quad<double> {
+  typedef double4 type;
+};
+
+__device__ __forceinline__ float wave_rotate(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x134 /* wave_rol:1 */, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double wave_rotate(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x134, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x134, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ int64_t sym_pair_index(int64_t I, int64_t J) { return J * (J + 1) / 2 + I; }
+
+template <typename T, int KIND, int DIM, bool COINC>
+__global__ __launch_bounds__(kBlock) void kmv_sym_kernel(
+    const double* const* __restrict__ points, int64_t off, int64_t n, double scale, double coinc_thr,
+    double coinc_shift, const double* __restrict__ x, double* __restrict__ part_i, double* __restrict__ part_j,
+    const int32_t* __restrict__ skip) {
+  if (skip != nullptr && *skip != 0) return;
+  typedef typename Quad<T>::type Q;
+  constexpr int kDup = sizeof(T) == 4 ? 2 : 1;  // copies of each 64-source group
+  constexpr int kGroups = kSymTile / kWave;
+  __shared__ Q src[kGroups][kDup * kWave];
+  __shared__ T red[kBlock / kWave][kSymTile];
+
+  const int64_t p = blockIdx.x;
+  int64_t J = (int64_t)((sqrt(8.0 * (double)p + 1.0) - 1.0) * 0.5);
+  while (J * (J + 1) / 2 > p) --J;
+  while ((J + 1) * (J + 2) / 2 <= p) ++J;
+  const int64_t I = p - J * (J + 1) / 2;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1);
+  const int w = tid / kWave;
+  const T thr = (T)coinc_thr, shift = (T)coinc_shift;
+
+  // two targets per lane: tile offsets w*128 + lane and w*128 + 64 + lane
+  T pi[2][DIM], xi[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int64_t i = I * kSymTile + w * 2 * kWave + h * kWave + lane;
+    const int64_t ic = i < n ? i : n - 1;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) pi[h][d] = (T)(points[d][off + ic] * scale);
+    xi[h] = i < n ? (T)x[i] : T(0);
+  }
+  for (int k = tid; k < kSymTile; k += kBlock) {
+    const int64_t j = J * kSymTile + k;
+    const int64_t jc = j < n ? j : n - 1;
+    Q q;
+    q.x = (T)(points[0][off + jc] * scale);
+    q.y = DIM > 1 ? (T)(points[1][off + jc] * scale) : T(0);
+    q.z = DIM > 2 ? (T)(points[2][off + jc] * scale) : T(0);
+    q.w = j < n ? (T)x[j] : T(0);
+#pragma unroll
+    for (int c = 0; c < kDup; ++c) src[k / kWave][c * kWave + (k & (kWave - 1))] = q;
+  }
+  __syncthreads();
+
+  T accI[2] = {T(0), T(0)};
+  auto entry2 = [&](const Q& q, T (&e)[2]) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const T dx = pi[h][0] - q.x;
+      T s = dx * dx;
+      if (DIM > 1) {
+        const T dy = pi[h][1] - q.y;
+        s = fma(dy, dy, s);
+      }
+      if (DIM > 2) {
+        const T dz = pi[h][2] - q.z;
+        s = fma(dz, dz, s);
+      }
+      T k = kernel_entry<T, KIND>(s);
+      if (COINC) k += (s < thr) ? shift : T(0);
+      e[h] = k;
+    }
+  };
+
+  if (I == J) {
+#pragma unroll 1
+    for (int g = 0; g < kGroups; ++g) {
+#pragma unroll 4
+      for (int s = 0; s < kWave; ++s) {
+        const Q q = src[g][s];
+        T e[2];
+        entry2(q, e);
+        accI[0] = fma(e[0], q.w, accI[0]);
+        accI[1] = fma(e[1], q.w, accI[1]);
+      }
+    }
+  } else {
+#pragma unroll 1
+    for (int g = 0; g < kGroups; ++g) {
+      T t = T(0);
+#pragma unroll 4
+      for (int s = 0; s < kWave; ++s) {
+        const Q q = kDup == 2 ? src[g][lane + s] : src[g][(lane + s) & (kWave - 1)];
+        T e[2];
+        entry2(q, e);
+        accI[0] = fma(e[0], q.w, accI[0]);
+        accI[1] = fma(e[1], q.w, accI[1]);
+        t = fma(e[0], xi[0], t);
+        t = fma(e[1], xi[1], t);
+        t = wave_rotate(t);
+      }
+      red[w][g * kWave + lane] = t;
+    }
+  }
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int k = w * 2 * kWave + h * kWave + lane;
+    part_i[p * kSymTile + k] = (double)accI[h];
+  }
+  if (I != J) {
+    __syncthreads();
+    for (int k = tid; k < kSymTile; k += kBlock) {
+      double s = (double)red[0][k];
+#pragma unroll
+      for (int q = 1; q < kBlock / kWave; ++q) s += (double)red[q][k];
+      part_j[p * kSymTile + k] = s;
+    }
+  }
+}
+
+// y_i = sum over partner tiles P (ascending) of the tile's partials, plus the nugget term.
+__global__ __launch_bounds__(kBlock) void kmv_sym_finish_kernel(int64_t n, int64_t ntiles,
+                                                                const double* __restrict__ part_i,
+                                                                const double* __restrict__ part_j, double nugget,
+                                                                const double* __restrict__ x, double* __restrict__ y,
+                                                                const int32_t* __restrict__ skip) {
+  if (skip != nullptr && *skip != 0) return;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const int64_t X = i / kSymTile, t = i % kSymTile;
+    double s = 0.0;
+    for (int64_t P = 0; P < ntiles; ++P)
+      s += P < X ? part_j[sym_pair_index(P, X) * kSymTile + t] : part_i[sym_pair_index(X, P) * kSymTile + t];
+    if (nugget != 0.0) s = fma(nugget, x[i], s);
+    y[i] = s;
+  }
+}
+
+inline bool use_symmetric(const mplak_kernel_desc* d) {
+  if (d->row_offset != d->col_offset || d->row_count != d->col_count || d->dim > 3) return false;
+  const int mode = sym_mode();
+  return mode < 0 ? d->row_count >= kSymMinN : mode == 1;
+}
+
+inline int64_t sym_tiles(int64_t n) { return (n + kSymTile - 1) / kSymTile; }
+
+template <typename T, int KIND, int DIM>
+void launch_sym_dim(hipStream_t st, bool coinc, const mplak_kernel_desc* d, double scale, double thr, const double* x,
+                    double* pi, double* pj, const int32_t* skip) {
+  const int64_t nt = sym_tiles(d->row_count);
+  const dim3 grid((unsigned)(nt * (nt + 1) / 2));
+  if (coinc)
+    hipLaunchKernelGGL((kmv_sym_kernel<T, KIND, DIM, true>), grid, dim3(kBlock), 0, st, d->points, d->row_offset,
+                       d->row_count, scale, thr, d->coincident_shift, x, pi, pj, skip);
+  else
+    hipLaunchKernelGGL((kmv_sym_kernel<T, KIND, DIM, false>), grid, dim3(kBlock), 0, st, d->points, d->row_offset,
+                       d->row_count, scale, thr, d->coincident_shift, x, pi, pj, skip);
+}
+
+template <typename T, int KIND>
+void launch_sym(hipStream_t st, bool coinc, const mplak_kernel_desc* d, double scale, double thr, const double* x,
+                double* pi, double* pj, const int32_t* skip) {
+  switch (d->dim) {
+    case 1: launch_sym_dim<T, KIND, 1>(st, coinc, d, scale, thr, x, pi, pj, skip); break;
+    case 2: launch_sym_dim<T, KIND, 2>(st, coinc, d, scale, thr, x, pi, pj, skip); break;
+    default: launch_sym_dim<T, KIND, 3>(st, coinc, d, scale, thr, x, pi, pj, skip); break;
+  }
+}
+
 int check_desc(const mplak_kernel_desc* d) {
   if (d == nullptr || d->points == nullptr) return MPLAK_ERR_ARG;
   if (d->dim < 1 || d->dim > kKmvMaxDim) return MPLAK_ERR_ARG;
@@ -251,6 +472,10 @@ extern "C" {
 
 size_t mplak_kernel_matvec_workspace_bytes(const mplak_kernel_desc* d) {
   if (d == nullptr || d->row_count <= 0 || d->col_count <= 0) return 0;
+  if (use_symmetric(d)) {
+    const int64_t nt = sym_tiles(d->row_count);
+    return (size_t)2 * (size_t)(nt * (nt + 1) / 2) * kSymTile * sizeof(double);
+  }
   const KmvPlan p = kmv_plan(d->row_count, d->col_count);
   return (size_t)p.slabs * (size_t)d->row_count * sizeof(double);
 }
@@ -267,11 +492,34 @@ int mplak_kernel_matvec(const mplak_kernel_desc* d, int precision, const double*
   const size_t need = mplak_kernel_matvec_workspace_bytes(d);
   if (work == nullptr || work_bytes < need) return MPLAK_ERR_WORKSPACE;
 
-  const KmvPlan p = kmv_plan(d->row_count, d->col_count);
   const double scale = coord_scale(d->kind, precision, d->length_scale);
   const bool coinc = d->coincident_shift != 0.0;
   const double thr = (1e-14 * scale) * (1e-14 * scale);
   double* part = static_cast<double*>(work);
+  if (use_symmetric(d)) {
+    const int64_t nt = sym_tiles(d->row_count);
+    double* pj = part + (size_t)(nt * (nt + 1) / 2) * kSymTile;
+    hipStream_t hs0 = as_stream(s);
+    if (precision == MPLAK_PREC_F64) {
+      if (d->kind == MPLAK_KERNEL_GAUSSIAN)
+        launch_sym<double, MPLAK_KERNEL_GAUSSIAN>(hs0, coinc, d, scale, thr, x, part, pj, skip_if);
+      else
+        launch_sym<double, MPLAK_KERNEL_MATERN32>(hs0, coinc, d, scale, thr, x, part, pj, skip_if);
+    } else {
+      if (d->kind == MPLAK_KERNEL_GAUSSIAN)
+        launch_sym<float, MPLAK_KERNEL_GAUSSIAN>(hs0, coinc, d, scale, thr, x, part, pj, skip_if);
+      else
+        launch_sym<float, MPLAK_KERNEL_MATERN32>(hs0, coinc, d, scale, thr, x, part, pj, skip_if);
+    }
+    st = launch_status();
+    if (st != MPLAK_OK) return st;
+    int64_t fb = (d->row_count + kBlock - 1) / kBlock;
+    if (fb > 4096) fb = 4096;
+    hipLaunchKernelGGL(kmv_sym_finish_kernel, dim3((unsigned)fb), dim3(kBlock), 0, hs0, d->row_count, nt, part, pj,
+                       d->nugget, x, y, skip_if);
+    return launch_status();
+  }
+  const KmvPlan p = kmv_plan(d->row_count, d->col_count);
   const dim3 grid((unsigned)p.row_tiles, (unsigned)p.slabs);
   hipStream_t hs = as_stream(s);
   if (precision == MPLAK_PREC_F64) {

@@ -245,7 +245,8 @@ def test_kernel_matvec_f32_tolerance(lib, oracle, kind, l):
 
 
 def test_kernel_matvec_full_size_linearity(lib, oracle):
-    # the C2 operator at full size: linearity K(a x + b y) = a Kx + b Ky
+    # Gaussian operator, N = 65536, l = 0.05, nugget 1 (symmetric path by size):
+    # linearity K(a x + b y) = a Kx + b Ky
     n = 65536
     pts = oracle.uniform(3 * n, 1235, 0).reshape(3, n)
     P = dev(pts)
@@ -264,6 +265,105 @@ def test_kernel_matvec_full_size_linearity(lib, oracle):
         ref = k.matvec(x, mode=1, row_off=int(i), rows=1)[0]
         bound = k.matvec(np.abs(x), mode=1, row_off=int(i), rows=1)[0]
         assert abs(kx[i] - ref) <= 1e-12 * bound
+
+
+# --------------------------------------------------------------------------
+# symmetric product (square block, rows == columns): each entry evaluated once
+# and used for row i and row j.  MPLAK_KMV_SYMMETRIC=1 forces it at sizes the
+# oracle checks quickly; unset it is chosen from 16384 points up.
+
+
+@pytest.mark.parametrize("kind,l", [(0, 1 / math.sqrt(2.0)), (0, 0.05), (1, 0.1)])
+@pytest.mark.parametrize("dim", [1, 2, 3])
+@pytest.mark.parametrize("n", [1, 2, 513, 1537])
+def test_kernel_matvec_symmetric_f64_vs_oracle(lib, oracle, monkeypatch, kind, l, dim, n):
+    monkeypatch.setenv("MPLAK_KMV_SYMMETRIC", "1")
+    pts, x = kmv_case(oracle, n, dim, kind, l, 0.0, 0.0, 4000 + 7 * dim + n)
+    P = dev(pts)
+    T = point_table(P)
+    y = host(kmv_gpu(lib, kernel_desc(P, T, kind, l, nugget=0.5), dev(x)))
+    k = oracle.Kernel(pts, kind, l, nugget=0.5)
+    ref = k.matvec(x, mode=1)
+    assert rel_err(y, ref) <= 1e-12
+    bound = k.matvec(np.abs(x), mode=1)
+    assert np.all(np.abs(y - ref) <= 1e-12 * bound)
+
+
+@pytest.mark.parametrize("kind,l", [(0, 0.05), (1, 0.1)])
+@pytest.mark.parametrize("dim", [1, 3])
+def test_kernel_matvec_symmetric_f32_tolerance(lib, oracle, monkeypatch, kind, l, dim):
+    monkeypatch.setenv("MPLAK_KMV_SYMMETRIC", "1")
+    n = 3000
+    pts, x = kmv_case(oracle, n, dim, kind, l, 0, 0, 31 + dim)
+    P = dev(pts)
+    T = point_table(P)
+    y = host(kmv_gpu(lib, kernel_desc(P, T, kind, l, nugget=0.5), dev(x), precision=1))
+    ref = oracle.Kernel(pts, kind, l, nugget=0.5).matvec(x, mode=1)
+    assert rel_err(y, ref) <= 2e-5
+
+
+def test_kernel_matvec_symmetric_coincident_points(lib, oracle, monkeypatch):
+    # duplicates inside one tile, across the two tiles of a pair, and in the
+    # padded last tile: the 1e-8 shift lands on exactly those entries
+    monkeypatch.setenv("MPLAK_KMV_SYMMETRIC", "1")
+    n = 1100
+    pts = oracle.uniform(3 * n, 78, 0).reshape(3, n)
+    for a, b in [(10, 20), (30, 700), (600, 1099)]:
+        pts[:, a] = pts[:, b]
+    x = oracle.uniform(n, 78, 1, -1, 1)
+    P = dev(pts)
+    T = point_table(P)
+    for prec, tol in [(0, 1e-12), (1, 2e-5)]:
+        y = host(kmv_gpu(lib, kernel_desc(P, T, 0, 1 / math.sqrt(2.0), shift=1e-8), dev(x), precision=prec))
+        ref = oracle.Kernel(pts, 0, 1 / math.sqrt(2.0), coincident_shift=1e-8).matvec(x, mode=1)
+        assert rel_err(y, ref) <= tol
+    y = host(kmv_gpu(lib, kernel_desc(P, T, 0, 1 / math.sqrt(2.0), shift=1e-8), dev(x)))
+    y0 = host(kmv_gpu(lib, kernel_desc(P, T, 0, 1 / math.sqrt(2.0)), dev(x)))
+    for a, b in [(10, 20), (30, 700), (600, 1099)]:
+        assert abs((y - y0)[a] - 1e-8 * (x[a] + x[b])) < 1e-14
+
+
+def test_kernel_matvec_symmetric_equals_plain_and_is_deterministic(lib, oracle, monkeypatch):
+    n = 20000  # 40 tiles, last one ragged; above the automatic threshold
+    pts, x = kmv_case(oracle, n, 3, 0, 0.05, 1.0, 0, 91)
+    P = dev(pts)
+    T = point_table(P)
+    d = kernel_desc(P, T, 0, 0.05, nugget=1.0)
+    X = dev(x)
+    monkeypatch.setenv("MPLAK_KMV_SYMMETRIC", "0")
+    plain = {p: host(kmv_gpu(lib, d, X, precision=p)) for p in (0, 1)}
+    monkeypatch.delenv("MPLAK_KMV_SYMMETRIC")
+    assert lib.mplak_kernel_matvec_workspace_bytes(C.byref(d)) == 2 * (40 * 41 // 2) * 512 * 8
+    for p, tol in [(0, 1e-13), (1, 2e-5)]:
+        a = host(kmv_gpu(lib, d, X, precision=p))
+        b = host(kmv_gpu(lib, d, X, precision=p))
+        assert np.array_equal(a, b)
+        assert rel_err(a, plain[p]) <= tol
+    ref = oracle.Kernel(pts, 0, 0.05, nugget=1.0).matvec(x, mode=1, row_off=19000, rows=1000)
+    assert rel_err(a[19000:], ref) <= 2e-5
+
+
+def test_kernel_matvec_symmetric_skip_flag(lib, oracle, monkeypatch):
+    import torch
+    monkeypatch.setenv("MPLAK_KMV_SYMMETRIC", "1")
+    n = 1500
+    pts, x = kmv_case(oracle, n, 2, 1, 0.2, 0, 0, 5)
+    P = dev(pts)
+    T = point_table(P)
+    d = kernel_desc(P, T, 1, 0.2)
+    y = dev(np.full(n, 7.0))
+    wb = lib.mplak_kernel_matvec_workspace_bytes(C.byref(d))
+    w = workspace(wb)
+    flag = torch.ones(1, dtype=torch.int32, device="cuda")
+    X = dev(x)
+    ck(lib.mplak_kernel_matvec(C.byref(d), 0, X.data_ptr(), y.data_ptr(), w.data_ptr(), wb,
+                               flag.data_ptr(), stream()))
+    assert np.all(host(y) == 7.0)
+    flag.zero_()
+    ck(lib.mplak_kernel_matvec(C.byref(d), 0, X.data_ptr(), y.data_ptr(), w.data_ptr(), wb, flag.data_ptr(),
+                               stream()))
+    ref = oracle.Kernel(pts, 1, 0.2).matvec(x, mode=1)
+    assert rel_err(host(y), ref) <= 1e-12
 
 
 def test_gaussian_rhs_vs_oracle(lib, oracle):
