@@ -42,6 +42,13 @@ PEAK_FP64 = 78.6e12
 PEAK_HBM = 8.0e12
 
 
+def sym_pairs(n, tile=512):
+    """Entry evaluations of the symmetric product: every 512 x 512 tile pair
+    I <= J once (diagonal tiles evaluate their full square)."""
+    nt = (n + tile - 1) // tile
+    return tile * tile * nt * (nt + 1) // 2
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -67,7 +74,7 @@ def _event_ms(ext, fn, reps):
     return e0.elapsed_time(e1) / reps
 
 
-def bench_c1(K, L, inst, stream_ptr, ext):
+def bench_dense_cg(K, L, inst, stream_ptr, ext):
     """Dense FP64 CG: A = B B^T/N + I, N = 4096, b ~ U[-1,1], x0 = 0, relative 1e-10,
     max 2000 iterations; and 100 dgemv y = A x (the 128 MiB matrix stays in the
     256 MB Infinity Cache, so this is cache bandwidth, not HBM)."""
@@ -114,12 +121,12 @@ def bench_c1(K, L, inst, stream_ptr, ext):
     L.mpla_free_matrix(C.byref(A), C.byref(inst))
     del Ad, Bm
     torch.cuda.empty_cache()
-    log(f"C1: dense CG {t * 1e3:.3f} ms/solve ({it} iterations, rel {rel:.2e}); dgemv {gemv_ms * 1e3:.1f} us")
+    log(f"dense CG n=4096: {t * 1e3:.3f} ms/solve ({it} iterations, rel {rel:.2e}); dgemv {gemv_ms * 1e3:.1f} us")
     return {"solves_per_s": 1.0 / t, "ms_per_solve": t * 1e3, "iterations": it, "rel_residual": rel,
             "dgemv_us": gemv_ms * 1e3, "dgemv_gbps_cache": (8.0 * n * n + 16 * n) / (gemv_ms * 1e-3) / 1e9}
 
 
-def bench_c4(K, L, inst, stream_ptr, ext):
+def bench_dense_large(K, L, inst, stream_ptr, ext):
     """dgemv at N = 131072 (137 GB matrix) and BiCGSTAB on A = D + R, N = 32768,
     R ~ U[-1,1]/sqrt(N), D = diag(2 + U[0,1]), relative 1e-10 (reference
     3-product iteration with the true-residual test)."""
@@ -146,7 +153,7 @@ def bench_c4(K, L, inst, stream_ptr, ext):
         ms = _event_ms(ext, f, 10)
         gb = 8.0 * n * n + 16.0 * n
         out["dgemv_131072"] = {"ms": ms, "gbps": gb / (ms * 1e-3) / 1e9, "frac_hbm": gb / (ms * 1e-3) / PEAK_HBM}
-        log(f"C4: dgemv n={n}: {ms:.3f} ms, {gb / (ms * 1e-3) / 1e9:.1f} GB/s")
+        log(f"dgemv n={n}: {ms:.3f} ms, {gb / (ms * 1e-3) / 1e9:.1f} GB/s")
         del Ad, xd, yd, gw
         torch.cuda.empty_cache()
     n = 32768
@@ -192,14 +199,14 @@ def bench_c4(K, L, inst, stream_ptr, ext):
     gb = mv * 8.0 * n * n
     out["bicgstab_32768"] = {"ms_per_solve": t * 1e3, "solves_per_s": 1.0 / t, "iterations": it,
                              "rel_residual": rel, "products": mv, "effective_gbps": gb / t / 1e9}
-    log(f"C4: BiCGSTAB n={n}: {t * 1e3:.2f} ms/solve, {it} iterations, rel {rel:.2e}, {gb / t / 1e9:.0f} GB/s eff")
+    log(f"BiCGSTAB n={n}: {t * 1e3:.2f} ms/solve, {it} iterations, rel {rel:.2e}, {gb / t / 1e9:.0f} GB/s eff")
     for v in (b, x):
         L.mpla_free_vector(C.byref(v), C.byref(inst))
     L.mpla_free_matrix(C.byref(A), C.byref(inst))
     return out
 
 
-def bench_c3(K, L, inst, stream_ptr, args):
+def bench_block(K, L, inst, stream_ptr, args):
     """Multi-RHS Matern-3/2 system: N = 131072 points U[0,1]^3, l = 0.1, nugget
     0.5, 64 right-hand sides ~ N(0,1), block CG with the shared K*V product in
     the given precision inside FP64 refinement to 1e-8."""
@@ -223,7 +230,7 @@ def bench_c3(K, L, inst, stream_ptr, args):
     names = {"bf16": mpla.MPLA_PREC_BF16, "bf16x3": mpla.MPLA_PREC_BF16X3, "f32": mpla.MPLA_PREC_F32,
              "f64": mpla.MPLA_PREC_F64}
     sols = {}
-    for name in [p for p in args.c3_precisions.split(",") if p]:
+    for name in [p for p in args.block_precisions.split(",") if p]:
         Xd.zero_()
         torch.cuda.synchronize()
         rel = (C.c_double * R)()
@@ -234,8 +241,8 @@ def bench_c3(K, L, inst, stream_ptr, args):
         sols[name] = Xd.cpu().numpy().reshape((n, R), order="F")
         out[name] = {"seconds": t, "rhs_solves_per_s": R / t, "shared_products": inst.last_iterations,
                      "max_true_rel_residual": max(rel), "converged": bool(inst.last_converged)}
-        log(f"C3 {name}: {t:.2f} s for {R} RHS, {inst.last_iterations} shared products, max rel {max(rel):.2e}")
-    if args.c3_oracle_columns and sols:
+        log(f"Matern block solve {name}: {t:.2f} s for {R} RHS, {inst.last_iterations} shared products, max rel {max(rel):.2e}")
+    if args.block_oracle_columns and sols:
         x = mpla.mpla_vector()
         bv = mpla.mpla_vector()
         L.mpla_init_vector(C.byref(x), C.byref(inst), n)
@@ -260,7 +267,7 @@ def bench_c3(K, L, inst, stream_ptr, args):
     return out
 
 
-def bench_c5(args):
+def bench_served(args):
     """Served stream: Poisson arrivals of kernel-interpolation solves (N
     log-uniform in [1024, 65536], d in {2,3}, Gaussian l=0.05 or Matern-3/2
     l=0.1, nugget U[0.5,1], 1-8 RHS, tolerance 1e-8, HBM budget 200 GB).  The
@@ -268,7 +275,7 @@ def bench_c5(args):
     (all requests offered at t=0), so the plain path is offered ~1x its
     capacity."""
     from mpla_amd import mpla, serving
-    count = args.c5_requests
+    count = args.served_requests
     reqs = serving.request_stream(count, rate=1.0, seed=1235)
     arrivals = [r["arrival"] for r in reqs]
     # capacity of the plain path: all requests offered at once
@@ -292,9 +299,59 @@ def bench_c5(args):
                      "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st.get("mean_ms"),
                      "max_true_rel_residual": max(x.max_rel_residual for x in res),
                      "iterations": sum(x.iterations for x in res)}
-        log(f"C5 {name}: {ok}/{count} converged, {ok / span:.2f} solves/s, p50 {st['p50_ms']:.1f} ms, "
+        log(f"served stream {name}: {ok}/{count} converged, {ok / span:.2f} solves/s, p50 {st['p50_ms']:.1f} ms, "
             f"p99 {st['p99_ms']:.1f} ms")
     return out
+
+
+def product_desc(table):
+    from mpla_amd import _native
+    desc = _native.mplak_kernel_desc()
+    desc.points, desc.dim, desc.kind = table.data_ptr(), DIM, _native.MPLAK_KERNEL_GAUSSIAN
+    desc.length_scale, desc.nugget, desc.coincident_shift = ELL, NUGGET, 0.0
+    desc.row_offset, desc.row_count, desc.col_offset, desc.col_count = 0, N, 0, N
+    return desc
+
+
+def profile_kernels(K, table, stream_ptr):
+    """Launch the measured kernels a fixed number of times, for counter runs:
+    3 FP32 and 2 FP64 Gaussian products (N=65536), 3 dgemv (65536 x 65536) and
+    3 copies of 2^27 doubles (1 GiB read + 1 GiB written, the traffic
+    calibration)."""
+    import torch
+    from mpla_amd import _native
+    desc = product_desc(table)
+    wb = K.mplak_kernel_matvec_workspace_bytes(C.byref(desc))
+    work = torch.empty(wb // 8 + 1, dtype=torch.float64, device="cuda")
+    xv = torch.empty(N, dtype=torch.float64, device="cuda")
+    yv = torch.empty(N, dtype=torch.float64, device="cuda")
+    _native.check(K.mplak_fill_uniform_f64(xv.data_ptr(), N, SEED, 2, 0, -1.0, 1.0, stream_ptr))
+    for prec, reps in ((_native.MPLAK_PREC_F32, 3), (_native.MPLAK_PREC_F64, 2)):
+        for _ in range(reps):
+            _native.check(K.mplak_kernel_matvec(C.byref(desc), prec, xv.data_ptr(), yv.data_ptr(), work.data_ptr(),
+                                                wb, None, stream_ptr))
+    torch.cuda.synchronize()
+    log("products done")
+    del work
+    n = 65536
+    Ad = torch.empty(n * n, dtype=torch.float64, device="cuda")
+    _native.check(K.mplak_fill_uniform_f64(Ad.data_ptr(), n * n, 1234, 1, 0, -1.0, 1.0, stream_ptr))
+    gwb = K.mplak_dgemv_workspace_bytes(n, n)
+    gw = torch.empty(gwb // 8 + 1, dtype=torch.float64, device="cuda")
+    for _ in range(3):
+        _native.check(K.mplak_dgemv_f64(n, n, Ad.data_ptr(), n, xv.data_ptr(), yv.data_ptr(), gw.data_ptr(), gwb,
+                                        None, stream_ptr))
+    torch.cuda.synchronize()
+    log("dgemv done")
+    cn = 1 << 27
+    src = Ad[:cn]
+    dst = Ad[cn:2 * cn]
+    for _ in range(3):
+        _native.check(K.mplak_copy_f64(dst.data_ptr(), src.data_ptr(), cn, stream_ptr))
+    torch.cuda.synchronize()
+    log("copies done")
+    print(json.dumps({"kernels_only": True, "product_n": N, "dgemv_n": n, "copy_bytes_each_way": 8 * cn}),
+          flush=True)
 
 
 def main():
@@ -307,10 +364,12 @@ def main():
     ap.add_argument("--skip-dgemv", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-configs", action="store_true", help="only the headline workload")
-    ap.add_argument("--c5-requests", type=int, default=48)
-    ap.add_argument("--c3-precisions", default="bf16x3,bf16")
-    ap.add_argument("--c3-oracle-columns", action="store_true",
+    ap.add_argument("--served-requests", type=int, default=48)
+    ap.add_argument("--block-precisions", default="bf16x3,bf16")
+    ap.add_argument("--block-oracle-columns", action="store_true",
                     help="also solve columns 0,21,42,63 with plain FP64 CG for the error report")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="launch only the measured kernels a few times (for rocprofv3 counter runs) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path runs on a 1x1 "
@@ -349,6 +408,10 @@ def main():
 
     def reset_x():
         _native.check(K.mplak_fill_f64(x.data, N, 0.0, stream_ptr))
+
+    if args.kernels_only:
+        profile_kernels(K, table, stream_ptr)
+        return
 
     def solve_mixed():
         reset_x()
@@ -401,10 +464,7 @@ def main():
     err_vs_fp64 = float(np.linalg.norm(x_mixed - x_plain) / np.linalg.norm(x_plain))
 
     # ---- dominant kernel: FP32 matrix-free product, HIP events on the stream --
-    desc = _native.mplak_kernel_desc()
-    desc.points, desc.dim, desc.kind = table.data_ptr(), DIM, _native.MPLAK_KERNEL_GAUSSIAN
-    desc.length_scale, desc.nugget, desc.coincident_shift = ELL, NUGGET, 0.0
-    desc.row_offset, desc.row_count, desc.col_offset, desc.col_count = 0, N, 0, N
+    desc = product_desc(table)
     wb = K.mplak_kernel_matvec_workspace_bytes(C.byref(desc))
     work = torch.empty(wb // 8 + 1, dtype=torch.float64, device="cuda")
     yv = torch.empty(N, dtype=torch.float64, device="cuda")
@@ -431,6 +491,8 @@ def main():
     entries = float(N) * N
     ach32 = FLOP_PER_ENTRY * entries / t32
     ach64 = FLOP_PER_ENTRY * entries / t64
+    pairs = sym_pairs(N)
+    exec32 = (FLOP_PER_ENTRY + 2) * pairs / t32
     log(f"kernel matvec: FP32 {t32 * 1e3:.3f} ms ({entries / t32:.3e} entries/s), "
         f"FP64 {t64 * 1e3:.3f} ms ({entries / t64:.3e} entries/s)")
     del work
@@ -489,10 +551,10 @@ def main():
 
     configs = {}
     if not args.skip_configs:
-        configs["c1_dense_cg"] = bench_c1(K, L, inst, stream_ptr, ext)
-        configs["c4_dense"] = bench_c4(K, L, inst, stream_ptr, ext)
-        configs["c3_block"] = bench_c3(K, L, inst, stream_ptr, args)
-        configs["c5_served"] = bench_c5(args)
+        configs["dense_cg_n4096"] = bench_dense_cg(K, L, inst, stream_ptr, ext)
+        configs["dense_dgemv_bicgstab"] = bench_dense_large(K, L, inst, stream_ptr, ext)
+        configs["matern_block_n131072"] = bench_block(K, L, inst, stream_ptr, args)
+        configs["served_stream"] = bench_served(args)
 
     value = args.steps / wall
     out = {
@@ -520,9 +582,12 @@ def main():
         "speedup_mixed_vs_plain": value / plain_sps,
         "roofline": {"bound": "valu", "achieved": ach32 / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
                      "frac": ach32 / PEAK_FP32, "traffic": None,
-                     "kernel": "kmv_partial_kernel<float, GAUSSIAN, 3> + finish",
+                     "kernel": "kmv_sym_kernel<float, GAUSSIAN, 3> + kmv_sym_finish_kernel",
                      "launch_ms": t32 * 1e3, "entries_per_s": entries / t32,
-                     "flop_per_entry": FLOP_PER_ENTRY},
+                     "flop_per_entry": FLOP_PER_ENTRY,
+                     "basis": "algorithmic: N^2 entries x 13 FLOP per product; the symmetric kernel evaluates "
+                              "each pair once (N(N+1)/2 evaluations, 15 FLOP each incl. both accumulations)",
+                     "executed_tflops": exec32 / 1e12, "executed_frac": exec32 / PEAK_FP32},
         "roofline_fp64_kernel": {"bound": "valu", "achieved": ach64 / 1e12, "peak": PEAK_FP64 / 1e12,
                                  "unit": "TFLOP/s", "frac": ach64 / PEAK_FP64, "launch_ms": t64 * 1e3,
                                  "entries_per_s": entries / t64},

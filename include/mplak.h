@@ -138,6 +138,38 @@ int mplak_kernel_matvec(const mplak_kernel_desc* desc, int precision, const doub
                         void* work, size_t work_bytes, const int32_t* skip_if,
                         mplak_stream_t stream);
 
+/* No reference counterpart (an approximation the reference does not make):
+ * the product of mplak_kernel_matvec with negligible entries dropped, for
+ * inner solves whose result is corrected in FP64 (iterative refinement).
+ * For a square block with row range == column range and dim <= 3, entries
+ * between 128-point target blocks and 64-point source blocks whose bounding
+ * boxes are so far apart that every entry is below 2^-drop_log2 (the kernel's
+ * value at distance 0 is 1) are treated as zero; everything else is computed
+ * exactly as by mplak_kernel_matvec.  Other blocks get the exact product.
+ * Pays off when consecutive points are spatially close (see
+ * mplak_spatial_order).  0 < drop_log2 <= 2000.  Deterministic. */
+size_t mplak_kernel_matvec_screened_workspace_bytes(const mplak_kernel_desc* desc);
+int mplak_kernel_matvec_screened(const mplak_kernel_desc* desc, int precision, double drop_log2,
+                                 const double* x, double* y, void* work, size_t work_bytes,
+                                 const int32_t* skip_if, mplak_stream_t stream);
+
+/* No reference counterpart.  Spatial (Z-order / Morton) permutation of the
+ * points p[d][offset + i], i < n: perm[k] is the index i of the k-th point
+ * when the points' bounding box is cut into 2^(30/dim) cells per axis and the
+ * cells are visited in Morton order; points in one cell keep their index
+ * order.  Deterministic.  n < 2^31. */
+size_t mplak_spatial_order_workspace_bytes(int64_t n, int32_t dim);
+int mplak_spatial_order(const double* const* points, int32_t dim, int64_t offset, int64_t n, int64_t* perm,
+                        void* work, size_t work_bytes, mplak_stream_t stream);
+
+/* out[d * ld + k] = points[d][offset + perm[k]] (d < dim, k < n). */
+int mplak_gather_points(const double* const* points, int32_t dim, int64_t offset, const int64_t* perm, int64_t n,
+                        double* out, int64_t ld, mplak_stream_t stream);
+
+/* dst[k] = src[perm[k]] (gather) and dst[perm[k]] = src[k] (scatter), k < n. */
+int mplak_gather_f64(double* dst, const double* src, const int64_t* perm, int64_t n, mplak_stream_t stream);
+int mplak_scatter_f64(double* dst, const double* src, const int64_t* perm, int64_t n, mplak_stream_t stream);
+
 /* b[i] = prod_d (sqrt(pi)/2) (erf(1 - p_d) - erf(-p_d)) for global point
  * row_offset + i: the integral of exp(-|p - y|^2) over y in [0,1]^dim.
  * Replaces gen_gaussian_kernel_rhs (src/kernel.cu:52-68). */

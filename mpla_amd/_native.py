@@ -14,6 +14,10 @@ import os
 LIB_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
 
 MPLAK_OK = 0
+MPLAK_ERR_ARG = 1
+MPLAK_ERR_HIP = 2
+MPLAK_ERR_WORKSPACE = 3
+MPLAK_ERR_UNSUPPORTED = 4
 MPLAK_KERNEL_GAUSSIAN = 0
 MPLAK_KERNEL_MATERN32 = 1
 MPLAK_PREC_F64 = 0
@@ -25,6 +29,14 @@ MPLAK_STOP_RELATIVE = 1
 MPLAK_RUNNING = 0
 MPLAK_CONVERGED = 1
 MPLAK_ITER_LIMIT = 2
+MPLAK_RAGGED_START = 0
+MPLAK_RAGGED_START_FINAL = 1
+MPLAK_RAGGED_GROUP = 2
+MPLAK_RAGGED_DZ = 3
+MPLAK_RAGGED_UPDATE = 4
+MPLAK_RAGGED_DIRECT = 5
+MPLAK_RAGGED_AXPY = 6
+MPLAK_RAGGED_FILL = 7
 
 
 class NativeLibraryMissing(RuntimeError):
@@ -103,6 +115,14 @@ MPLAK_SIGNATURES = {
     "mplak_dwaxpby_f64": (C.c_int, [_i64, _dbl, _vp, _dbl, _vp, _vp, _vp]),
     "mplak_kernel_matvec_workspace_bytes": (_sz, [C.POINTER(mplak_kernel_desc)]),
     "mplak_kernel_matvec": (C.c_int, [C.POINTER(mplak_kernel_desc), C.c_int, _vp, _vp, _vp, _sz, _vp, _vp]),
+    "mplak_kernel_matvec_screened_workspace_bytes": (_sz, [C.POINTER(mplak_kernel_desc)]),
+    "mplak_kernel_matvec_screened": (C.c_int, [C.POINTER(mplak_kernel_desc), C.c_int, _dbl, _vp, _vp, _vp, _sz,
+                                               _vp, _vp]),
+    "mplak_spatial_order_workspace_bytes": (_sz, [_i64, _i32]),
+    "mplak_spatial_order": (C.c_int, [_vp, _i32, _i64, _i64, _vp, _vp, _sz, _vp]),
+    "mplak_gather_points": (C.c_int, [_vp, _i32, _i64, _vp, _i64, _vp, _i64, _vp]),
+    "mplak_gather_f64": (C.c_int, [_vp, _vp, _vp, _i64, _vp]),
+    "mplak_scatter_f64": (C.c_int, [_vp, _vp, _vp, _i64, _vp]),
     "mplak_gaussian_kernel_rhs": (C.c_int, [_vp, _i32, _i64, _i64, _vp, _vp]),
     "mplak_kernel_mrhs_workspace_bytes": (_sz, [C.POINTER(mplak_kernel_desc), C.c_int, _i64]),
     "mplak_kernel_mrhs": (C.c_int, [C.POINTER(mplak_kernel_desc), C.c_int, _i64, _vp, _i64, _vp, _i64, _vp, _sz, _vp,

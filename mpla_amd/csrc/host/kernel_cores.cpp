@@ -55,6 +55,13 @@ void apply_kernel(const mplak_kernel_desc& d, int precision, const double* x, do
   ck(mplak_kernel_matvec(&d, precision, x, y, w, wb, skip, stream_of(inst)), what);
 }
 
+void apply_screened(const mplak_kernel_desc& d, int precision, double drop_log2, const double* x, double* y,
+                    const int32_t* skip, mpla_instance* inst, const char* what) {
+  const size_t wb = mplak_kernel_matvec_screened_workspace_bytes(&d);
+  void* w = arena(inst, kCore, wb);
+  ck(mplak_kernel_matvec_screened(&d, precision, drop_log2, x, y, w, wb, skip, stream_of(inst)), what);
+}
+
 size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
 
 // Plain FP64 CG (relative rule) through the fused core, printing nothing.
@@ -119,14 +126,25 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
     fail("mpla_kernel_conjugate_gradient_mixed", "inner precision must be MPLA_PREC_F32 or MPLA_PREC_F64");
   (void)kd;
 
-  const mplak_kernel_desc desc = fused_desc(A);
+  const mplak_kernel_desc desc0 = fused_desc(A);
   const int64_t n = x->cur_proc_row_count;
   const double n_global = (double)x->vec_row_count;
+  // Spatial order: the solve runs on the points sorted along a Morton curve
+  // (b, x permuted in, x permuted back out), so that the FP32 inner products
+  // can skip blocks of entries below 2^-drop (MPLA_IR_DROP_LOG2, default 30;
+  // <= 0 keeps every entry).  FP64 residual products stay exact.
+  const double drop = env_double("MPLA_IR_DROP_LOG2", 30.0);
+  const bool ordered = drop > 0.0 && desc0.row_offset == desc0.col_offset && desc0.row_count == desc0.col_count &&
+                       desc0.dim <= 3 && n > 0;
+  const int dim = desc0.dim;
   const size_t vb = align256(sizeof(double) * (size_t)n);
   const size_t sb = align256(2 * sizeof(mplak_cg_state));
   const size_t wb = mplak_cg_workspace_bytes(n);
   const size_t dwb = mplak_dot_workspace_bytes(n);
-  char* base = static_cast<char*>(arena(inst, kMixed, 4 * vb + sb + wb + align256(dwb) + 256));
+  const size_t ob = ordered ? mplak_spatial_order_workspace_bytes(n, dim) : 0;
+  if (ordered && ob == 0) fail("mpla_kernel_conjugate_gradient_mixed", "spatial order workspace query failed");
+  const size_t order_bytes = ordered ? 3 * vb + (size_t)dim * vb + 256 + align256(ob) : 0;
+  char* base = static_cast<char*>(arena(inst, kMixed, 4 * vb + sb + wb + align256(dwb) + 256 + order_bytes));
   double* r = reinterpret_cast<double*>(base);
   double* d = reinterpret_cast<double*>(base + vb);
   double* z = reinterpret_cast<double*>(base + 2 * vb);
@@ -137,11 +155,42 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
   double* dscalar = reinterpret_cast<double*>(dotw + align256(dwb));
   mplak_stream_t s = stream_of(inst);
   Runtime* rt = runtime(inst);
-  // size the product workspace now: it must not grow during graph capture
-  (void)arena(inst, kCore, mplak_kernel_matvec_workspace_bytes(&desc));
+
+  mplak_kernel_desc desc = desc0;
+  double* xs = x->data;        // solution in solver order
+  const double* bs = b->data;  // right-hand side in solver order
+  int64_t* perm = nullptr;
+  if (ordered) {
+    char* ob0 = dotw + align256(dwb) + 256;
+    perm = reinterpret_cast<int64_t*>(ob0);
+    double* bp = reinterpret_cast<double*>(ob0 + vb);
+    double* xp = reinterpret_cast<double*>(ob0 + 2 * vb);
+    double* pts = reinterpret_cast<double*>(ob0 + 3 * vb);
+    double** table = reinterpret_cast<double**>(ob0 + 3 * vb + (size_t)dim * vb);
+    void* owork = ob0 + 3 * vb + (size_t)dim * vb + 256;
+    ck(mplak_spatial_order(desc0.points, dim, desc0.row_offset, n, perm, owork, ob, s), "mixed CG spatial order");
+    const int64_t ld = (int64_t)(vb / sizeof(double));
+    ck(mplak_gather_points(desc0.points, dim, desc0.row_offset, perm, n, pts, ld, s), "mixed CG point gather");
+    double** host_table = static_cast<double**>(rt->pinned);
+    for (int k = 0; k < dim; ++k) host_table[k] = pts + (size_t)k * ld;
+    hip_ck(hipMemcpyAsync(table, host_table, sizeof(double*) * dim, hipMemcpyHostToDevice, rt->stream),
+           "hipMemcpyAsync");
+    ck(mplak_gather_f64(bp, b->data, perm, n, s), "mixed CG permute");
+    ck(mplak_gather_f64(xp, x->data, perm, n, s), "mixed CG permute");
+    desc.points = table;
+    desc.row_offset = desc.col_offset = 0;
+    xs = xp;
+    bs = bp;
+  }
+  auto unpermute = [&]() {
+    if (ordered) ck(mplak_scatter_f64(x->data, xs, perm, n, s), "mixed CG permute back");
+  };
+  // size the product workspaces now: they must not grow during graph capture
+  (void)arena(inst, kCore,
+              std::max(mplak_kernel_matvec_workspace_bytes(&desc), mplak_kernel_matvec_screened_workspace_bytes(&desc)));
 
   // an all-zero initial guess needs no FP64 product for its residual
-  ck(mplak_ddot_f64(n, x->data, x->data, dscalar, dotw, dwb, s), "mixed CG");
+  ck(mplak_ddot_f64(n, xs, xs, dscalar, dotw, dwb, s), "mixed CG");
   bool x_zero = *static_cast<double*>(read_back(inst, dscalar, sizeof(double))) == 0.0;
 
   const double eta_floor = env_double("MPLA_IR_INNER_TOL", 1.0e-5);
@@ -154,7 +203,10 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
   auto iteration = [&](int parity) {
     mplak_cg_state* in = st + parity;
     mplak_cg_state* out = st + (1 - parity);
-    apply_kernel(desc, MPLAK_PREC_F32, d, z, &in->done, inst, "mixed CG inner product");
+    if (ordered)
+      apply_screened(desc, MPLAK_PREC_F32, drop, d, z, &in->done, inst, "mixed CG inner product");
+    else
+      apply_kernel(desc, MPLAK_PREC_F32, d, z, &in->done, inst, "mixed CG inner product");
     ck(mplak_cg_iterate(n, e, r, d, z, in, out, work, wb, s), "mixed CG inner iteration");
   };
   auto normalize = [&]() { ck(mplak_cg_copy_state(st + 1, st, s), "mixed CG state copy"); };
@@ -166,10 +218,10 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
     if (x_zero) {
       ck(mplak_fill_f64(z, n, 0.0, s), "mixed CG");
     } else {
-      apply_kernel(desc, MPLAK_PREC_F64, x->data, z, nullptr, inst, "mixed CG residual product");
+      apply_kernel(desc, MPLAK_PREC_F64, xs, z, nullptr, inst, "mixed CG residual product");
       ++matvecs64;
     }
-    ck(mplak_cg_start(n, b->data, z, r, d, st, MPLAK_STOP_RELATIVE, epsilon, 2, n_global, work, wb, s),
+    ck(mplak_cg_start(n, bs, z, r, d, st, MPLAK_STOP_RELATIVE, epsilon, 2, n_global, work, wb, s),
        "mixed CG residual");
     cur = *static_cast<mplak_cg_state*>(read_back(inst, st, sizeof(mplak_cg_state)));
     if (verbosity(inst) >= 2 && inst->is_parent)
@@ -202,11 +254,12 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
       ist = *static_cast<mplak_cg_state*>(read_back(inst, st, sizeof(mplak_cg_state)));
     }
     total_inner += ist.k;
-    ck(mplak_daxpy_f64(n, 1.0, e, x->data, s), "mixed CG update");
+    ck(mplak_daxpy_f64(n, 1.0, e, xs, s), "mixed CG update");
     x_zero = false;
     ++outer;
   }
 
+  unpermute();
   int iterations = total_inner;
   double rel = cur.rel;
   bool converged = !(cur.rel >= epsilon);

@@ -260,7 +260,7 @@ void dispatch_dim(dim3 grid, hipStream_t st, bool coinc, const mplak_kernel_desc
 // ---------------------------------------------------------------------------
 
 constexpr int kSymTile = 2 * kBlock;  // 512 points per tile
-constexpr int kSymMinN = 16384;       // below this the plain kernel fills the GPU better
+constexpr int kSymMinN = 12288;       // crossover measured on MI355X (plain kernel faster below)
 
 // MPLAK_KMV_SYMMETRIC=0 turns the symmetric product off, =1 uses it for every
 // eligible block regardless of size; unset: blocks of at least kSymMinN points.
@@ -292,17 +292,36 @@ __device__ __forceinline__ double wave_rotate(double v) {
 
 __device__ __forceinline__ int64_t sym_pair_index(int64_t I, int64_t J) { return J * (J + 1) / 2 + I; }
 
-template <typename T, int KIND, int DIM, bool COINC>
+template <typename T>
+__device__ __forceinline__ T wave_min(T v) {
+#pragma unroll
+  for (int m = 1; m < kWave; m <<= 1) v = fmin(v, __shfl_xor(v, m));
+  return v;
+}
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+  for (int m = 1; m < kWave; m <<= 1) v = fmax(v, __shfl_xor(v, m));
+  return v;
+}
+
+// SCREEN: a wave skips a 64-source group when the bounding boxes of its 128
+// targets and of the group are further apart (squared, in the scaled
+// coordinates) than drop_s, i.e. when every entry of that 128 x 64 block is
+// below the caller's drop level.  The test uses wave-reduced boxes, so the
+// branch is uniform across the wave.
+template <typename T, int KIND, int DIM, bool COINC, bool SCREEN>
 __global__ __launch_bounds__(kBlock) void kmv_sym_kernel(
     const double* const* __restrict__ points, int64_t off, int64_t n, double scale, double coinc_thr,
-    double coinc_shift, const double* __restrict__ x, double* __restrict__ part_i, double* __restrict__ part_j,
-    const int32_t* __restrict__ skip) {
+    double coinc_shift, double drop_s, const double* __restrict__ x, double* __restrict__ part_i,
+    double* __restrict__ part_j, const int32_t* __restrict__ skip) {
   if (skip != nullptr && *skip != 0) return;
   typedef typename Quad<T>::type Q;
   constexpr int kDup = sizeof(T) == 4 ? 2 : 1;  // copies of each 64-source group
   constexpr int kGroups = kSymTile / kWave;
   __shared__ Q src[kGroups][kDup * kWave];
   __shared__ T red[kBlock / kWave][kSymTile];
+  __shared__ T gbox[kGroups][6];
 
   const int64_t p = blockIdx.x;
   int64_t J = (int64_t)((sqrt(8.0 * (double)p + 1.0) - 1.0) * 0.5);
@@ -335,7 +354,34 @@ __global__ __launch_bounds__(kBlock) void kmv_sym_kernel(
     q.w = j < n ? (T)x[j] : T(0);
 #pragma unroll
     for (int c = 0; c < kDup; ++c) src[k / kWave][c * kWave + (k & (kWave - 1))] = q;
+    if (SCREEN) {  // k / kWave is uniform across the wave: one group per wave and pass
+      const T lo0 = wave_min(q.x), lo1 = wave_min(q.y), lo2 = wave_min(q.z);
+      const T hi0 = wave_max(q.x), hi1 = wave_max(q.y), hi2 = wave_max(q.z);
+      if (lane == 0) {
+        T* b = gbox[k / kWave];
+        b[0] = lo0, b[1] = lo1, b[2] = lo2, b[3] = hi0, b[4] = hi1, b[5] = hi2;
+      }
+    }
   }
+  T tlo[3] = {T(0), T(0), T(0)}, thi[3] = {T(0), T(0), T(0)};
+  if (SCREEN) {
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      tlo[d] = wave_min(fmin(pi[0][d], pi[1][d]));
+      thi[d] = wave_max(fmax(pi[0][d], pi[1][d]));
+    }
+  }
+  // true when every entry between this wave's targets and group g is dropped
+  auto far_group = [&](int g) -> bool {
+    if (!SCREEN) return false;
+    T d2 = T(0);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      const T gap = fmax(T(0), fmax(gbox[g][d] - thi[d], tlo[d] - gbox[g][3 + d]));
+      d2 = fma(gap, gap, d2);
+    }
+    return __builtin_amdgcn_readfirstlane((int)(d2 > (T)drop_s)) != 0;
+  };
   __syncthreads();
 
   T accI[2] = {T(0), T(0)};
@@ -361,6 +407,7 @@ __global__ __launch_bounds__(kBlock) void kmv_sym_kernel(
   if (I == J) {
 #pragma unroll 1
     for (int g = 0; g < kGroups; ++g) {
+      if (far_group(g)) continue;
 #pragma unroll 4
       for (int s = 0; s < kWave; ++s) {
         const Q q = src[g][s];
@@ -374,6 +421,10 @@ __global__ __launch_bounds__(kBlock) void kmv_sym_kernel(
 #pragma unroll 1
     for (int g = 0; g < kGroups; ++g) {
       T t = T(0);
+      if (far_group(g)) {
+        red[w][g * kWave + lane] = t;
+        continue;
+      }
 #pragma unroll 4
       for (int s = 0; s < kWave; ++s) {
         const Q q = kDup == 2 ? src[g][lane + s] : src[g][(lane + s) & (kWave - 1)];
@@ -430,27 +481,93 @@ inline bool use_symmetric(const mplak_kernel_desc* d) {
 
 inline int64_t sym_tiles(int64_t n) { return (n + kSymTile - 1) / kSymTile; }
 
-template <typename T, int KIND, int DIM>
-void launch_sym_dim(hipStream_t st, bool coinc, const mplak_kernel_desc* d, double scale, double thr, const double* x,
-                    double* pi, double* pj, const int32_t* skip) {
+template <typename T, int KIND, int DIM, bool SCREEN>
+void launch_sym_dim(hipStream_t st, bool coinc, const mplak_kernel_desc* d, double scale, double thr,
+                    double drop_s, const double* x, double* pi, double* pj, const int32_t* skip) {
   const int64_t nt = sym_tiles(d->row_count);
   const dim3 grid((unsigned)(nt * (nt + 1) / 2));
   if (coinc)
-    hipLaunchKernelGGL((kmv_sym_kernel<T, KIND, DIM, true>), grid, dim3(kBlock), 0, st, d->points, d->row_offset,
-                       d->row_count, scale, thr, d->coincident_shift, x, pi, pj, skip);
+    hipLaunchKernelGGL((kmv_sym_kernel<T, KIND, DIM, true, SCREEN>), grid, dim3(kBlock), 0, st, d->points,
+                       d->row_offset, d->row_count, scale, thr, d->coincident_shift, drop_s, x, pi, pj, skip);
   else
-    hipLaunchKernelGGL((kmv_sym_kernel<T, KIND, DIM, false>), grid, dim3(kBlock), 0, st, d->points, d->row_offset,
-                       d->row_count, scale, thr, d->coincident_shift, x, pi, pj, skip);
+    hipLaunchKernelGGL((kmv_sym_kernel<T, KIND, DIM, false, SCREEN>), grid, dim3(kBlock), 0, st, d->points,
+                       d->row_offset, d->row_count, scale, thr, d->coincident_shift, drop_s, x, pi, pj, skip);
 }
 
-template <typename T, int KIND>
-void launch_sym(hipStream_t st, bool coinc, const mplak_kernel_desc* d, double scale, double thr, const double* x,
-                double* pi, double* pj, const int32_t* skip) {
+template <typename T, int KIND, bool SCREEN>
+void launch_sym(hipStream_t st, bool coinc, const mplak_kernel_desc* d, double scale, double thr, double drop_s,
+                const double* x, double* pi, double* pj, const int32_t* skip) {
   switch (d->dim) {
-    case 1: launch_sym_dim<T, KIND, 1>(st, coinc, d, scale, thr, x, pi, pj, skip); break;
-    case 2: launch_sym_dim<T, KIND, 2>(st, coinc, d, scale, thr, x, pi, pj, skip); break;
-    default: launch_sym_dim<T, KIND, 3>(st, coinc, d, scale, thr, x, pi, pj, skip); break;
+    case 1: launch_sym_dim<T, KIND, 1, SCREEN>(st, coinc, d, scale, thr, drop_s, x, pi, pj, skip); break;
+    case 2: launch_sym_dim<T, KIND, 2, SCREEN>(st, coinc, d, scale, thr, drop_s, x, pi, pj, skip); break;
+    default: launch_sym_dim<T, KIND, 3, SCREEN>(st, coinc, d, scale, thr, drop_s, x, pi, pj, skip); break;
   }
+}
+
+template <bool SCREEN>
+void launch_sym_any(hipStream_t st, int precision, bool coinc, const mplak_kernel_desc* d, double scale, double thr,
+                    double drop_s, const double* x, double* pi, double* pj, const int32_t* skip) {
+  const bool gauss = d->kind == MPLAK_KERNEL_GAUSSIAN;
+  if (precision == MPLAK_PREC_F64) {
+    if (gauss)
+      launch_sym<double, MPLAK_KERNEL_GAUSSIAN, SCREEN>(st, coinc, d, scale, thr, drop_s, x, pi, pj, skip);
+    else
+      launch_sym<double, MPLAK_KERNEL_MATERN32, SCREEN>(st, coinc, d, scale, thr, drop_s, x, pi, pj, skip);
+  } else {
+    if (gauss)
+      launch_sym<float, MPLAK_KERNEL_GAUSSIAN, SCREEN>(st, coinc, d, scale, thr, drop_s, x, pi, pj, skip);
+    else
+      launch_sym<float, MPLAK_KERNEL_MATERN32, SCREEN>(st, coinc, d, scale, thr, drop_s, x, pi, pj, skip);
+  }
+}
+
+bool sym_eligible(const mplak_kernel_desc* d) {
+  return d->row_offset == d->col_offset && d->row_count == d->col_count && d->dim <= 3;
+}
+
+size_t sym_workspace_bytes(int64_t n) {
+  const int64_t nt = sym_tiles(n);
+  return (size_t)2 * (size_t)(nt * (nt + 1) / 2) * kSymTile * sizeof(double);
+}
+
+// Symmetric product: tile-pair kernel (screened or exact) and the finish.
+int run_sym(const mplak_kernel_desc* d, int precision, bool screen, double drop_s, const double* x, double* y,
+            void* work, const int32_t* skip_if, mplak_stream_t s) {
+  const double scale = coord_scale(d->kind, precision, d->length_scale);
+  const bool coinc = d->coincident_shift != 0.0;
+  const double thr = (1e-14 * scale) * (1e-14 * scale);
+  const int64_t nt = sym_tiles(d->row_count);
+  double* pi = static_cast<double*>(work);
+  double* pj = pi + (size_t)(nt * (nt + 1) / 2) * kSymTile;
+  hipStream_t hs = as_stream(s);
+  if (screen)
+    launch_sym_any<true>(hs, precision, coinc, d, scale, thr, drop_s, x, pi, pj, skip_if);
+  else
+    launch_sym_any<false>(hs, precision, coinc, d, scale, thr, drop_s, x, pi, pj, skip_if);
+  int st = launch_status();
+  if (st != MPLAK_OK) return st;
+  int64_t fb = (d->row_count + kBlock - 1) / kBlock;
+  if (fb > 4096) fb = 4096;
+  hipLaunchKernelGGL(kmv_sym_finish_kernel, dim3((unsigned)fb), dim3(kBlock), 0, hs, d->row_count, nt, pi, pj,
+                     d->nugget, x, y, skip_if);
+  return launch_status();
+}
+
+// Squared scaled distance beyond which every entry is below 2^-drop_log2
+// (in the coordinates coord_scale gives for this kind and precision).
+double drop_distance2(int kind, int precision, double drop_log2) {
+  if (kind == MPLAK_KERNEL_GAUSSIAN)
+    return precision == MPLAK_PREC_F32 ? drop_log2 : drop_log2 * 0.69314718055994530942;
+  // (1 + r) exp(-r) is decreasing for r > 0: bisection for (1 + r) exp(-r) = 2^-drop_log2
+  const double target = -drop_log2 * 0.69314718055994530942;
+  double lo = 0.0, hi = 1.0;
+  while (log1p(hi) - hi > target) hi *= 2.0;
+  for (int i = 0; i < 200; ++i) {
+    const double mid = 0.5 * (lo + hi);
+    if (log1p(mid) - mid > target) lo = mid;
+    else hi = mid;
+  }
+  return hi * hi;
 }
 
 int check_desc(const mplak_kernel_desc* d) {
@@ -472,12 +589,29 @@ extern "C" {
 
 size_t mplak_kernel_matvec_workspace_bytes(const mplak_kernel_desc* d) {
   if (d == nullptr || d->row_count <= 0 || d->col_count <= 0) return 0;
-  if (use_symmetric(d)) {
-    const int64_t nt = sym_tiles(d->row_count);
-    return (size_t)2 * (size_t)(nt * (nt + 1) / 2) * kSymTile * sizeof(double);
-  }
+  if (use_symmetric(d)) return sym_workspace_bytes(d->row_count);
   const KmvPlan p = kmv_plan(d->row_count, d->col_count);
   return (size_t)p.slabs * (size_t)d->row_count * sizeof(double);
+}
+
+size_t mplak_kernel_matvec_screened_workspace_bytes(const mplak_kernel_desc* d) {
+  if (d == nullptr || d->row_count <= 0 || d->col_count <= 0) return 0;
+  if (sym_eligible(d)) return sym_workspace_bytes(d->row_count);
+  return mplak_kernel_matvec_workspace_bytes(d);
+}
+
+int mplak_kernel_matvec_screened(const mplak_kernel_desc* d, int precision, double drop_log2, const double* x,
+                                 double* y, void* work, size_t work_bytes, const int32_t* skip_if,
+                                 mplak_stream_t s) {
+  int st = check_desc(d);
+  if (st != MPLAK_OK) return st;
+  if (precision != MPLAK_PREC_F64 && precision != MPLAK_PREC_F32) return MPLAK_ERR_ARG;
+  if (!(drop_log2 > 0.0) || drop_log2 > 2000.0) return MPLAK_ERR_ARG;
+  if (!sym_eligible(d)) return mplak_kernel_matvec(d, precision, x, y, work, work_bytes, skip_if, s);
+  if (d->row_count == 0) return MPLAK_OK;
+  if (x == nullptr || y == nullptr) return MPLAK_ERR_ARG;
+  if (work == nullptr || work_bytes < sym_workspace_bytes(d->row_count)) return MPLAK_ERR_WORKSPACE;
+  return run_sym(d, precision, true, drop_distance2(d->kind, precision, drop_log2), x, y, work, skip_if, s);
 }
 
 int mplak_kernel_matvec(const mplak_kernel_desc* d, int precision, const double* x, double* y,
@@ -491,34 +625,12 @@ int mplak_kernel_matvec(const mplak_kernel_desc* d, int precision, const double*
   if (x == nullptr) return MPLAK_ERR_ARG;
   const size_t need = mplak_kernel_matvec_workspace_bytes(d);
   if (work == nullptr || work_bytes < need) return MPLAK_ERR_WORKSPACE;
+  if (use_symmetric(d)) return run_sym(d, precision, false, 0.0, x, y, work, skip_if, s);
 
   const double scale = coord_scale(d->kind, precision, d->length_scale);
   const bool coinc = d->coincident_shift != 0.0;
   const double thr = (1e-14 * scale) * (1e-14 * scale);
   double* part = static_cast<double*>(work);
-  if (use_symmetric(d)) {
-    const int64_t nt = sym_tiles(d->row_count);
-    double* pj = part + (size_t)(nt * (nt + 1) / 2) * kSymTile;
-    hipStream_t hs0 = as_stream(s);
-    if (precision == MPLAK_PREC_F64) {
-      if (d->kind == MPLAK_KERNEL_GAUSSIAN)
-        launch_sym<double, MPLAK_KERNEL_GAUSSIAN>(hs0, coinc, d, scale, thr, x, part, pj, skip_if);
-      else
-        launch_sym<double, MPLAK_KERNEL_MATERN32>(hs0, coinc, d, scale, thr, x, part, pj, skip_if);
-    } else {
-      if (d->kind == MPLAK_KERNEL_GAUSSIAN)
-        launch_sym<float, MPLAK_KERNEL_GAUSSIAN>(hs0, coinc, d, scale, thr, x, part, pj, skip_if);
-      else
-        launch_sym<float, MPLAK_KERNEL_MATERN32>(hs0, coinc, d, scale, thr, x, part, pj, skip_if);
-    }
-    st = launch_status();
-    if (st != MPLAK_OK) return st;
-    int64_t fb = (d->row_count + kBlock - 1) / kBlock;
-    if (fb > 4096) fb = 4096;
-    hipLaunchKernelGGL(kmv_sym_finish_kernel, dim3((unsigned)fb), dim3(kBlock), 0, hs0, d->row_count, nt, part, pj,
-                       d->nugget, x, y, skip_if);
-    return launch_status();
-  }
   const KmvPlan p = kmv_plan(d->row_count, d->col_count);
   const dim3 grid((unsigned)p.row_tiles, (unsigned)p.slabs);
   hipStream_t hs = as_stream(s);

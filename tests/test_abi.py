@@ -55,6 +55,17 @@ def test_bindings_cover_header():
     assert sorted(_native.MPLAK_SIGNATURES) == names
 
 
+def test_binding_constants_match_header():
+    from mpla_amd import _native
+    text = open(os.path.join(INCLUDE, "mplak.h")).read()
+    defines = dict(re.findall(r"#define\s+(MPLAK_[A-Z0-9_]+)\s+(-?\d+)\b", text))
+    assert len(defines) >= 15
+    for name, value in defines.items():
+        if name == "MPLAK_VERSION":
+            continue
+        assert getattr(_native, name) == int(value), name
+
+
 def test_status_strings_without_gpu():
     from mpla_amd import _native
     lib = _native.mplak()

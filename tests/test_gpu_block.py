@@ -82,7 +82,7 @@ _ORACLE_X = {}
 @pytest.mark.parametrize("prec", [mpla.MPLA_PREC_BF16X3, mpla.MPLA_PREC_BF16, mpla.MPLA_PREC_F32,
                                   mpla.MPLA_PREC_F64])
 def test_block_mixed_solver_c3_shape(oracle, prec):
-    """Config 3 shape at N = 4096: Matern-3/2 l = 0.1, nugget 0.5, 16 RHS ~ N(0,1), 1e-8."""
+    """Matern-3/2 multi-RHS system at N = 4096: l = 0.1, nugget 0.5, 16 RHS ~ N(0,1), 1e-8."""
     require_gpu()
     s = Session(verbosity=0)
     try:

@@ -131,7 +131,7 @@ def dense_spd(oracle, n, seed=1235):
 
 @pytest.mark.parametrize("n", [512, 4096])
 def test_checkpoint_cg_dense_spd(s, oracle, n):
-    """Config 1 shape: A = B B^T/N + I, b ~ U[-1,1], x0 = 0, relative 1e-10, max 2000 iterations."""
+    """Dense SPD system: A = B B^T/N + I, b ~ U[-1,1], x0 = 0, relative 1e-10, max 2000 iterations."""
     lib = s.lib
     Ah = dense_spd(oracle, n)
     bh = oracle.uniform(n, 1235, 11, -1, 1)
@@ -208,7 +208,7 @@ print("not reached")
 
 
 # --------------------------------------------------------------------------
-# BiCGSTAB (src/mpla.cpp:1003-1183), config 4 shape at reduced N
+# BiCGSTAB (src/mpla.cpp:1003-1183), diagonally dominant dense system at reduced N
 
 
 def nonsym(oracle, n, seed=1235):
@@ -306,16 +306,16 @@ def test_test_kernel_cg_manufactured(s, oracle):
         assert abs(s.inst.last_iterations - info.iterations) <= max(3, info.iterations // 10)
 
 
-def c2_system(oracle, n, ell=0.05, nugget=1.0, kind=0, seed=1235):
+def gaussian_system(oracle, n, ell=0.05, nugget=1.0, kind=0, seed=1235):
     pts = oracle.uniform(3 * n, seed, 0).reshape(3, n)
     bh = oracle.uniform(n, seed, 1, -1, 1)
     return pts, bh, oracle.Kernel(pts, kind, ell, nugget=nugget)
 
 
 def test_fused_core_plain_fp64_cg(s, oracle):
-    """Config 2 operator at N=8192: (K + 1.0 I) x = b, Gaussian l=0.05, relative 1e-8."""
+    """Gaussian operator at N=8192: (K + 1.0 I) x = b, l=0.05, relative 1e-8."""
     n = 8192
-    pts, bh, k = c2_system(oracle, n)
+    pts, bh, k = gaussian_system(oracle, n)
     P = Points(pts)
     kd = mpla.mpla_kernel_matrix_data(P.ptr, 3, 0, mpla.MPLA_KERNEL_GAUSSIAN, mpla.MPLA_PREC_F64, 0.05, 1.0, 0.0)
     A = s.generic(n, n, C.addressof(kd))
@@ -333,7 +333,7 @@ def test_fused_core_plain_fp64_cg(s, oracle):
 @pytest.mark.parametrize("kind,ell,nugget", [(0, 0.05, 1.0), (1, 0.1, 0.5)])
 def test_mixed_precision_ir_reaches_fp64_residual(s, oracle, kind, ell, nugget):
     n = 8192
-    pts, bh, k = c2_system(oracle, n, ell, nugget, kind)
+    pts, bh, k = gaussian_system(oracle, n, ell, nugget, kind)
     P = Points(pts)
     kd = mpla.mpla_kernel_matrix_data(P.ptr, 3, 0, kind, mpla.MPLA_PREC_F64, ell, nugget, 0.0)
     A = s.generic(n, n, C.addressof(kd))
@@ -350,9 +350,38 @@ def test_mixed_precision_ir_reaches_fp64_residual(s, oracle, kind, ell, nugget):
     assert np.linalg.norm(xs - xo) <= 1e-6 * np.linalg.norm(xo)
 
 
+def test_mixed_precision_ordered_screened_vs_exact_inner(s, oracle, monkeypatch):
+    """The mixed solve runs on spatially ordered points with FP32 inner
+    products that drop entries below 2^-30; with MPLA_IR_DROP_LOG2=0 it keeps
+    every entry in the original order.  Both reach the FP64 bar, agree to the
+    solution tolerance, and each is bitwise repeatable."""
+    n = 13000  # above the symmetric-product size threshold, ragged last tile
+    pts, bh, k = gaussian_system(oracle, n, 0.05, 1.0, 0)
+    P = Points(pts)
+    kd = mpla.mpla_kernel_matrix_data(P.ptr, 3, 0, 0, mpla.MPLA_PREC_F64, 0.05, 1.0, 0.0)
+    A = s.generic(n, n, C.addressof(kd))
+    b = s.vector(n, bh)
+    x = s.vector(n)
+    sols = {}
+    for drop in ("30", "0"):
+        monkeypatch.setenv("MPLA_IR_DROP_LOG2", drop)
+        runs = []
+        for _ in range(2):
+            s.set(x, np.zeros(n))
+            s.lib.mpla_kernel_conjugate_gradient_mixed(C.byref(b), C.byref(A), C.byref(x), 5000, 1e-8,
+                                                       mpla.MPLA_PREC_F32, s.I)
+            assert s.inst.last_converged == 1
+            runs.append(s.get(x))
+        assert np.array_equal(runs[0], runs[1])
+        xs = runs[0]
+        assert np.linalg.norm(bh - k.matvec(xs)) / np.linalg.norm(bh) < 1e-8 * 1.01
+        sols[drop] = xs
+    assert np.linalg.norm(sols["30"] - sols["0"]) <= 1e-7 * np.linalg.norm(sols["0"])
+
+
 def test_graph_replay_is_deterministic(s, oracle):
     n = 4096
-    pts, bh, k = c2_system(oracle, n)
+    pts, bh, k = gaussian_system(oracle, n)
     P = Points(pts)
     kd = mpla.mpla_kernel_matrix_data(P.ptr, 3, 0, 0, mpla.MPLA_PREC_F64, 0.05, 1.0, 0.0)
     A = s.generic(n, n, C.addressof(kd))

@@ -270,7 +270,7 @@ def test_kernel_matvec_full_size_linearity(lib, oracle):
 # --------------------------------------------------------------------------
 # symmetric product (square block, rows == columns): each entry evaluated once
 # and used for row i and row j.  MPLAK_KMV_SYMMETRIC=1 forces it at sizes the
-# oracle checks quickly; unset it is chosen from 16384 points up.
+# oracle checks quickly; unset it is chosen from 12288 points up.
 
 
 @pytest.mark.parametrize("kind,l", [(0, 1 / math.sqrt(2.0)), (0, 0.05), (1, 0.1)])

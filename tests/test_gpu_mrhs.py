@@ -117,7 +117,7 @@ def test_mrhs_deterministic(lib, oracle, prec):
 
 
 def test_mrhs_full_size_c3_linearity(lib, oracle):
-    """Config 3 operator at full size (N = 131072, Matern-3/2 l = 0.1, 64 RHS) in
+    """Matern-3/2 operator at full size (N = 131072, Matern-3/2 l = 0.1, 64 RHS) in
     BF16x3: linearity in V and spot rows against the oracle."""
     n, R = 131072, 64
     pts = oracle.uniform(3 * n, 1235, 0).reshape(3, n)

@@ -1,0 +1,167 @@
+"""Spatial ordering and the screened product (no reference counterpart).
+
+Bars: the permutation equals an independent numpy computation of the same
+Morton keys (exact integer work: bitwise); gather/scatter are exact inverses;
+the screened product with nothing to drop is bitwise the exact symmetric
+product; with a drop level of 2^-30 on ordered points every row differs from
+the oracle's exact FP64 product by at most the dropped mass bound
+n * 2^-30 * max|x| plus the FP32 product tolerance; the mixed-precision solve
+on ordered points reaches the FP64 residual bar.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+from mpla_amd import _native
+from tests.gpu_util import dev, empty, host, point_table, require_gpu, stream, workspace
+from tests.test_gpu_mplak import kernel_desc, kmv_gpu, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def lib():
+    require_gpu()
+    return _native.mplak()
+
+
+def ck(status):
+    _native.check(status)
+
+
+def morton_perm_numpy(pts):
+    """Z-order over 2^(30/dim) cells per axis of the bounding box, ties by index."""
+    dim, n = pts.shape
+    bits = 30 // dim
+    cells = float(1 << bits)
+    code = np.zeros(n, dtype=np.uint64)
+    for d in range(dim):
+        lo, hi = pts[d].min(), pts[d].max()
+        ext = hi - lo
+        c = np.floor((pts[d] - lo) / ext * cells) if ext > 0 else np.zeros(n)
+        c = np.clip(c, 0.0, cells - 1.0).astype(np.uint64)
+        for b in range(bits):
+            code |= ((c >> np.uint64(b)) & np.uint64(1)) << np.uint64(b * dim + d)
+    return np.argsort(code, kind="stable")
+
+
+def spatial_order(lib, T, dim, n, offset=0):
+    import torch
+    perm = torch.empty(n, dtype=torch.int64, device="cuda")
+    wb = lib.mplak_spatial_order_workspace_bytes(n, dim)
+    assert wb > 0
+    w = workspace(wb)
+    ck(lib.mplak_spatial_order(T.data_ptr(), dim, offset, n, perm.data_ptr(), w.data_ptr(), wb, stream()))
+    return perm
+
+
+@pytest.mark.parametrize("dim,n", [(1, 1), (1, 1000), (2, 4097), (3, 20000), (4, 777)])
+def test_spatial_order_matches_numpy_morton(lib, oracle, dim, n):
+    pts = oracle.uniform(dim * n, 300 + dim, 0, -2.0, 3.0).reshape(dim, n)
+    pts[:, n // 2] = pts[:, 0]  # duplicate point: tie broken by index
+    P = dev(pts)
+    T = point_table(P)
+    perm = host(spatial_order(lib, T, dim, n))
+    assert np.array_equal(np.sort(perm), np.arange(n))
+    assert np.array_equal(perm, morton_perm_numpy(pts))
+
+
+def test_spatial_order_offset_and_degenerate_axis(lib, oracle):
+    n, off = 3000, 500
+    pts = oracle.uniform(3 * (n + off), 9, 0).reshape(3, n + off)
+    pts[2, :] = 0.25  # flat axis: every point in cell 0 along z
+    P = dev(pts)
+    T = point_table(P)
+    perm = host(spatial_order(lib, T, 3, n, offset=off))
+    assert np.array_equal(perm, morton_perm_numpy(pts[:, off:]))
+
+
+def test_gather_scatter_points_round_trip(lib, oracle):
+    import torch
+    n, dim = 5000, 3
+    pts = oracle.uniform(dim * n, 4, 0).reshape(dim, n)
+    P = dev(pts)
+    T = point_table(P)
+    perm = spatial_order(lib, T, dim, n)
+    ph = host(perm)
+    out = torch.empty(dim * (n + 3), dtype=torch.float64, device="cuda")
+    ck(lib.mplak_gather_points(T.data_ptr(), dim, 0, perm.data_ptr(), n, out.data_ptr(), n + 3, stream()))
+    o = host(out).reshape(dim, n + 3)[:, :n]
+    assert np.array_equal(o, pts[:, ph])
+    x = oracle.uniform(n, 4, 1, -1, 1)
+    X = dev(x)
+    g = empty(n)
+    ck(lib.mplak_gather_f64(g.data_ptr(), X.data_ptr(), perm.data_ptr(), n, stream()))
+    assert np.array_equal(host(g), x[ph])
+    back = empty(n)
+    ck(lib.mplak_scatter_f64(back.data_ptr(), g.data_ptr(), perm.data_ptr(), n, stream()))
+    assert np.array_equal(host(back), x)
+    assert lib.mplak_gather_f64(g.data_ptr(), g.data_ptr(), perm.data_ptr(), n, stream()) == _native.MPLAK_ERR_ARG
+
+
+def screened(lib, desc, X, prec, drop):
+    y = empty(desc.row_count)
+    wb = lib.mplak_kernel_matvec_screened_workspace_bytes(C.byref(desc))
+    w = workspace(wb)
+    ck(lib.mplak_kernel_matvec_screened(C.byref(desc), prec, drop, X.data_ptr(), y.data_ptr(), w.data_ptr(), wb,
+                                        None, stream()))
+    return host(y)
+
+
+def ordered_points(lib, oracle, n, dim, seed):
+    pts = oracle.uniform(dim * n, seed, 0).reshape(dim, n)
+    return pts[:, morton_perm_numpy(pts)].copy()
+
+
+@pytest.mark.parametrize("kind,l", [(0, 0.05), (1, 0.1)])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_screened_without_drops_is_the_exact_symmetric_product(lib, oracle, monkeypatch, kind, l, prec):
+    monkeypatch.setenv("MPLAK_KMV_SYMMETRIC", "1")
+    n = 3000
+    pts = ordered_points(lib, oracle, n, 3, 51)
+    x = oracle.uniform(n, 51, 1, -1, 1)
+    P = dev(pts)
+    T = point_table(P)
+    d = kernel_desc(P, T, kind, l, nugget=0.5)
+    X = dev(x)
+    exact = host(kmv_gpu(lib, d, X, precision=prec))
+    assert np.array_equal(screened(lib, d, X, prec, 2000.0), exact)
+
+
+@pytest.mark.parametrize("kind,l,dim", [(0, 0.05, 3), (0, 0.02, 2), (1, 0.05, 3), (0, 0.01, 1)])
+def test_screened_error_within_dropped_mass_bound(lib, oracle, kind, l, dim):
+    n = 20000
+    pts = ordered_points(lib, oracle, n, dim, 60 + dim)
+    x = oracle.uniform(n, 61, 1, -1, 1)
+    P = dev(pts)
+    T = point_table(P)
+    d = kernel_desc(P, T, kind, l, nugget=1.0)
+    X = dev(x)
+    k = oracle.Kernel(pts, kind, l, nugget=1.0)
+    rows = np.arange(0, n, 97)
+    ref = np.array([k.matvec(x, mode=1, row_off=int(i), rows=1)[0] for i in rows])
+    absref = np.array([k.matvec(np.abs(x), mode=1, row_off=int(i), rows=1)[0] for i in rows])
+    drop_bound = n * 2.0 ** -30 * np.abs(x).max()
+    for prec, tol in [(0, 1e-12), (1, 2e-5)]:
+        y = screened(lib, d, X, prec, 30.0)[rows]
+        assert np.all(np.abs(y - ref) <= drop_bound + tol * absref)
+    # something was dropped: the FP64 screened result is not the exact one
+    y64 = screened(lib, d, X, 0, 30.0)
+    exact = host(kmv_gpu(lib, d, X, precision=0))
+    assert not np.array_equal(y64, exact)
+    assert rel_err(y64, exact) <= 1e-7
+
+
+def test_screened_falls_back_to_exact_for_rectangular_blocks(lib, oracle):
+    n = 2000
+    pts, x = oracle.uniform(3 * n, 8, 0).reshape(3, n), oracle.uniform(1500, 8, 1, -1, 1)
+    P = dev(pts)
+    T = point_table(P)
+    d = kernel_desc(P, T, 1, 0.1, nugget=0.75, row_off=300, rows=900, col_off=100, cols=1500)
+    X = dev(x)
+    assert np.array_equal(screened(lib, d, X, 0, 30.0), host(kmv_gpu(lib, d, X)))
+    y = empty(900)
+    assert lib.mplak_kernel_matvec_screened(C.byref(d), 0, 0.0, X.data_ptr(), y.data_ptr(), None, 0, None,
+                                            stream()) == _native.MPLAK_ERR_ARG
