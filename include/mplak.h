@@ -153,6 +153,23 @@ int mplak_kernel_matvec_screened(const mplak_kernel_desc* desc, int precision, d
                                  const double* x, double* y, void* work, size_t work_bytes,
                                  const int32_t* skip_if, mplak_stream_t stream);
 
+/* No reference counterpart.  A screening plan for repeated screened products
+ * with the same points, precision and drop level (an inner solve): built once,
+ * it lists the 512 x 512 tile pairs that hold at least one kept block,
+ * heaviest first, and owns the product's partial sums (so one planned product
+ * at a time per plan).  mplak_kernel_matvec_planned gives bitwise the result
+ * of mplak_kernel_matvec_screened with the same arguments, without launching
+ * work for tile pairs whose entries are all dropped.  The desc, precision and
+ * drop_log2 passed to the product must be the ones the plan was built with.
+ * Square blocks with row range == column range and dim <= 3 only
+ * (mplak_screen_plan_bytes returns 0 otherwise). */
+size_t mplak_screen_plan_bytes(const mplak_kernel_desc* desc);
+int mplak_screen_plan_build(const mplak_kernel_desc* desc, int precision, double drop_log2, void* plan,
+                            size_t plan_bytes, mplak_stream_t stream);
+int mplak_kernel_matvec_planned(const mplak_kernel_desc* desc, int precision, double drop_log2, void* plan,
+                                size_t plan_bytes, const double* x, double* y, const int32_t* skip_if,
+                                mplak_stream_t stream);
+
 /* No reference counterpart.  Spatial (Z-order / Morton) permutation of the
  * points p[d][offset + i], i < n: perm[k] is the index i of the k-th point
  * when the points' bounding box is cut into 2^(30/dim) cells per axis and the

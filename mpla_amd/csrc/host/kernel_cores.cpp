@@ -55,13 +55,6 @@ void apply_kernel(const mplak_kernel_desc& d, int precision, const double* x, do
   ck(mplak_kernel_matvec(&d, precision, x, y, w, wb, skip, stream_of(inst)), what);
 }
 
-void apply_screened(const mplak_kernel_desc& d, int precision, double drop_log2, const double* x, double* y,
-                    const int32_t* skip, mpla_instance* inst, const char* what) {
-  const size_t wb = mplak_kernel_matvec_screened_workspace_bytes(&d);
-  void* w = arena(inst, kCore, wb);
-  ck(mplak_kernel_matvec_screened(&d, precision, drop_log2, x, y, w, wb, skip, stream_of(inst)), what);
-}
-
 size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
 
 // Plain FP64 CG (relative rule) through the fused core, printing nothing.
@@ -129,13 +122,20 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
   const mplak_kernel_desc desc0 = fused_desc(A);
   const int64_t n = x->cur_proc_row_count;
   const double n_global = (double)x->vec_row_count;
-  // Spatial order: the solve runs on the points sorted along a Morton curve
+  // Inner solves stop at a relative residual eta >= MPLA_IR_INNER_TOL
+  // (default 1e-4).  The solve runs on the points sorted along a Morton curve
   // (b, x permuted in, x permuted back out), so that the FP32 inner products
-  // can skip blocks of entries below 2^-drop (MPLA_IR_DROP_LOG2, default 30;
-  // <= 0 keeps every entry).  FP64 residual products stay exact.
-  const double drop = env_double("MPLA_IR_DROP_LOG2", 30.0);
+  // can skip blocks of entries below 2^-drop: MPLA_IR_DROP_LOG2, by default
+  // 6 bits below the inner tolerance (20 for 1e-4); <= 0 keeps every entry.
+  // The FP64 residual products are exact, so the final residual is the true
+  // one whatever the inner operator dropped.
+  const double eta_floor = env_double("MPLA_IR_INNER_TOL", 1.0e-4);
+  const double drop = env_double("MPLA_IR_DROP_LOG2", std::ceil(-std::log2(eta_floor)) + 6.0);
+  mplak_kernel_desc shape = desc0;
+  shape.row_offset = shape.col_offset = 0;
+  const size_t pb = mplak_screen_plan_bytes(&shape);
   const bool ordered = drop > 0.0 && desc0.row_offset == desc0.col_offset && desc0.row_count == desc0.col_count &&
-                       desc0.dim <= 3 && n > 0;
+                       pb > 0 && n > 0;
   const int dim = desc0.dim;
   const size_t vb = align256(sizeof(double) * (size_t)n);
   const size_t sb = align256(2 * sizeof(mplak_cg_state));
@@ -143,7 +143,7 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
   const size_t dwb = mplak_dot_workspace_bytes(n);
   const size_t ob = ordered ? mplak_spatial_order_workspace_bytes(n, dim) : 0;
   if (ordered && ob == 0) fail("mpla_kernel_conjugate_gradient_mixed", "spatial order workspace query failed");
-  const size_t order_bytes = ordered ? 3 * vb + (size_t)dim * vb + 256 + align256(ob) : 0;
+  const size_t order_bytes = ordered ? 3 * vb + (size_t)dim * vb + 256 + align256(ob) + align256(pb) : 0;
   char* base = static_cast<char*>(arena(inst, kMixed, 4 * vb + sb + wb + align256(dwb) + 256 + order_bytes));
   double* r = reinterpret_cast<double*>(base);
   double* d = reinterpret_cast<double*>(base + vb);
@@ -160,6 +160,7 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
   double* xs = x->data;        // solution in solver order
   const double* bs = b->data;  // right-hand side in solver order
   int64_t* perm = nullptr;
+  void* plan = nullptr;
   if (ordered) {
     char* ob0 = dotw + align256(dwb) + 256;
     perm = reinterpret_cast<int64_t*>(ob0);
@@ -168,6 +169,7 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
     double* pts = reinterpret_cast<double*>(ob0 + 3 * vb);
     double** table = reinterpret_cast<double**>(ob0 + 3 * vb + (size_t)dim * vb);
     void* owork = ob0 + 3 * vb + (size_t)dim * vb + 256;
+    plan = ob0 + 3 * vb + (size_t)dim * vb + 256 + align256(ob);
     ck(mplak_spatial_order(desc0.points, dim, desc0.row_offset, n, perm, owork, ob, s), "mixed CG spatial order");
     const int64_t ld = (int64_t)(vb / sizeof(double));
     ck(mplak_gather_points(desc0.points, dim, desc0.row_offset, perm, n, pts, ld, s), "mixed CG point gather");
@@ -181,19 +183,18 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
     desc.row_offset = desc.col_offset = 0;
     xs = xp;
     bs = bp;
+    ck(mplak_screen_plan_build(&desc, MPLAK_PREC_F32, drop, plan, pb, s), "mixed CG screening plan");
   }
   auto unpermute = [&]() {
     if (ordered) ck(mplak_scatter_f64(x->data, xs, perm, n, s), "mixed CG permute back");
   };
-  // size the product workspaces now: they must not grow during graph capture
-  (void)arena(inst, kCore,
-              std::max(mplak_kernel_matvec_workspace_bytes(&desc), mplak_kernel_matvec_screened_workspace_bytes(&desc)));
+  // size the product workspace now: it must not grow during graph capture
+  (void)arena(inst, kCore, mplak_kernel_matvec_workspace_bytes(&desc));
 
   // an all-zero initial guess needs no FP64 product for its residual
   ck(mplak_ddot_f64(n, xs, xs, dscalar, dotw, dwb, s), "mixed CG");
   bool x_zero = *static_cast<double*>(read_back(inst, dscalar, sizeof(double))) == 0.0;
 
-  const double eta_floor = env_double("MPLA_IR_INNER_TOL", 1.0e-5);
   const int max_outer = 40;
   int total_inner = 0, outer = 0, matvecs64 = 0;
   double prev_rel = INFINITY;
@@ -204,7 +205,8 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
     mplak_cg_state* in = st + parity;
     mplak_cg_state* out = st + (1 - parity);
     if (ordered)
-      apply_screened(desc, MPLAK_PREC_F32, drop, d, z, &in->done, inst, "mixed CG inner product");
+      ck(mplak_kernel_matvec_planned(&desc, MPLAK_PREC_F32, drop, plan, pb, d, z, &in->done, s),
+         "mixed CG inner product");
     else
       apply_kernel(desc, MPLAK_PREC_F32, d, z, &in->done, inst, "mixed CG inner product");
     ck(mplak_cg_iterate(n, e, r, d, z, in, out, work, wb, s), "mixed CG inner iteration");

@@ -18,6 +18,10 @@
 #include <math.h>
 #include <stdlib.h>
 
+#include <algorithm>
+
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include "common.hpp"
 
 namespace mplak {
@@ -305,17 +309,45 @@ __device__ __forceinline__ T wave_max(T v) {
   return v;
 }
 
-// SCREEN: a wave skips a 64-source group when the bounding boxes of its 128
-// targets and of the group are further apart (squared, in the scaled
+// Tile pair p = J (J + 1) / 2 + I, I <= J.
+__device__ __forceinline__ void decode_pair(int64_t p, int64_t& I, int64_t& J) {
+  J = (int64_t)((sqrt(8.0 * (double)p + 1.0) - 1.0) * 0.5);
+  while (J * (J + 1) / 2 > p) --J;
+  while ((J + 1) * (J + 2) / 2 <= p) ++J;
+  I = p - J * (J + 1) / 2;
+}
+
+// Squared distance between two axis-aligned boxes (lo at [0..2], hi at [3..5]).
+template <typename T, int DIM>
+__device__ __forceinline__ T box_gap2(const T* a, const T* b) {
+  T d2 = T(0);
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    const T gap = fmax(T(0), fmax(b[d] - a[3 + d], a[d] - b[3 + d]));
+    d2 = fma(gap, gap, d2);
+  }
+  return d2;
+}
+
+enum SymMode { kSymExact = 0, kSymScreened = 1, kSymPlanned = 2 };
+
+// kSymScreened: a wave skips a 64-source group when the bounding boxes of its
+// 128 targets and of the group are further apart (squared, in the scaled
 // coordinates) than drop_s, i.e. when every entry of that 128 x 64 block is
 // below the caller's drop level.  The test uses wave-reduced boxes, so the
 // branch is uniform across the wave.
-template <typename T, int KIND, int DIM, bool COINC, bool SCREEN>
+// kSymPlanned: as kSymScreened, over the tile pairs list[0 .. *count) of a
+// screening plan (pairs with at least one kept block, heaviest first); the
+// partials of the other pairs were zeroed when the plan was built.
+template <typename T, int KIND, int DIM, bool COINC, int MODE>
 __global__ __launch_bounds__(kBlock) void kmv_sym_kernel(
     const double* const* __restrict__ points, int64_t off, int64_t n, double scale, double coinc_thr,
     double coinc_shift, double drop_s, const double* __restrict__ x, double* __restrict__ part_i,
-    double* __restrict__ part_j, const int32_t* __restrict__ skip) {
+    double* __restrict__ part_j, const uint32_t* __restrict__ list, const int64_t* __restrict__ count,
+    const int32_t* __restrict__ skip) {
+  constexpr bool SCREEN = MODE != kSymExact;
   if (skip != nullptr && *skip != 0) return;
+  if (MODE == kSymPlanned && (int64_t)blockIdx.x >= *count) return;
   typedef typename Quad<T>::type Q;
   constexpr int kDup = sizeof(T) == 4 ? 2 : 1;  // copies of each 64-source group
   constexpr int kGroups = kSymTile / kWave;
@@ -323,11 +355,9 @@ __global__ __launch_bounds__(kBlock) void kmv_sym_kernel(
   __shared__ T red[kBlock / kWave][kSymTile];
   __shared__ T gbox[kGroups][6];
 
-  const int64_t p = blockIdx.x;
-  int64_t J = (int64_t)((sqrt(8.0 * (double)p + 1.0) - 1.0) * 0.5);
-  while (J * (J + 1) / 2 > p) --J;
-  while ((J + 1) * (J + 2) / 2 <= p) ++J;
-  const int64_t I = p - J * (J + 1) / 2;
+  const int64_t p = MODE == kSymPlanned ? (int64_t)list[blockIdx.x] : (int64_t)blockIdx.x;
+  int64_t I, J;
+  decode_pair(p, I, J);
 
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1);
@@ -363,23 +393,18 @@ __global__ __launch_bounds__(kBlock) void kmv_sym_kernel(
       }
     }
   }
-  T tlo[3] = {T(0), T(0), T(0)}, thi[3] = {T(0), T(0), T(0)};
+  T tbox[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
   if (SCREEN) {
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
-      tlo[d] = wave_min(fmin(pi[0][d], pi[1][d]));
-      thi[d] = wave_max(fmax(pi[0][d], pi[1][d]));
+      tbox[d] = wave_min(fmin(pi[0][d], pi[1][d]));
+      tbox[3 + d] = wave_max(fmax(pi[0][d], pi[1][d]));
     }
   }
   // true when every entry between this wave's targets and group g is dropped
   auto far_group = [&](int g) -> bool {
     if (!SCREEN) return false;
-    T d2 = T(0);
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) {
-      const T gap = fmax(T(0), fmax(gbox[g][d] - thi[d], tlo[d] - gbox[g][3 + d]));
-      d2 = fma(gap, gap, d2);
-    }
+    const T d2 = box_gap2<T, DIM>(tbox, gbox[g]);
     return __builtin_amdgcn_readfirstlane((int)(d2 > (T)drop_s)) != 0;
   };
   __syncthreads();
@@ -473,56 +498,68 @@ __global__ __launch_bounds__(kBlock) void kmv_sym_finish_kernel(int64_t n, int64
   }
 }
 
+inline int64_t sym_tiles(int64_t n);
+bool sym_eligible(const mplak_kernel_desc* d);
+
 inline bool use_symmetric(const mplak_kernel_desc* d) {
-  if (d->row_offset != d->col_offset || d->row_count != d->col_count || d->dim > 3) return false;
+  if (!sym_eligible(d)) return false;
   const int mode = sym_mode();
   return mode < 0 ? d->row_count >= kSymMinN : mode == 1;
 }
 
 inline int64_t sym_tiles(int64_t n) { return (n + kSymTile - 1) / kSymTile; }
 
-template <typename T, int KIND, int DIM, bool SCREEN>
-void launch_sym_dim(hipStream_t st, bool coinc, const mplak_kernel_desc* d, double scale, double thr,
-                    double drop_s, const double* x, double* pi, double* pj, const int32_t* skip) {
+struct SymArgs {
+  double scale, thr, drop_s;
+  const double* x;
+  double *pi, *pj;
+  const uint32_t* list;
+  const int64_t* count;
+  const int32_t* skip;
+};
+
+template <typename T, int KIND, int DIM, int MODE>
+void launch_sym_dim(hipStream_t st, bool coinc, const mplak_kernel_desc* d, const SymArgs& a) {
   const int64_t nt = sym_tiles(d->row_count);
   const dim3 grid((unsigned)(nt * (nt + 1) / 2));
   if (coinc)
-    hipLaunchKernelGGL((kmv_sym_kernel<T, KIND, DIM, true, SCREEN>), grid, dim3(kBlock), 0, st, d->points,
-                       d->row_offset, d->row_count, scale, thr, d->coincident_shift, drop_s, x, pi, pj, skip);
+    hipLaunchKernelGGL((kmv_sym_kernel<T, KIND, DIM, true, MODE>), grid, dim3(kBlock), 0, st, d->points,
+                       d->row_offset, d->row_count, a.scale, a.thr, d->coincident_shift, a.drop_s, a.x, a.pi, a.pj,
+                       a.list, a.count, a.skip);
   else
-    hipLaunchKernelGGL((kmv_sym_kernel<T, KIND, DIM, false, SCREEN>), grid, dim3(kBlock), 0, st, d->points,
-                       d->row_offset, d->row_count, scale, thr, d->coincident_shift, drop_s, x, pi, pj, skip);
+    hipLaunchKernelGGL((kmv_sym_kernel<T, KIND, DIM, false, MODE>), grid, dim3(kBlock), 0, st, d->points,
+                       d->row_offset, d->row_count, a.scale, a.thr, d->coincident_shift, a.drop_s, a.x, a.pi, a.pj,
+                       a.list, a.count, a.skip);
 }
 
-template <typename T, int KIND, bool SCREEN>
-void launch_sym(hipStream_t st, bool coinc, const mplak_kernel_desc* d, double scale, double thr, double drop_s,
-                const double* x, double* pi, double* pj, const int32_t* skip) {
+template <typename T, int KIND, int MODE>
+void launch_sym(hipStream_t st, bool coinc, const mplak_kernel_desc* d, const SymArgs& a) {
   switch (d->dim) {
-    case 1: launch_sym_dim<T, KIND, 1, SCREEN>(st, coinc, d, scale, thr, drop_s, x, pi, pj, skip); break;
-    case 2: launch_sym_dim<T, KIND, 2, SCREEN>(st, coinc, d, scale, thr, drop_s, x, pi, pj, skip); break;
-    default: launch_sym_dim<T, KIND, 3, SCREEN>(st, coinc, d, scale, thr, drop_s, x, pi, pj, skip); break;
+    case 1: launch_sym_dim<T, KIND, 1, MODE>(st, coinc, d, a); break;
+    case 2: launch_sym_dim<T, KIND, 2, MODE>(st, coinc, d, a); break;
+    default: launch_sym_dim<T, KIND, 3, MODE>(st, coinc, d, a); break;
   }
 }
 
-template <bool SCREEN>
-void launch_sym_any(hipStream_t st, int precision, bool coinc, const mplak_kernel_desc* d, double scale, double thr,
-                    double drop_s, const double* x, double* pi, double* pj, const int32_t* skip) {
+template <int MODE>
+void launch_sym_any(hipStream_t st, int precision, bool coinc, const mplak_kernel_desc* d, const SymArgs& a) {
   const bool gauss = d->kind == MPLAK_KERNEL_GAUSSIAN;
   if (precision == MPLAK_PREC_F64) {
     if (gauss)
-      launch_sym<double, MPLAK_KERNEL_GAUSSIAN, SCREEN>(st, coinc, d, scale, thr, drop_s, x, pi, pj, skip);
+      launch_sym<double, MPLAK_KERNEL_GAUSSIAN, MODE>(st, coinc, d, a);
     else
-      launch_sym<double, MPLAK_KERNEL_MATERN32, SCREEN>(st, coinc, d, scale, thr, drop_s, x, pi, pj, skip);
+      launch_sym<double, MPLAK_KERNEL_MATERN32, MODE>(st, coinc, d, a);
   } else {
     if (gauss)
-      launch_sym<float, MPLAK_KERNEL_GAUSSIAN, SCREEN>(st, coinc, d, scale, thr, drop_s, x, pi, pj, skip);
+      launch_sym<float, MPLAK_KERNEL_GAUSSIAN, MODE>(st, coinc, d, a);
     else
-      launch_sym<float, MPLAK_KERNEL_MATERN32, SCREEN>(st, coinc, d, scale, thr, drop_s, x, pi, pj, skip);
+      launch_sym<float, MPLAK_KERNEL_MATERN32, MODE>(st, coinc, d, a);
   }
 }
 
 bool sym_eligible(const mplak_kernel_desc* d) {
-  return d->row_offset == d->col_offset && d->row_count == d->col_count && d->dim <= 3;
+  return d->row_offset == d->col_offset && d->row_count == d->col_count && d->dim <= 3 &&
+         sym_tiles(d->row_count) < 65536;  // pair count fits a 1-D grid
 }
 
 size_t sym_workspace_bytes(int64_t n) {
@@ -530,20 +567,31 @@ size_t sym_workspace_bytes(int64_t n) {
   return (size_t)2 * (size_t)(nt * (nt + 1) / 2) * kSymTile * sizeof(double);
 }
 
-// Symmetric product: tile-pair kernel (screened or exact) and the finish.
-int run_sym(const mplak_kernel_desc* d, int precision, bool screen, double drop_s, const double* x, double* y,
-            void* work, const int32_t* skip_if, mplak_stream_t s) {
-  const double scale = coord_scale(d->kind, precision, d->length_scale);
+// Symmetric product: tile-pair kernel (exact, screened or planned) and the
+// finish.  `work` holds part_i then part_j ([pairs][512] doubles each).
+int run_sym(const mplak_kernel_desc* d, int precision, int mode, double drop_s, const uint32_t* list,
+            const int64_t* count, const double* x, double* y, void* work, const int32_t* skip_if, mplak_stream_t s) {
+  SymArgs a;
+  a.scale = coord_scale(d->kind, precision, d->length_scale);
   const bool coinc = d->coincident_shift != 0.0;
-  const double thr = (1e-14 * scale) * (1e-14 * scale);
+  a.thr = (1e-14 * a.scale) * (1e-14 * a.scale);
+  a.drop_s = drop_s;
+  a.x = x;
+  a.list = list;
+  a.count = count;
+  a.skip = skip_if;
   const int64_t nt = sym_tiles(d->row_count);
-  double* pi = static_cast<double*>(work);
-  double* pj = pi + (size_t)(nt * (nt + 1) / 2) * kSymTile;
+  a.pi = static_cast<double*>(work);
+  a.pj = a.pi + (size_t)(nt * (nt + 1) / 2) * kSymTile;
+  double* pi = a.pi;
+  double* pj = a.pj;
   hipStream_t hs = as_stream(s);
-  if (screen)
-    launch_sym_any<true>(hs, precision, coinc, d, scale, thr, drop_s, x, pi, pj, skip_if);
+  if (mode == kSymPlanned)
+    launch_sym_any<kSymPlanned>(hs, precision, coinc, d, a);
+  else if (mode == kSymScreened)
+    launch_sym_any<kSymScreened>(hs, precision, coinc, d, a);
   else
-    launch_sym_any<false>(hs, precision, coinc, d, scale, thr, drop_s, x, pi, pj, skip_if);
+    launch_sym_any<kSymExact>(hs, precision, coinc, d, a);
   int st = launch_status();
   if (st != MPLAK_OK) return st;
   int64_t fb = (d->row_count + kBlock - 1) / kBlock;
@@ -568,6 +616,149 @@ double drop_distance2(int kind, int precision, double drop_log2) {
     else hi = mid;
   }
   return hi * hi;
+}
+
+
+// ---------------------------------------------------------------------------
+// Screening plan: which tile pairs of the symmetric product have at least one
+// kept 128 x 64 block (same test as the screened kernel, on the same
+// precision-cast coordinates), ordered by kept block count, heaviest first
+// (ties by pair index), so that the planned product launches no all-dropped
+// pair and starts the longest workgroups first.  The partials of the dropped
+// pairs are zeroed once at build time and never written again.
+// ---------------------------------------------------------------------------
+
+constexpr int kSymGroups = kSymTile / kWave;
+constexpr uint64_t kPlanDropped = 63;  // key class of pairs with no kept block
+
+struct PlanLayout {
+  size_t count, boxes, list, keys_in, keys_out, tmp, tmp_bytes, part, total;
+};
+
+PlanLayout plan_layout(int64_t n) {
+  PlanLayout L{};
+  const int64_t nt = sym_tiles(n);
+  const size_t np = (size_t)(nt * (nt + 1) / 2);
+  L.count = 0;
+  L.boxes = 256;
+  L.list = L.boxes + align_up(sizeof(double) * 6 * kSymGroups * (size_t)nt, 256);
+  L.keys_in = L.list + align_up(sizeof(uint32_t) * np, 256);
+  L.keys_out = L.keys_in + align_up(sizeof(uint64_t) * np, 256);
+  L.tmp = L.keys_out + align_up(sizeof(uint64_t) * np, 256);
+  size_t tb = 0;
+  if (rocprim::radix_sort_keys(nullptr, tb, static_cast<uint64_t*>(nullptr), static_cast<uint64_t*>(nullptr),
+                               (unsigned)np, 32, 38) != hipSuccess)
+    return PlanLayout{};
+  L.tmp_bytes = tb;
+  L.part = L.tmp + align_up(tb, 256);
+  L.total = L.part + 2 * np * kSymTile * sizeof(double);
+  return L;
+}
+
+// Bounding box of every 64-point group of every tile, from the same
+// precision-cast scaled coordinates the product kernel uses (padding points
+// are clamped to the last point, as there).
+template <typename T, int DIM>
+__global__ __launch_bounds__(kBlock) void plan_box_kernel(const double* const* __restrict__ points, int64_t off,
+                                                          int64_t n, double scale, double* __restrict__ boxes) {
+  const int lane = threadIdx.x & (kWave - 1);
+  for (int k = threadIdx.x; k < kSymTile; k += kBlock) {
+    const int64_t j = (int64_t)blockIdx.x * kSymTile + k;
+    const int64_t jc = j < n ? j : n - 1;
+    T lo[3] = {T(0), T(0), T(0)}, hi[3] = {T(0), T(0), T(0)};
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      const T v = (T)(points[d][off + jc] * scale);
+      lo[d] = wave_min(v);
+      hi[d] = wave_max(v);
+    }
+    if (lane == 0) {
+      double* b = boxes + ((size_t)blockIdx.x * kSymGroups + k / kWave) * 6;
+      for (int d = 0; d < 3; ++d) {
+        b[d] = (double)lo[d];
+        b[3 + d] = (double)hi[d];
+      }
+    }
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void load_box(const double* b, T* out) {
+#pragma unroll
+  for (int d = 0; d < 6; ++d) out[d] = (T)b[d];
+}
+
+// key[p] = (32 - kept blocks) << 32 | p, or kPlanDropped << 32 | p.
+template <typename T, int DIM>
+__global__ __launch_bounds__(kBlock) void plan_pair_kernel(const double* __restrict__ boxes, int64_t np,
+                                                           double drop_s, uint64_t* __restrict__ keys) {
+  const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (p >= np) return;
+  int64_t I, J;
+  decode_pair(p, I, J);
+  int kept = 0;
+  for (int w = 0; w < kBlock / kWave; ++w) {
+    T a[6], b[6], t[6];
+    load_box(boxes + ((size_t)I * kSymGroups + 2 * w) * 6, a);
+    load_box(boxes + ((size_t)I * kSymGroups + 2 * w + 1) * 6, b);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      t[d] = fmin(a[d], b[d]);
+      t[3 + d] = fmax(a[3 + d], b[3 + d]);
+    }
+    for (int g = 0; g < kSymGroups; ++g) {
+      T gb[6];
+      load_box(boxes + ((size_t)J * kSymGroups + g) * 6, gb);
+      kept += box_gap2<T, DIM>(t, gb) > (T)drop_s ? 0 : 1;
+    }
+  }
+  keys[p] = ((kept > 0 ? (uint64_t)(32 - kept) : kPlanDropped) << 32) | (uint64_t)p;
+}
+
+__global__ __launch_bounds__(kBlock) void plan_zero_kernel(const uint64_t* __restrict__ keys, int64_t np,
+                                                           double* __restrict__ part_i, double* __restrict__ part_j) {
+  for (int64_t p = blockIdx.x; p < np; p += gridDim.x) {
+    if ((keys[p] >> 32) != kPlanDropped) continue;
+    for (int k = threadIdx.x; k < kSymTile; k += kBlock) {
+      part_i[p * kSymTile + k] = 0.0;
+      part_j[p * kSymTile + k] = 0.0;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void plan_list_kernel(const uint64_t* __restrict__ keys, int64_t np,
+                                                           uint32_t* __restrict__ list, int64_t* __restrict__ count) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= np) return;
+  list[i] = (uint32_t)(keys[i] & 0xffffffffull);
+  const bool kept = (keys[i] >> 32) != kPlanDropped;
+  if (kept && (i + 1 == np || (keys[i + 1] >> 32) == kPlanDropped)) *count = i + 1;
+  if (i == 0 && !kept) *count = 0;
+}
+
+template <typename T>
+void launch_plan_geometry(hipStream_t st, const mplak_kernel_desc* d, double scale, double drop_s, double* boxes,
+                          uint64_t* keys) {
+  const int64_t nt = sym_tiles(d->row_count);
+  const int64_t np = nt * (nt + 1) / 2;
+  const unsigned pg = (unsigned)((np + kBlock - 1) / kBlock);
+  switch (d->dim) {
+    case 1:
+      hipLaunchKernelGGL((plan_box_kernel<T, 1>), dim3((unsigned)nt), dim3(kBlock), 0, st, d->points, d->row_offset,
+                         d->row_count, scale, boxes);
+      hipLaunchKernelGGL((plan_pair_kernel<T, 1>), dim3(pg), dim3(kBlock), 0, st, boxes, np, drop_s, keys);
+      break;
+    case 2:
+      hipLaunchKernelGGL((plan_box_kernel<T, 2>), dim3((unsigned)nt), dim3(kBlock), 0, st, d->points, d->row_offset,
+                         d->row_count, scale, boxes);
+      hipLaunchKernelGGL((plan_pair_kernel<T, 2>), dim3(pg), dim3(kBlock), 0, st, boxes, np, drop_s, keys);
+      break;
+    default:
+      hipLaunchKernelGGL((plan_box_kernel<T, 3>), dim3((unsigned)nt), dim3(kBlock), 0, st, d->points, d->row_offset,
+                         d->row_count, scale, boxes);
+      hipLaunchKernelGGL((plan_pair_kernel<T, 3>), dim3(pg), dim3(kBlock), 0, st, boxes, np, drop_s, keys);
+      break;
+  }
 }
 
 int check_desc(const mplak_kernel_desc* d) {
@@ -611,7 +802,70 @@ int mplak_kernel_matvec_screened(const mplak_kernel_desc* d, int precision, doub
   if (d->row_count == 0) return MPLAK_OK;
   if (x == nullptr || y == nullptr) return MPLAK_ERR_ARG;
   if (work == nullptr || work_bytes < sym_workspace_bytes(d->row_count)) return MPLAK_ERR_WORKSPACE;
-  return run_sym(d, precision, true, drop_distance2(d->kind, precision, drop_log2), x, y, work, skip_if, s);
+  return run_sym(d, precision, kSymScreened, drop_distance2(d->kind, precision, drop_log2), nullptr, nullptr, x, y,
+                 work, skip_if, s);
+}
+
+size_t mplak_screen_plan_bytes(const mplak_kernel_desc* d) {
+  if (d == nullptr || d->row_count <= 0 || !sym_eligible(d)) return 0;
+  return plan_layout(d->row_count).total;
+}
+
+int mplak_screen_plan_build(const mplak_kernel_desc* d, int precision, double drop_log2, void* plan,
+                            size_t plan_bytes, mplak_stream_t s) {
+  int st = check_desc(d);
+  if (st != MPLAK_OK) return st;
+  if (precision != MPLAK_PREC_F64 && precision != MPLAK_PREC_F32) return MPLAK_ERR_ARG;
+  if (!(drop_log2 > 0.0) || drop_log2 > 2000.0) return MPLAK_ERR_ARG;
+  if (!sym_eligible(d) || d->row_count == 0) return MPLAK_ERR_ARG;
+  const PlanLayout L = plan_layout(d->row_count);
+  if (L.total == 0) return MPLAK_ERR_HIP;
+  if (plan == nullptr || plan_bytes < L.total) return MPLAK_ERR_WORKSPACE;
+  char* base = static_cast<char*>(plan);
+  const int64_t nt = sym_tiles(d->row_count);
+  const int64_t np = nt * (nt + 1) / 2;
+  double* boxes = reinterpret_cast<double*>(base + L.boxes);
+  uint64_t* kin = reinterpret_cast<uint64_t*>(base + L.keys_in);
+  uint64_t* kout = reinterpret_cast<uint64_t*>(base + L.keys_out);
+  double* pi = reinterpret_cast<double*>(base + L.part);
+  double* pj = pi + (size_t)np * kSymTile;
+  const double scale = coord_scale(d->kind, precision, d->length_scale);
+  const double drop_s = drop_distance2(d->kind, precision, drop_log2);
+  hipStream_t hs = as_stream(s);
+  if (precision == MPLAK_PREC_F64)
+    launch_plan_geometry<double>(hs, d, scale, drop_s, boxes, kin);
+  else
+    launch_plan_geometry<float>(hs, d, scale, drop_s, boxes, kin);
+  st = launch_status();
+  if (st != MPLAK_OK) return st;
+  hipLaunchKernelGGL(plan_zero_kernel, dim3((unsigned)std::min<int64_t>(np, 4096)), dim3(kBlock), 0, hs, kin, np, pi,
+                     pj);
+  st = launch_status();
+  if (st != MPLAK_OK) return st;
+  size_t tb = L.tmp_bytes;
+  if (rocprim::radix_sort_keys(base + L.tmp, tb, kin, kout, (unsigned)np, 32, 38, hs) != hipSuccess)
+    return MPLAK_ERR_HIP;
+  hipLaunchKernelGGL(plan_list_kernel, dim3((unsigned)((np + kBlock - 1) / kBlock)), dim3(kBlock), 0, hs, kout, np,
+                     reinterpret_cast<uint32_t*>(base + L.list), reinterpret_cast<int64_t*>(base + L.count));
+  return launch_status();
+}
+
+int mplak_kernel_matvec_planned(const mplak_kernel_desc* d, int precision, double drop_log2, void* plan,
+                                size_t plan_bytes, const double* x, double* y, const int32_t* skip_if,
+                                mplak_stream_t s) {
+  int st = check_desc(d);
+  if (st != MPLAK_OK) return st;
+  if (precision != MPLAK_PREC_F64 && precision != MPLAK_PREC_F32) return MPLAK_ERR_ARG;
+  if (!(drop_log2 > 0.0) || drop_log2 > 2000.0) return MPLAK_ERR_ARG;
+  if (!sym_eligible(d) || d->row_count == 0) return MPLAK_ERR_ARG;
+  if (x == nullptr || y == nullptr) return MPLAK_ERR_ARG;
+  const PlanLayout L = plan_layout(d->row_count);
+  if (L.total == 0) return MPLAK_ERR_HIP;
+  if (plan == nullptr || plan_bytes < L.total) return MPLAK_ERR_WORKSPACE;
+  char* base = static_cast<char*>(plan);
+  return run_sym(d, precision, kSymPlanned, drop_distance2(d->kind, precision, drop_log2),
+                 reinterpret_cast<const uint32_t*>(base + L.list), reinterpret_cast<const int64_t*>(base + L.count),
+                 x, y, base + L.part, skip_if, s);
 }
 
 int mplak_kernel_matvec(const mplak_kernel_desc* d, int precision, const double* x, double* y,
@@ -625,7 +879,7 @@ int mplak_kernel_matvec(const mplak_kernel_desc* d, int precision, const double*
   if (x == nullptr) return MPLAK_ERR_ARG;
   const size_t need = mplak_kernel_matvec_workspace_bytes(d);
   if (work == nullptr || work_bytes < need) return MPLAK_ERR_WORKSPACE;
-  if (use_symmetric(d)) return run_sym(d, precision, false, 0.0, x, y, work, skip_if, s);
+  if (use_symmetric(d)) return run_sym(d, precision, kSymExact, 0.0, nullptr, nullptr, x, y, work, skip_if, s);
 
   const double scale = coord_scale(d->kind, precision, d->length_scale);
   const bool coinc = d->coincident_shift != 0.0;

@@ -5,8 +5,9 @@ Morton keys (exact integer work: bitwise); gather/scatter are exact inverses;
 the screened product with nothing to drop is bitwise the exact symmetric
 product; with a drop level of 2^-30 on ordered points every row differs from
 the oracle's exact FP64 product by at most the dropped mass bound
-n * 2^-30 * max|x| plus the FP32 product tolerance; the mixed-precision solve
-on ordered points reaches the FP64 residual bar.
+n * 2^-30 * max|x| plus the FP32 product tolerance; the planned product
+(screening plan built once, all-dropped tile pairs never launched) is bitwise
+the screened product, also when the plan is reused for new vectors.
 """
 import ctypes as C
 import math
@@ -165,3 +166,51 @@ def test_screened_falls_back_to_exact_for_rectangular_blocks(lib, oracle):
     y = empty(900)
     assert lib.mplak_kernel_matvec_screened(C.byref(d), 0, 0.0, X.data_ptr(), y.data_ptr(), None, 0, None,
                                             stream()) == _native.MPLAK_ERR_ARG
+
+
+def build_plan(lib, desc, prec, drop):
+    wb = lib.mplak_screen_plan_bytes(C.byref(desc))
+    assert wb > 0
+    plan = workspace(wb)
+    ck(lib.mplak_screen_plan_build(C.byref(desc), prec, drop, plan.data_ptr(), wb, stream()))
+    return plan, wb
+
+
+def planned(lib, desc, prec, drop, plan, wb, X):
+    y = empty(desc.row_count)
+    ck(lib.mplak_kernel_matvec_planned(C.byref(desc), prec, drop, plan.data_ptr(), wb, X.data_ptr(), y.data_ptr(),
+                                       None, stream()))
+    return host(y)
+
+
+@pytest.mark.parametrize("kind,l,dim,n", [(0, 0.05, 3, 20000), (1, 0.1, 3, 7000), (0, 0.02, 2, 9000),
+                                          (0, 0.01, 1, 5000), (0, 0.05, 3, 511)])
+@pytest.mark.parametrize("drop", [20.0, 30.0])
+def test_planned_product_equals_screened_bitwise(lib, oracle, kind, l, dim, n, drop):
+    pts = ordered_points(lib, oracle, n, dim, 70 + dim)
+    P = dev(pts)
+    T = point_table(P)
+    d = kernel_desc(P, T, kind, l, nugget=1.0)
+    for prec in (1, 0):
+        plan, wb = build_plan(lib, d, prec, drop)
+        # the plan is reused across products with different vectors
+        for seed in (1, 2):
+            X = dev(oracle.uniform(n, 71, seed, -1, 1))
+            assert np.array_equal(planned(lib, d, prec, drop, plan, wb, X), screened(lib, d, X, prec, drop))
+
+
+def test_planned_product_unordered_points_and_errors(lib, oracle):
+    n = 3000
+    pts = oracle.uniform(3 * n, 12, 0).reshape(3, n)  # no spatial order: nothing can be dropped
+    P = dev(pts)
+    T = point_table(P)
+    d = kernel_desc(P, T, 0, 0.05, nugget=1.0)
+    x = oracle.uniform(n, 12, 1, -1, 1)
+    X = dev(x)
+    plan, wb = build_plan(lib, d, 0, 30.0)
+    y = planned(lib, d, 0, 30.0, plan, wb, X)
+    assert rel_err(y, oracle.Kernel(pts, 0, 0.05, nugget=1.0).matvec(x, mode=1)) <= 1e-7
+    assert lib.mplak_kernel_matvec_planned(C.byref(d), 0, 30.0, plan.data_ptr(), wb - 8, X.data_ptr(),
+                                           X.data_ptr(), None, stream()) == _native.MPLAK_ERR_WORKSPACE
+    rect = kernel_desc(P, T, 0, 0.05, rows=100, cols=200)
+    assert lib.mplak_screen_plan_bytes(C.byref(rect)) == 0
