@@ -37,6 +37,8 @@ EPS = 1e-8
 SEED = 1235
 ITER_MAX = 20001
 FLOP_PER_ENTRY = 3 * DIM + 4  # d subs, d FMA, 1 scale, 1 exp, 1 FMA accumulate
+DROP_LOG2 = 20.0  # the mixed solver's default drop level at its default inner tolerance 1e-4
+PEAK_FP32_NONPACKED = 78.6e12  # one FP32 FMA per lane per cycle; the 157.3 TF figure counts packed FP32
 PEAK_FP32 = 157.3e12
 PEAK_FP64 = 78.6e12
 PEAK_HBM = 8.0e12
@@ -313,9 +315,69 @@ def product_desc(table):
     return desc
 
 
+def ordered_problem(K, table, stream_ptr):
+    """The solver's inner operator: the points in Morton order (as the mixed
+    solver arranges them) and a screening plan at the default drop level."""
+    import torch
+    from mpla_amd import _native
+    desc = product_desc(table)
+    perm = torch.empty(N, dtype=torch.int64, device="cuda")
+    ob = K.mplak_spatial_order_workspace_bytes(N, DIM)
+    ow = torch.empty(ob // 8 + 1, dtype=torch.float64, device="cuda")
+    _native.check(K.mplak_spatial_order(table.data_ptr(), DIM, 0, N, perm.data_ptr(), ow.data_ptr(), ob, stream_ptr))
+    pts = torch.empty(DIM, N, dtype=torch.float64, device="cuda")
+    _native.check(K.mplak_gather_points(table.data_ptr(), DIM, 0, perm.data_ptr(), N, pts.data_ptr(), N, stream_ptr))
+    otable = torch.tensor([pts[d].data_ptr() for d in range(DIM)], dtype=torch.int64, device="cuda")
+    odesc = product_desc(otable)
+    pb = K.mplak_screen_plan_bytes(C.byref(odesc))
+    plan = torch.empty(pb // 8 + 1, dtype=torch.float64, device="cuda")
+    _native.check(K.mplak_screen_plan_build(C.byref(odesc), _native.MPLAK_PREC_F32, DROP_LOG2, plan.data_ptr(), pb,
+                                            stream_ptr))
+    torch.cuda.synchronize()
+    return {"desc": odesc, "plan": plan, "plan_bytes": pb, "keep": (pts, otable, perm)}
+
+
+def measure_inner_product(K, table, xv, yv, stream_ptr, ext, reps=20):
+    """Launch time (HIP events on the library stream) of the planned screened
+    FP32 product, the kernel that dominates a mixed solve, and its executed
+    work: every kept 128 x 64 block is 8192 pair evaluations of 15 FLOP
+    (13 + the second accumulation) off the diagonal tiles, 13 FLOP on them."""
+    import torch
+    from mpla_amd import _native
+    op = ordered_problem(K, table, stream_ptr)
+    d, plan, pb = op["desc"], op["plan"], op["plan_bytes"]
+
+    def launch():
+        _native.check(K.mplak_kernel_matvec_planned(C.byref(d), _native.MPLAK_PREC_F32, DROP_LOG2, plan.data_ptr(), pb,
+                                                    xv.data_ptr(), yv.data_ptr(), None, stream_ptr))
+
+    for _ in range(2):
+        launch()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(ext)
+    for _ in range(reps):
+        launch()
+    e1.record(ext)
+    e1.synchronize()
+    t = e0.elapsed_time(e1) / reps * 1e-3
+    counts = torch.zeros(3, dtype=torch.int64, device="cuda")
+    _native.check(K.mplak_screen_plan_counts(C.byref(d), plan.data_ptr(), pb, counts.data_ptr(), stream_ptr))
+    torch.cuda.synchronize()
+    kept_pairs, kept, diag = (int(v) for v in counts.cpu())
+    nt = (N + 511) // 512
+    flop = 8192.0 * ((kept - diag) * (FLOP_PER_ENTRY + 2) + diag * FLOP_PER_ENTRY)
+    return {"kernel": "kmv_sym_kernel<float, GAUSSIAN, 3, planned> + kmv_sym_finish_kernel",
+            "launch_ms": t * 1e3, "drop_log2": DROP_LOG2, "kept_pairs": kept_pairs,
+            "all_pairs": nt * (nt + 1) // 2, "kept_blocks": kept, "all_blocks": 32 * nt * (nt + 1) // 2,
+            "executed_flop": flop, "executed_tflops": flop / t / 1e12,
+            "reference_equivalent_tflops": FLOP_PER_ENTRY * float(N) * N / t / 1e12}
+
+
 def profile_kernels(K, table, stream_ptr):
     """Launch the measured kernels a fixed number of times, for counter runs:
-    3 FP32 and 2 FP64 Gaussian products (N=65536), 3 dgemv (65536 x 65536) and
+    3 FP32 and 2 FP64 exact Gaussian products (N=65536), 3 planned screened
+    FP32 products on the ordered points, 3 dgemv (65536 x 65536) and
     3 copies of 2^27 doubles (1 GiB read + 1 GiB written, the traffic
     calibration)."""
     import torch
@@ -331,6 +393,13 @@ def profile_kernels(K, table, stream_ptr):
             _native.check(K.mplak_kernel_matvec(C.byref(desc), prec, xv.data_ptr(), yv.data_ptr(), work.data_ptr(),
                                                 wb, None, stream_ptr))
     torch.cuda.synchronize()
+    op = ordered_problem(K, table, stream_ptr)
+    for _ in range(3):
+        _native.check(K.mplak_kernel_matvec_planned(C.byref(op["desc"]), _native.MPLAK_PREC_F32, DROP_LOG2,
+                                                    op["plan"].data_ptr(), op["plan_bytes"], xv.data_ptr(),
+                                                    yv.data_ptr(), None, stream_ptr))
+    torch.cuda.synchronize()
+    del op
     log("products done")
     del work
     n = 65536
@@ -496,6 +565,9 @@ def main():
     log(f"kernel matvec: FP32 {t32 * 1e3:.3f} ms ({entries / t32:.3e} entries/s), "
         f"FP64 {t64 * 1e3:.3f} ms ({entries / t64:.3e} entries/s)")
     del work
+    inner = measure_inner_product(K, table, xv, yv, stream_ptr, ext)
+    log(f"inner product (ordered, screened at 2^-{DROP_LOG2:g}, planned): {inner['launch_ms']:.3f} ms, "
+        f"{inner['kept_blocks']} of {inner['all_blocks']} blocks, {inner['executed_tflops']:.1f} TFLOP/s executed")
 
     # ---- dense dgemv at N = 65536 (34 GB, far above the 256 MB MALL) ---------
     dgemv = None
@@ -580,14 +652,20 @@ def main():
                        "converged": bool(conv_plain),
                        "latency_ms": {"p50": percentile(plat, 50) * 1e3, "p99": percentile(plat, 99) * 1e3}},
         "speedup_mixed_vs_plain": value / plain_sps,
-        "roofline": {"bound": "valu", "achieved": ach32 / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
-                     "frac": ach32 / PEAK_FP32, "traffic": None,
-                     "kernel": "kmv_sym_kernel<float, GAUSSIAN, 3> + kmv_sym_finish_kernel",
-                     "launch_ms": t32 * 1e3, "entries_per_s": entries / t32,
-                     "flop_per_entry": FLOP_PER_ENTRY,
-                     "basis": "algorithmic: N^2 entries x 13 FLOP per product; the symmetric kernel evaluates "
-                              "each pair once (N(N+1)/2 evaluations, 15 FLOP each incl. both accumulations)",
-                     "executed_tflops": exec32 / 1e12, "executed_frac": exec32 / PEAK_FP32},
+        "roofline": {"bound": "valu", "achieved": inner["executed_tflops"], "peak": PEAK_FP32 / 1e12,
+                     "unit": "TFLOP/s", "frac": inner["executed_tflops"] * 1e12 / PEAK_FP32, "traffic": None,
+                     "kernel": inner["kernel"], "launch_ms": inner["launch_ms"],
+                     "basis": "FLOP of the evaluated entries (kept 128x64 blocks x 8192 pairs x 15 FLOP off the "
+                              "diagonal tiles, 13 on them) per launch / launch time (HIP events, library stream)",
+                     "frac_of_nonpacked_fp32": inner["executed_tflops"] * 1e12 / PEAK_FP32_NONPACKED,
+                     "inner_product": inner},
+        "roofline_exact_fp32": {"bound": "valu", "achieved": ach32 / 1e12, "peak": PEAK_FP32 / 1e12,
+                                "unit": "TFLOP/s", "frac": ach32 / PEAK_FP32,
+                                "kernel": "kmv_sym_kernel<float, GAUSSIAN, 3> + kmv_sym_finish_kernel",
+                                "launch_ms": t32 * 1e3, "entries_per_s": entries / t32,
+                                "basis": "algorithmic: N^2 entries x 13 FLOP per exact product; the symmetric "
+                                         "kernel evaluates each pair once (N(N+1)/2 evaluations, 15 FLOP each)",
+                                "executed_tflops": exec32 / 1e12, "executed_frac": exec32 / PEAK_FP32},
         "roofline_fp64_kernel": {"bound": "valu", "achieved": ach64 / 1e12, "peak": PEAK_FP64 / 1e12,
                                  "unit": "TFLOP/s", "frac": ach64 / PEAK_FP64, "launch_ms": t64 * 1e3,
                                  "entries_per_s": entries / t64},

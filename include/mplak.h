@@ -145,7 +145,8 @@ int mplak_kernel_matvec(const mplak_kernel_desc* desc, int precision, const doub
  * between 128-point target blocks and 64-point source blocks whose bounding
  * boxes are so far apart that every entry is below 2^-drop_log2 (the kernel's
  * value at distance 0 is 1) are treated as zero; everything else is computed
- * exactly as by mplak_kernel_matvec.  Other blocks get the exact product.
+ * by the symmetric product mplak_kernel_matvec uses for large square blocks
+ * (at every size here).  Other blocks get mplak_kernel_matvec's product.
  * Pays off when consecutive points are spatially close (see
  * mplak_spatial_order).  0 < drop_log2 <= 2000.  Deterministic. */
 size_t mplak_kernel_matvec_screened_workspace_bytes(const mplak_kernel_desc* desc);
@@ -169,6 +170,11 @@ int mplak_screen_plan_build(const mplak_kernel_desc* desc, int precision, double
 int mplak_kernel_matvec_planned(const mplak_kernel_desc* desc, int precision, double drop_log2, void* plan,
                                 size_t plan_bytes, const double* x, double* y, const int32_t* skip_if,
                                 mplak_stream_t stream);
+/* Writes counts[0] = tile pairs the plan keeps, counts[1] = 128 x 64 blocks
+ * they hold that pass the screen, counts[2] = those of counts[1] on diagonal
+ * tile pairs (counts: 3 int64 in device memory). */
+int mplak_screen_plan_counts(const mplak_kernel_desc* desc, const void* plan, size_t plan_bytes, int64_t* counts,
+                             mplak_stream_t stream);
 
 /* No reference counterpart.  Spatial (Z-order / Morton) permutation of the
  * points p[d][offset + i], i < n: perm[k] is the index i of the k-th point

@@ -122,6 +122,7 @@ MPLAK_SIGNATURES = {
     "mplak_screen_plan_build": (C.c_int, [C.POINTER(mplak_kernel_desc), C.c_int, _dbl, _vp, _sz, _vp]),
     "mplak_kernel_matvec_planned": (C.c_int, [C.POINTER(mplak_kernel_desc), C.c_int, _dbl, _vp, _sz, _vp, _vp, _vp,
                                               _vp]),
+    "mplak_screen_plan_counts": (C.c_int, [C.POINTER(mplak_kernel_desc), _vp, _sz, _vp, _vp]),
     "mplak_spatial_order_workspace_bytes": (_sz, [_i64, _i32]),
     "mplak_spatial_order": (C.c_int, [_vp, _i32, _i64, _i64, _vp, _vp, _sz, _vp]),
     "mplak_gather_points": (C.c_int, [_vp, _i32, _i64, _vp, _i64, _vp, _i64, _vp]),

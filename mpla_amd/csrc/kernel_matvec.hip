@@ -351,9 +351,17 @@ __global__ __launch_bounds__(kBlock) void kmv_sym_kernel(
   typedef typename Quad<T>::type Q;
   constexpr int kDup = sizeof(T) == 4 ? 2 : 1;  // copies of each 64-source group
   constexpr int kGroups = kSymTile / kWave;
+  // Column-side sums of the four waves, red[w * kSymTile + k].  In FP32 each
+  // wave holds its 8 group sums in registers and they reuse the source buffer
+  // once every wave is done with it (16 KB of LDS per workgroup, 8 resident
+  // workgroups per CU); in FP64 registers are the scarcer resource, and the
+  // sums go to a buffer of their own as each group completes.
+  constexpr bool kHoldSums = sizeof(T) == 4;
+  static_assert(sizeof(Q) * kGroups * kDup * kWave >= sizeof(T) * (kBlock / kWave) * kSymTile, "LDS reuse");
   __shared__ Q src[kGroups][kDup * kWave];
-  __shared__ T red[kBlock / kWave][kSymTile];
   __shared__ T gbox[kGroups][6];
+  __shared__ T own_red[kHoldSums ? 1 : (kBlock / kWave) * kSymTile];
+  T* red = kHoldSums ? reinterpret_cast<T*>(&src[0][0]) : own_red;
 
   const int64_t p = MODE == kSymPlanned ? (int64_t)list[blockIdx.x] : (int64_t)blockIdx.x;
   int64_t I, J;
@@ -443,25 +451,33 @@ __global__ __launch_bounds__(kBlock) void kmv_sym_kernel(
       }
     }
   } else {
-#pragma unroll 1
-    for (int g = 0; g < kGroups; ++g) {
+    auto group_sum = [&](int g) -> T {
       T t = T(0);
-      if (far_group(g)) {
-        red[w][g * kWave + lane] = t;
-        continue;
-      }
+      if (!far_group(g)) {
 #pragma unroll 4
-      for (int s = 0; s < kWave; ++s) {
-        const Q q = kDup == 2 ? src[g][lane + s] : src[g][(lane + s) & (kWave - 1)];
-        T e[2];
-        entry2(q, e);
-        accI[0] = fma(e[0], q.w, accI[0]);
-        accI[1] = fma(e[1], q.w, accI[1]);
-        t = fma(e[0], xi[0], t);
-        t = fma(e[1], xi[1], t);
-        t = wave_rotate(t);
+        for (int s = 0; s < kWave; ++s) {
+          const Q q = kDup == 2 ? src[g][lane + s] : src[g][(lane + s) & (kWave - 1)];
+          T e[2];
+          entry2(q, e);
+          accI[0] = fma(e[0], q.w, accI[0]);
+          accI[1] = fma(e[1], q.w, accI[1]);
+          t = fma(e[0], xi[0], t);
+          t = fma(e[1], xi[1], t);
+          t = wave_rotate(t);
+        }
       }
-      red[w][g * kWave + lane] = t;
+      return t;
+    };
+    if constexpr (kHoldSums) {
+      T tg[kGroups];
+#pragma unroll
+      for (int g = 0; g < kGroups; ++g) tg[g] = group_sum(g);
+      __syncthreads();  // every wave is done reading src
+#pragma unroll
+      for (int g = 0; g < kGroups; ++g) red[w * kSymTile + g * kWave + lane] = tg[g];
+    } else {
+#pragma unroll 1
+      for (int g = 0; g < kGroups; ++g) red[w * kSymTile + g * kWave + lane] = group_sum(g);
     }
   }
 #pragma unroll
@@ -472,29 +488,60 @@ __global__ __launch_bounds__(kBlock) void kmv_sym_kernel(
   if (I != J) {
     __syncthreads();
     for (int k = tid; k < kSymTile; k += kBlock) {
-      double s = (double)red[0][k];
+      double s = (double)red[k];
 #pragma unroll
-      for (int q = 1; q < kBlock / kWave; ++q) s += (double)red[q][k];
+      for (int q = 1; q < kBlock / kWave; ++q) s += (double)red[q * kSymTile + k];
       part_j[p * kSymTile + k] = s;
     }
   }
 }
 
-// y_i = sum over partner tiles P (ascending) of the tile's partials, plus the nugget term.
-__global__ __launch_bounds__(kBlock) void kmv_sym_finish_kernel(int64_t n, int64_t ntiles,
-                                                                const double* __restrict__ part_i,
-                                                                const double* __restrict__ part_j, double nugget,
-                                                                const double* __restrict__ x, double* __restrict__ y,
-                                                                const int32_t* __restrict__ skip) {
+// y_i = sum over partner tiles P of row i's tile X of that pair's partial,
+// plus the nugget term.  A workgroup takes 64 consecutive rows (one tile) and
+// four partner slices: wave k sums the partners P = k (mod 4) in ascending
+// order, then the four slice sums are added in slice order.  With a partner
+// list (planned product) a wave visits only the listed partners of its slice,
+// in the same order; the partials it leaves out are zero, so the result is
+// the same.
+constexpr int kFinishRows = kWave;
+constexpr int kFinishSlices = kBlock / kWave;
+
+__device__ __forceinline__ double sym_partial(const double* __restrict__ part_i, const double* __restrict__ part_j,
+                                              int64_t X, int64_t P, int64_t t) {
+  return P < X ? part_j[sym_pair_index(P, X) * kSymTile + t] : part_i[sym_pair_index(X, P) * kSymTile + t];
+}
+
+template <bool LIST>
+__global__ __launch_bounds__(kBlock) void kmv_sym_finish_kernel(
+    int64_t n, int64_t ntiles, const double* __restrict__ part_i, const double* __restrict__ part_j, double nugget,
+    const double* __restrict__ x, double* __restrict__ y, const uint32_t* __restrict__ partners,
+    const int32_t* __restrict__ pcount, int64_t pcap, const int32_t* __restrict__ skip) {
   if (skip != nullptr && *skip != 0) return;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const int64_t X = i / kSymTile, t = i % kSymTile;
+  __shared__ double red[kFinishSlices][kFinishRows];
+  const int r = threadIdx.x % kFinishRows, k = threadIdx.x / kFinishRows;
+  for (int64_t row0 = (int64_t)blockIdx.x * kFinishRows; row0 < n; row0 += (int64_t)gridDim.x * kFinishRows) {
+    const int64_t i = row0 + r;
+    const int64_t X = row0 / kSymTile, t = i % kSymTile;
     double s = 0.0;
-    for (int64_t P = 0; P < ntiles; ++P)
-      s += P < X ? part_j[sym_pair_index(P, X) * kSymTile + t] : part_i[sym_pair_index(X, P) * kSymTile + t];
-    if (nugget != 0.0) s = fma(nugget, x[i], s);
-    y[i] = s;
+    if (i < n) {
+      if (LIST) {
+        const uint32_t* lst = partners + (X * kFinishSlices + k) * pcap;
+        const int cnt = pcount[X * kFinishSlices + k];
+        for (int q = 0; q < cnt; ++q) s += sym_partial(part_i, part_j, X, (int64_t)lst[q], t);
+      } else {
+        for (int64_t P = k; P < ntiles; P += kFinishSlices) s += sym_partial(part_i, part_j, X, P, t);
+      }
+    }
+    __syncthreads();
+    red[k][r] = s;
+    __syncthreads();
+    if (k == 0 && i < n) {
+      double v = red[0][r];
+#pragma unroll
+      for (int q = 1; q < kFinishSlices; ++q) v += red[q][r];
+      if (nugget != 0.0) v = fma(nugget, x[i], v);
+      y[i] = v;
+    }
   }
 }
 
@@ -570,7 +617,8 @@ size_t sym_workspace_bytes(int64_t n) {
 // Symmetric product: tile-pair kernel (exact, screened or planned) and the
 // finish.  `work` holds part_i then part_j ([pairs][512] doubles each).
 int run_sym(const mplak_kernel_desc* d, int precision, int mode, double drop_s, const uint32_t* list,
-            const int64_t* count, const double* x, double* y, void* work, const int32_t* skip_if, mplak_stream_t s) {
+            const int64_t* count, const uint32_t* partners, const int32_t* pcount, int64_t pcap, const double* x,
+            double* y, void* work, const int32_t* skip_if, mplak_stream_t s) {
   SymArgs a;
   a.scale = coord_scale(d->kind, precision, d->length_scale);
   const bool coinc = d->coincident_shift != 0.0;
@@ -594,10 +642,14 @@ int run_sym(const mplak_kernel_desc* d, int precision, int mode, double drop_s, 
     launch_sym_any<kSymExact>(hs, precision, coinc, d, a);
   int st = launch_status();
   if (st != MPLAK_OK) return st;
-  int64_t fb = (d->row_count + kBlock - 1) / kBlock;
-  if (fb > 4096) fb = 4096;
-  hipLaunchKernelGGL(kmv_sym_finish_kernel, dim3((unsigned)fb), dim3(kBlock), 0, hs, d->row_count, nt, pi, pj,
-                     d->nugget, x, y, skip_if);
+  int64_t fb = (d->row_count + kFinishRows - 1) / kFinishRows;
+  if (fb > 65536) fb = 65536;
+  if (partners != nullptr)
+    hipLaunchKernelGGL(kmv_sym_finish_kernel<true>, dim3((unsigned)fb), dim3(kBlock), 0, hs, d->row_count, nt, pi,
+                       pj, d->nugget, x, y, partners, pcount, pcap, skip_if);
+  else
+    hipLaunchKernelGGL(kmv_sym_finish_kernel<false>, dim3((unsigned)fb), dim3(kBlock), 0, hs, d->row_count, nt, pi,
+                       pj, d->nugget, x, y, nullptr, nullptr, 0, skip_if);
   return launch_status();
 }
 
@@ -632,7 +684,8 @@ constexpr int kSymGroups = kSymTile / kWave;
 constexpr uint64_t kPlanDropped = 63;  // key class of pairs with no kept block
 
 struct PlanLayout {
-  size_t count, boxes, list, keys_in, keys_out, tmp, tmp_bytes, part, total;
+  size_t count, boxes, list, keys_in, keys_out, tmp, tmp_bytes, pcount, partners, part, total;
+  int64_t pcap;  // partner list capacity per (tile, slice)
 };
 
 PlanLayout plan_layout(int64_t n) {
@@ -650,7 +703,10 @@ PlanLayout plan_layout(int64_t n) {
                                (unsigned)np, 32, 38) != hipSuccess)
     return PlanLayout{};
   L.tmp_bytes = tb;
-  L.part = L.tmp + align_up(tb, 256);
+  L.pcap = (nt + kFinishSlices - 1) / kFinishSlices;
+  L.pcount = L.tmp + align_up(tb, 256);
+  L.partners = L.pcount + align_up(sizeof(int32_t) * kFinishSlices * (size_t)nt, 256);
+  L.part = L.partners + align_up(sizeof(uint32_t) * (size_t)nt * kFinishSlices * (size_t)L.pcap, 256);
   L.total = L.part + 2 * np * kSymTile * sizeof(double);
   return L;
 }
@@ -736,6 +792,56 @@ __global__ __launch_bounds__(kBlock) void plan_list_kernel(const uint64_t* __res
   if (i == 0 && !kept) *count = 0;
 }
 
+// counts[0] = tile pairs with a kept block, counts[1] = kept 128 x 64 blocks,
+// counts[2] = kept blocks on diagonal pairs (I == J) (one workgroup, fixed order).
+__global__ __launch_bounds__(kBlock) void plan_count_kernel(const uint64_t* __restrict__ keys, int64_t np,
+                                                            int64_t* __restrict__ counts) {
+  __shared__ int64_t red[3][kBlock];
+  int64_t pairs = 0, blocks = 0, diag = 0;
+  for (int64_t i = threadIdx.x; i < np; i += kBlock) {
+    const uint64_t c = keys[i] >> 32;
+    if (c != kPlanDropped) {
+      int64_t I, J;
+      decode_pair((int64_t)(keys[i] & 0xffffffffull), I, J);
+      pairs += 1;
+      blocks += 32 - (int64_t)c;
+      if (I == J) diag += 32 - (int64_t)c;
+    }
+  }
+  red[0][threadIdx.x] = pairs;
+  red[1][threadIdx.x] = blocks;
+  red[2][threadIdx.x] = diag;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < kBlock; ++k) {
+      pairs += red[0][k];
+      blocks += red[1][k];
+      diag += red[2][k];
+    }
+    counts[0] = pairs;
+    counts[1] = blocks;
+    counts[2] = diag;
+  }
+}
+
+// Partner lists of the finish: for tile X and slice k, the tiles P = k (mod 4)
+// in ascending order whose pair with X holds a kept block (one thread per
+// (tile, slice); keys are the unsorted per-pair keys).
+__global__ __launch_bounds__(kBlock) void plan_partner_kernel(const uint64_t* __restrict__ keys, int64_t nt,
+                                                              int64_t pcap, uint32_t* __restrict__ partners,
+                                                              int32_t* __restrict__ pcount) {
+  const int64_t id = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (id >= nt * kFinishSlices) return;
+  const int64_t X = id / kFinishSlices, k = id % kFinishSlices;
+  uint32_t* lst = partners + id * pcap;
+  int32_t c = 0;
+  for (int64_t P = k; P < nt; P += kFinishSlices) {
+    const int64_t p = P < X ? sym_pair_index(P, X) : sym_pair_index(X, P);
+    if ((keys[p] >> 32) != kPlanDropped) lst[c++] = (uint32_t)P;
+  }
+  pcount[id] = c;
+}
+
 template <typename T>
 void launch_plan_geometry(hipStream_t st, const mplak_kernel_desc* d, double scale, double drop_s, double* boxes,
                           uint64_t* keys) {
@@ -802,8 +908,8 @@ int mplak_kernel_matvec_screened(const mplak_kernel_desc* d, int precision, doub
   if (d->row_count == 0) return MPLAK_OK;
   if (x == nullptr || y == nullptr) return MPLAK_ERR_ARG;
   if (work == nullptr || work_bytes < sym_workspace_bytes(d->row_count)) return MPLAK_ERR_WORKSPACE;
-  return run_sym(d, precision, kSymScreened, drop_distance2(d->kind, precision, drop_log2), nullptr, nullptr, x, y,
-                 work, skip_if, s);
+  return run_sym(d, precision, kSymScreened, drop_distance2(d->kind, precision, drop_log2), nullptr, nullptr,
+                 nullptr, nullptr, 0, x, y, work, skip_if, s);
 }
 
 size_t mplak_screen_plan_bytes(const mplak_kernel_desc* d) {
@@ -842,11 +948,30 @@ int mplak_screen_plan_build(const mplak_kernel_desc* d, int precision, double dr
                      pj);
   st = launch_status();
   if (st != MPLAK_OK) return st;
+  hipLaunchKernelGGL(plan_partner_kernel, dim3((unsigned)((nt * kFinishSlices + kBlock - 1) / kBlock)), dim3(kBlock),
+                     0, hs, kin, nt, L.pcap, reinterpret_cast<uint32_t*>(base + L.partners),
+                     reinterpret_cast<int32_t*>(base + L.pcount));
+  st = launch_status();
+  if (st != MPLAK_OK) return st;
   size_t tb = L.tmp_bytes;
   if (rocprim::radix_sort_keys(base + L.tmp, tb, kin, kout, (unsigned)np, 32, 38, hs) != hipSuccess)
     return MPLAK_ERR_HIP;
   hipLaunchKernelGGL(plan_list_kernel, dim3((unsigned)((np + kBlock - 1) / kBlock)), dim3(kBlock), 0, hs, kout, np,
                      reinterpret_cast<uint32_t*>(base + L.list), reinterpret_cast<int64_t*>(base + L.count));
+  return launch_status();
+}
+
+int mplak_screen_plan_counts(const mplak_kernel_desc* d, const void* plan, size_t plan_bytes, int64_t* counts,
+                             mplak_stream_t s) {
+  if (d == nullptr || plan == nullptr || counts == nullptr || d->row_count <= 0 || !sym_eligible(d))
+    return MPLAK_ERR_ARG;
+  const PlanLayout L = plan_layout(d->row_count);
+  if (L.total == 0) return MPLAK_ERR_HIP;
+  if (plan_bytes < L.total) return MPLAK_ERR_WORKSPACE;
+  const int64_t nt = sym_tiles(d->row_count);
+  const char* base = static_cast<const char*>(plan);
+  hipLaunchKernelGGL(plan_count_kernel, dim3(1), dim3(kBlock), 0, as_stream(s),
+                     reinterpret_cast<const uint64_t*>(base + L.keys_out), nt * (nt + 1) / 2, counts);
   return launch_status();
 }
 
@@ -865,7 +990,8 @@ int mplak_kernel_matvec_planned(const mplak_kernel_desc* d, int precision, doubl
   char* base = static_cast<char*>(plan);
   return run_sym(d, precision, kSymPlanned, drop_distance2(d->kind, precision, drop_log2),
                  reinterpret_cast<const uint32_t*>(base + L.list), reinterpret_cast<const int64_t*>(base + L.count),
-                 x, y, base + L.part, skip_if, s);
+                 reinterpret_cast<const uint32_t*>(base + L.partners), reinterpret_cast<const int32_t*>(base + L.pcount),
+                 L.pcap, x, y, base + L.part, skip_if, s);
 }
 
 int mplak_kernel_matvec(const mplak_kernel_desc* d, int precision, const double* x, double* y,
@@ -879,7 +1005,8 @@ int mplak_kernel_matvec(const mplak_kernel_desc* d, int precision, const double*
   if (x == nullptr) return MPLAK_ERR_ARG;
   const size_t need = mplak_kernel_matvec_workspace_bytes(d);
   if (work == nullptr || work_bytes < need) return MPLAK_ERR_WORKSPACE;
-  if (use_symmetric(d)) return run_sym(d, precision, kSymExact, 0.0, nullptr, nullptr, x, y, work, skip_if, s);
+  if (use_symmetric(d))
+    return run_sym(d, precision, kSymExact, 0.0, nullptr, nullptr, nullptr, nullptr, 0, x, y, work, skip_if, s);
 
   const double scale = coord_scale(d->kind, precision, d->length_scale);
   const bool coinc = d->coincident_shift != 0.0;

@@ -214,3 +214,24 @@ def test_planned_product_unordered_points_and_errors(lib, oracle):
                                            X.data_ptr(), None, stream()) == _native.MPLAK_ERR_WORKSPACE
     rect = kernel_desc(P, T, 0, 0.05, rows=100, cols=200)
     assert lib.mplak_screen_plan_bytes(C.byref(rect)) == 0
+
+
+def test_plan_counts(lib, oracle):
+    import torch
+    n = 3000  # 6 tiles, 21 pairs
+    pts = oracle.uniform(3 * n, 12, 0).reshape(3, n)
+    P = dev(pts)
+    T = point_table(P)
+    d = kernel_desc(P, T, 0, 0.05, nugget=1.0)
+    counts = torch.zeros(3, dtype=torch.int64, device="cuda")
+    plan, wb = build_plan(lib, d, 1, 20.0)  # unordered points: every block is kept
+    ck(lib.mplak_screen_plan_counts(C.byref(d), plan.data_ptr(), wb, counts.data_ptr(), stream()))
+    assert host(counts).tolist() == [21, 21 * 32, 6 * 32]
+    po = ordered_points(lib, oracle, 20000, 3, 5)
+    Po = dev(po)
+    To = point_table(Po)
+    do = kernel_desc(Po, To, 0, 0.05, nugget=1.0)
+    plan, wb = build_plan(lib, do, 1, 20.0)
+    ck(lib.mplak_screen_plan_counts(C.byref(do), plan.data_ptr(), wb, counts.data_ptr(), stream()))
+    pairs, blocks, diag = host(counts).tolist()
+    assert 40 <= pairs < 40 * 41 // 2 and pairs * 1 <= blocks < 32 * pairs and diag <= 40 * 32
