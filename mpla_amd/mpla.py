@@ -223,4 +223,5 @@ MPLA_SIGNATURES.update({
     "mpla_server_run_trace": (_int, [_vp, _P(mpla_solve_request), _int, _P(mpla_solve_result)]),
     "mpla_server_clock": (C.c_double, [_vp]),
     "mpla_server_peak_bytes": (C.c_size_t, [_vp]),
+    "mpla_halton_points": (C.c_int, [_vp, C.c_int32, C.c_int64, C.c_int64, C.c_int64]),
 })

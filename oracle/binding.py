@@ -81,6 +81,7 @@ def lib():
             "oracle_save_vector": (C.c_int, [C.c_char_p, vp, C.c_int32]),
             "oracle_load_vector": (C.c_int, [C.c_char_p, vp, C.c_int32]),
             "oracle_threads": (C.c_int, []),
+            "oracle_halton": (C.c_int, [vp, C.c_int, i64, i64, i64]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -216,6 +217,14 @@ def load_vector(path, n):
     x = np.zeros(n)
     rc = lib().oracle_load_vector(str(path).encode(), _ptr(x), n)
     return rc, x
+
+
+def halton(n, dim, first=0) -> np.ndarray:
+    """Halton points k = first+1 .. first+n as a (dim, n) array (GSL's sequence)."""
+    out = np.empty((dim, n), dtype=np.float64)
+    if lib().oracle_halton(_ptr(out), dim, n, first, n) != 0:
+        raise ValueError(f"unsupported dim {dim}")
+    return out
 
 
 def threads() -> int:

@@ -25,6 +25,11 @@
  *   BiCGSTAB       manufactured solution on D + R (nonsymmetric), true
  *                  residual below eps at exit, numpy.linalg.solve.
  *   vector files   committed byte fixture and round trip; length mismatch.
+ *   halton         exact rational radical inverses (Python fractions): base 2
+ *                  bitwise, every base within 4 ulp; the van der Corput
+ *                  digits of known indices; the first points (1/2, 1/3, 1/5).
+ *                  GSL itself is absent, so agreement with gsl_qrng_halton's
+ *                  last-bit rounding is "parity unpinned" beyond that.
  */
 #include "oracle.h"
 
@@ -486,4 +491,31 @@ int oracle_threads(void) {
 #else
   return 1;
 #endif
+}
+
+/* ---- Halton points ------------------------------------------------------ */
+
+static const int oracle_primes[64] = {
+    2,   3,   5,   7,   11,  13,  17,  19,  23,  29,  31,  37,  41,  43,  47,  53,
+    59,  61,  67,  71,  73,  79,  83,  89,  97,  101, 103, 107, 109, 113, 127, 131,
+    137, 139, 149, 151, 157, 163, 167, 173, 179, 181, 191, 193, 197, 199, 211, 223,
+    227, 229, 233, 239, 241, 251, 257, 263, 269, 271, 277, 281, 283, 293, 307, 311};
+
+int oracle_halton(double* out, int dim, int64_t n, int64_t first, int64_t ld) {
+  if (dim < 1 || dim > 64) return -1;
+  for (int d = 0; d < dim; ++d) {
+    const int64_t b = oracle_primes[d];
+    const double w = 1.0 / (double)b;
+    for (int64_t i = 0; i < n; ++i) {
+      int64_t k = first + i + 1;
+      double r = 0.0, v = 1.0;
+      while (k > 0) {
+        v *= w;
+        r += v * (double)(k % b);
+        k /= b;
+      }
+      out[(int64_t)d * ld + i] = r;
+    }
+  }
+  return 0;
 }

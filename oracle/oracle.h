@@ -107,6 +107,17 @@ int oracle_save_vector(const char* path, const double* x, int32_t n);
 /* Returns 0 on success, -1 on open/read error, -2 on length mismatch. */
 int oracle_load_vector(const char* path, double* x, int32_t n);
 
+/* ---- Halton points (src/generate_quadrature_weights.cpp:107-117) -------- */
+/* The reference draws its quadrature nodes from GSL's gsl_qrng_halton (GSL,
+ * version unpinned by the reference; not vendored).  Restated from the
+ * published definition that generator implements: point k (k = 1, 2, ...,
+ * the first call returns k = 1) has coordinate d equal to the radical
+ * inverse of k in the d-th prime base b, evaluated as
+ *   r = 0, v = 1, w = 1/b;  while k > 0: v *= w, r += v * (k mod b), k /= b.
+ * out[d * ld + i] is coordinate d of point first + i + 1.  dim <= 64.
+ * Returns 0, or -1 for an unsupported dim. */
+int oracle_halton(double* out, int dim, int64_t n, int64_t first, int64_t ld);
+
 /* Number of OpenMP threads the oracle's parallel loops use. */
 int oracle_threads(void);
 

@@ -18,6 +18,7 @@ HEADERS = {
     "generic_system_adapter.h": "libmpla.so",
     "kernel_system_assembler.h": "libmpla.so",
     "mpla_server.h": "libmpla.so",
+    "quadrature.h": "libmpla.so",
 }
 
 DECL = re.compile(r"^\s*(?:extern\s+)?[A-Za-z_][\w\s\*]*?[\s\*]((?:mplak|mpla)_\w+|info|checkCUDAError|"
@@ -44,7 +45,7 @@ def test_library_exports_header(header, libname):
     assert os.path.exists(path), f"{path} not built"
     lib = C.CDLL(path)
     names = declared(header)
-    assert len(names) >= 2
+    assert len(names) >= 1
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, f"{libname} lacks {missing}"
 
@@ -79,7 +80,7 @@ def test_python_bindings_cover_headers():
     from mpla_amd import mpla
     names = set()
     for h in ("mpla.h", "kernel.h", "mpla_kernel.h", "generic_system_adapter.h", "kernel_system_assembler.h",
-              "mpla_server.h"):
+              "mpla_server.h", "quadrature.h"):
         names |= set(declared(h))
     assert sorted(names) == sorted(mpla.MPLA_SIGNATURES)
 
