@@ -19,6 +19,7 @@
 #include <stdlib.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include <rocprim/device/device_radix_sort.hpp>
 
@@ -346,6 +347,16 @@ __global__ __launch_bounds__(kBlock) void kmv_sym_kernel(
     double* __restrict__ part_j, const uint32_t* __restrict__ list, const int64_t* __restrict__ count,
     const int32_t* __restrict__ skip) {
   constexpr bool SCREEN = MODE != kSymExact;
+  // CENTER: on compact target tiles (spatially ordered points) the screened
+  // FP32 product evaluates |p - q|^2 as n_p + n_q - 2 p.q in coordinates
+  // centred on the target tile (n = squared norm; one add and DIM FMAs
+  // instead of 2 DIM subtractions and DIM products).  The tile-centred norms
+  // are small, so the cancellation error stays at the level of the FP32
+  // coordinates themselves; tiles wider than kCenterRadius2 (unordered
+  // points) and the coincidence test (which needs s == 0 exactly) use the
+  // difference form.
+  constexpr bool CENTER = SCREEN && !COINC && sizeof(T) == 4;
+  constexpr float kCenterRadius2 = 49.0f;
   if (skip != nullptr && *skip != 0) return;
   if (MODE == kSymPlanned && (int64_t)blockIdx.x >= *count) return;
   typedef typename Quad<T>::type Q;
@@ -361,6 +372,8 @@ __global__ __launch_bounds__(kBlock) void kmv_sym_kernel(
   __shared__ Q src[kGroups][kDup * kWave];
   __shared__ T gbox[kGroups][6];
   __shared__ T own_red[kHoldSums ? 1 : (kBlock / kWave) * kSymTile];
+  __shared__ T nsrc[CENTER ? kGroups : 1][CENTER ? kDup * kWave : 1];  // centred source norms
+  __shared__ T ibox[CENTER ? kBlock / kWave : 1][6];                 // target-tile box per wave
   T* red = kHoldSums ? reinterpret_cast<T*>(&src[0][0]) : own_red;
 
   const int64_t p = MODE == kSymPlanned ? (int64_t)list[blockIdx.x] : (int64_t)blockIdx.x;
@@ -382,6 +395,38 @@ __global__ __launch_bounds__(kBlock) void kmv_sym_kernel(
     for (int d = 0; d < DIM; ++d) pi[h][d] = (T)(points[d][off + ic] * scale);
     xi[h] = i < n ? (T)x[i] : T(0);
   }
+  T tbox[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+  if (SCREEN) {
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      tbox[d] = wave_min(fmin(pi[0][d], pi[1][d]));
+      tbox[3 + d] = wave_max(fmax(pi[0][d], pi[1][d]));
+    }
+  }
+  // centring: the target tile's box from the four wave boxes
+  T ctr[3] = {T(0), T(0), T(0)};
+  bool centered = false;
+  if (CENTER) {
+    if (lane == 0) {
+#pragma unroll
+      for (int q = 0; q < 6; ++q) ibox[w][q] = tbox[q];
+    }
+    __syncthreads();
+    T r2 = T(0);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      T lo = ibox[0][d], hi = ibox[0][3 + d];
+#pragma unroll
+      for (int q = 1; q < kBlock / kWave; ++q) {
+        lo = fmin(lo, ibox[q][d]);
+        hi = fmax(hi, ibox[q][3 + d]);
+      }
+      ctr[d] = T(0.5) * (lo + hi);
+      const T half = T(0.5) * (hi - lo);
+      r2 = fma(half, half, r2);
+    }
+    centered = __builtin_amdgcn_readfirstlane((int)(r2 <= (T)kCenterRadius2)) != 0;
+  }
   for (int k = tid; k < kSymTile; k += kBlock) {
     const int64_t j = J * kSymTile + k;
     const int64_t jc = j < n ? j : n - 1;
@@ -390,8 +435,6 @@ __global__ __launch_bounds__(kBlock) void kmv_sym_kernel(
     q.y = DIM > 1 ? (T)(points[1][off + jc] * scale) : T(0);
     q.z = DIM > 2 ? (T)(points[2][off + jc] * scale) : T(0);
     q.w = j < n ? (T)x[j] : T(0);
-#pragma unroll
-    for (int c = 0; c < kDup; ++c) src[k / kWave][c * kWave + (k & (kWave - 1))] = q;
     if (SCREEN) {  // k / kWave is uniform across the wave: one group per wave and pass
       const T lo0 = wave_min(q.x), lo1 = wave_min(q.y), lo2 = wave_min(q.z);
       const T hi0 = wave_max(q.x), hi1 = wave_max(q.y), hi2 = wave_max(q.z);
@@ -400,13 +443,30 @@ __global__ __launch_bounds__(kBlock) void kmv_sym_kernel(
         b[0] = lo0, b[1] = lo1, b[2] = lo2, b[3] = hi0, b[4] = hi1, b[5] = hi2;
       }
     }
-  }
-  T tbox[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
-  if (SCREEN) {
+    if (CENTER && centered) {
+      q.x -= ctr[0];
+      if (DIM > 1) q.y -= ctr[1];
+      if (DIM > 2) q.z -= ctr[2];
+      T nq = q.x * q.x;
+      if (DIM > 1) nq = fma(q.y, q.y, nq);
+      if (DIM > 2) nq = fma(q.z, q.z, nq);
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) {
-      tbox[d] = wave_min(fmin(pi[0][d], pi[1][d]));
-      tbox[3 + d] = wave_max(fmax(pi[0][d], pi[1][d]));
+      for (int c = 0; c < kDup; ++c) nsrc[k / kWave][c * kWave + (k & (kWave - 1))] = nq;
+    }
+#pragma unroll
+    for (int c = 0; c < kDup; ++c) src[k / kWave][c * kWave + (k & (kWave - 1))] = q;
+  }
+  // centred targets: pa = -2 (p - c), np = |p - c|^2
+  T pa[2][DIM], np_[2] = {T(0), T(0)};
+  if (CENTER && centered) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        const T v = pi[h][d] - ctr[d];
+        pa[h][d] = T(-2) * v;
+        np_[h] = d == 0 ? v * v : fma(v, v, np_[h]);
+      }
     }
   }
   // true when every entry between this wave's targets and group g is dropped
@@ -418,6 +478,18 @@ __global__ __launch_bounds__(kBlock) void kmv_sym_kernel(
   __syncthreads();
 
   T accI[2] = {T(0), T(0)};
+  // centred form: s = (n_p + n_q) + pa . q
+  auto entry2c = [&](const Q& q, T nq, T (&e)[2]) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      T s = np_[h] + nq;
+      s = fma(pa[h][0], q.x, s);
+      if (DIM > 1) s = fma(pa[h][1], q.y, s);
+      if (DIM > 2) s = fma(pa[h][2], q.z, s);
+      if (KIND == MPLAK_KERNEL_MATERN32) s = fmax(s, T(0));  // sqrt of a rounding-negative s
+      e[h] = kernel_entry<T, KIND>(s);
+    }
+  };
   auto entry2 = [&](const Q& q, T (&e)[2]) {
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
@@ -438,35 +510,51 @@ __global__ __launch_bounds__(kBlock) void kmv_sym_kernel(
   };
 
   if (I == J) {
-#pragma unroll 1
-    for (int g = 0; g < kGroups; ++g) {
-      if (far_group(g)) continue;
+    auto diag_group = [&](int g, auto centred) {
 #pragma unroll 4
       for (int s = 0; s < kWave; ++s) {
         const Q q = src[g][s];
         T e[2];
-        entry2(q, e);
+        if constexpr (decltype(centred)::value)
+          entry2c(q, nsrc[g][s], e);
+        else
+          entry2(q, e);
         accI[0] = fma(e[0], q.w, accI[0]);
         accI[1] = fma(e[1], q.w, accI[1]);
       }
+    };
+#pragma unroll 1
+    for (int g = 0; g < kGroups; ++g) {
+      if (far_group(g)) continue;
+      if (CENTER && centered)
+        diag_group(g, std::true_type{});
+      else
+        diag_group(g, std::false_type{});
     }
   } else {
-    auto group_sum = [&](int g) -> T {
+    auto rotate_group = [&](int g, auto centred) -> T {
       T t = T(0);
-      if (!far_group(g)) {
 #pragma unroll 4
-        for (int s = 0; s < kWave; ++s) {
-          const Q q = kDup == 2 ? src[g][lane + s] : src[g][(lane + s) & (kWave - 1)];
-          T e[2];
+      for (int s = 0; s < kWave; ++s) {
+        const int jj = kDup == 2 ? lane + s : (lane + s) & (kWave - 1);
+        const Q q = src[g][jj];
+        T e[2];
+        if constexpr (decltype(centred)::value)
+          entry2c(q, nsrc[g][jj], e);
+        else
           entry2(q, e);
-          accI[0] = fma(e[0], q.w, accI[0]);
-          accI[1] = fma(e[1], q.w, accI[1]);
-          t = fma(e[0], xi[0], t);
-          t = fma(e[1], xi[1], t);
-          t = wave_rotate(t);
-        }
+        accI[0] = fma(e[0], q.w, accI[0]);
+        accI[1] = fma(e[1], q.w, accI[1]);
+        t = fma(e[0], xi[0], t);
+        t = fma(e[1], xi[1], t);
+        t = wave_rotate(t);
       }
       return t;
+    };
+    auto group_sum = [&](int g) -> T {
+      if (far_group(g)) return T(0);
+      if (CENTER && centered) return rotate_group(g, std::true_type{});
+      return rotate_group(g, std::false_type{});
     };
     if constexpr (kHoldSums) {
       T tg[kGroups];

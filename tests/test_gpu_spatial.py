@@ -117,18 +117,31 @@ def ordered_points(lib, oracle, n, dim, seed):
 
 
 @pytest.mark.parametrize("kind,l", [(0, 0.05), (1, 0.1)])
-@pytest.mark.parametrize("prec", [0, 1])
-def test_screened_without_drops_is_the_exact_symmetric_product(lib, oracle, monkeypatch, kind, l, prec):
+@pytest.mark.parametrize("ordered", [True, False])
+def test_screened_without_drops_is_the_exact_symmetric_product(lib, oracle, monkeypatch, kind, l, ordered):
+    """FP64: bitwise the exact symmetric product.  FP32: on compact (ordered)
+    tiles the screened kernel uses the tile-centred distance form, so it
+    agrees with the exact FP32 product to FP32 accuracy and with the FP64
+    oracle to the FP32 product bar; on unordered points it is bitwise."""
     monkeypatch.setenv("MPLAK_KMV_SYMMETRIC", "1")
     n = 3000
-    pts = ordered_points(lib, oracle, n, 3, 51)
+    pts = ordered_points(lib, oracle, n, 3, 51) if ordered else oracle.uniform(3 * n, 51, 0).reshape(3, n)
     x = oracle.uniform(n, 51, 1, -1, 1)
     P = dev(pts)
     T = point_table(P)
     d = kernel_desc(P, T, kind, l, nugget=0.5)
     X = dev(x)
-    exact = host(kmv_gpu(lib, d, X, precision=prec))
-    assert np.array_equal(screened(lib, d, X, prec, 2000.0), exact)
+    assert np.array_equal(screened(lib, d, X, 0, 2000.0), host(kmv_gpu(lib, d, X, precision=0)))
+    exact32 = host(kmv_gpu(lib, d, X, precision=1))
+    y32 = screened(lib, d, X, 1, 2000.0)
+    if ordered:
+        assert rel_err(y32, exact32) <= 3e-6
+        k = oracle.Kernel(pts, kind, l, nugget=0.5)
+        ref = k.matvec(x, mode=1)
+        assert rel_err(y32, ref) <= 2e-5
+        assert np.all(np.abs(y32 - ref) <= 2e-5 * k.matvec(np.abs(x), mode=1))
+    else:
+        assert np.array_equal(y32, exact32)
 
 
 @pytest.mark.parametrize("kind,l,dim", [(0, 0.05, 3), (0, 0.02, 2), (1, 0.05, 3), (0, 0.01, 1)])
