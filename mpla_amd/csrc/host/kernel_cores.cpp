@@ -57,6 +57,11 @@ void apply_kernel(const mplak_kernel_desc& d, int precision, const double* x, do
 
 size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
 
+// Drop level of the FP64 residual products of the mixed solver on ordered
+// points: an entry below 2^-80 of the kernel's maximum changes no FP64 row
+// sum of realistic size.
+constexpr double kResidualDropLog2 = 80.0;
+
 // Plain FP64 CG (relative rule) through the fused core, printing nothing.
 void plain_fp64_cg(mpla_vector* b, mpla_generic_matrix* A, mpla_vector* x, int iter_max, double eps,
                    mpla_instance* inst) {
@@ -127,8 +132,9 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
   // (b, x permuted in, x permuted back out), so that the FP32 inner products
   // can skip blocks of entries below 2^-drop: MPLA_IR_DROP_LOG2, by default
   // 6 bits below the inner tolerance (20 for 1e-4); <= 0 keeps every entry.
-  // The FP64 residual products are exact, so the final residual is the true
-  // one whatever the inner operator dropped.
+  // The FP64 residual products keep every entry above 2^-80 (the rest cannot
+  // move an FP64 row sum), so the final residual is the true one whatever the
+  // inner operator dropped.
   const double eta_floor = env_double("MPLA_IR_INNER_TOL", 1.0e-4);
   const double drop = env_double("MPLA_IR_DROP_LOG2", std::ceil(-std::log2(eta_floor)) + 6.0);
   mplak_kernel_desc shape = desc0;
@@ -188,8 +194,9 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
   auto unpermute = [&]() {
     if (ordered) ck(mplak_scatter_f64(x->data, xs, perm, n, s), "mixed CG permute back");
   };
-  // size the product workspace now: it must not grow during graph capture
-  (void)arena(inst, kCore, mplak_kernel_matvec_workspace_bytes(&desc));
+  // size the product workspaces now: they must not grow during graph capture
+  (void)arena(inst, kCore,
+              std::max(mplak_kernel_matvec_workspace_bytes(&desc), mplak_kernel_matvec_screened_workspace_bytes(&desc)));
 
   // an all-zero initial guess needs no FP64 product for its residual
   ck(mplak_ddot_f64(n, xs, xs, dscalar, dotw, dwb, s), "mixed CG");
@@ -220,7 +227,16 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
     if (x_zero) {
       ck(mplak_fill_f64(z, n, 0.0, s), "mixed CG");
     } else {
-      apply_kernel(desc, MPLAK_PREC_F64, xs, z, nullptr, inst, "mixed CG residual product");
+      if (ordered) {
+        // entries below 2^-80 change each row by at most n 2^-80 max|x|,
+        // far below the rounding error of the FP64 sum itself
+        const size_t rwb = mplak_kernel_matvec_screened_workspace_bytes(&desc);
+        ck(mplak_kernel_matvec_screened(&desc, MPLAK_PREC_F64, kResidualDropLog2, xs, z, arena(inst, kCore, rwb), rwb,
+                                        nullptr, s),
+           "mixed CG residual product");
+      } else {
+        apply_kernel(desc, MPLAK_PREC_F64, xs, z, nullptr, inst, "mixed CG residual product");
+      }
       ++matvecs64;
     }
     ck(mplak_cg_start(n, bs, z, r, d, st, MPLAK_STOP_RELATIVE, epsilon, 2, n_global, work, wb, s),
