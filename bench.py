@@ -51,6 +51,22 @@ def sym_pairs(n, tile=512):
     return tile * tile * nt * (nt + 1) // 2
 
 
+COUNTERS = os.path.join(ROOT, "profiles", "counters_kernel_matvec_gaussian_n65536.md")
+
+
+def committed_traffic(name):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 counter
+    summary (FETCH_SIZE x its copy-kernel calibration + WRITE_SIZE), recorded
+    there as a line `traffic[<name>] = <bytes> bytes`; None when absent."""
+    try:
+        text = open(COUNTERS).read()
+    except OSError:
+        return None
+    import re
+    m = re.search(r"traffic\[" + re.escape(name) + r"\] = (\d+) bytes", text)
+    return int(m.group(1)) if m else None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -653,7 +669,8 @@ def main():
                        "latency_ms": {"p50": percentile(plat, 50) * 1e3, "p99": percentile(plat, 99) * 1e3}},
         "speedup_mixed_vs_plain": value / plain_sps,
         "roofline": {"bound": "valu", "achieved": inner["executed_tflops"], "peak": PEAK_FP32 / 1e12,
-                     "unit": "TFLOP/s", "frac": inner["executed_tflops"] * 1e12 / PEAK_FP32, "traffic": None,
+                     "unit": "TFLOP/s", "frac": inner["executed_tflops"] * 1e12 / PEAK_FP32,
+                     "traffic": committed_traffic("planned_inner_product_n65536"),
                      "kernel": inner["kernel"], "launch_ms": inner["launch_ms"],
                      "basis": "FLOP of the evaluated entries (kept 128x64 blocks x 8192 pairs x 15 FLOP off the "
                               "diagonal tiles, 13 on them) per launch / launch time (HIP events, library stream)",
@@ -675,7 +692,7 @@ def main():
     }
     if dgemv is not None:
         out["roofline_dgemv"] = {"bound": "hbm", "achieved": dgemv["gbps"], "peak": PEAK_HBM / 1e9, "unit": "GB/s",
-                                 "frac": dgemv["frac_hbm"], "traffic": None}
+                                 "frac": dgemv["frac_hbm"], "traffic": committed_traffic("dgemv_n65536")}
     L.mpla_free_vector(C.byref(b), C.byref(inst))
     L.mpla_free_vector(C.byref(x), C.byref(inst))
     L.mpla_free_generic_matrix(C.byref(A), C.byref(inst))
