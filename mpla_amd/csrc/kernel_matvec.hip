@@ -24,6 +24,7 @@
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include "common.hpp"
+#include "entry.hpp"
 
 namespace mplak {
 namespace {
@@ -33,7 +34,6 @@ constexpr int kKmvTile = kBlock * kKmvTpt;     // 512 targets per workgroup
 constexpr int kKmvChunk = kBlock;              // sources per LDS chunk
 constexpr int kKmvMaxDim = 16;
 constexpr int kKmvTargetBlocks = 4096;
-constexpr double kLog2e = 1.4426950408889634074;
 
 struct KmvPlan {
   int64_t row_tiles, slabs, slab_cols;
@@ -55,43 +55,6 @@ inline KmvPlan kmv_plan(int64_t rows, int64_t cols) {
   return p;
 }
 
-// Coordinate scale so that the entry is a function of a plain sum of squares.
-//   Gaussian  FP64: s = |d|^2 / (2 l^2),  k = exp(-s)
-//   Gaussian  FP32: s = |d|^2 log2(e) / (2 l^2),  k = exp2(-s)
-//   Matern32  both: s = 3 |d|^2 / l^2, r = sqrt(s), k = (1 + r) exp(-r)
-inline double coord_scale(int kind, int precision, double l) {
-  if (kind == MPLAK_KERNEL_GAUSSIAN) {
-    double c = 1.0 / (sqrt(2.0) * l);
-    if (precision == MPLAK_PREC_F32) c *= sqrt(kLog2e);
-    return c;
-  }
-  return sqrt(3.0) / l;
-}
-
-template <typename T, int KIND>
-__device__ __forceinline__ T kernel_entry(T s);
-
-template <>
-__device__ __forceinline__ double kernel_entry<double, MPLAK_KERNEL_GAUSSIAN>(double s) {
-  return exp(-s);
-}
-template <>
-__device__ __forceinline__ double kernel_entry<double, MPLAK_KERNEL_MATERN32>(double s) {
-  const double r = sqrt(s);
-  const double e = exp(-r);
-  return fma(r, e, e);
-}
-template <>
-__device__ __forceinline__ float kernel_entry<float, MPLAK_KERNEL_GAUSSIAN>(float s) {
-  return __builtin_amdgcn_exp2f(-s);
-}
-template <>
-__device__ __forceinline__ float kernel_entry<float, MPLAK_KERNEL_MATERN32>(float s) {
-  const float r = __builtin_amdgcn_sqrtf(s);
-  const float e = __builtin_amdgcn_exp2f(-r * (float)kLog2e);
-  return fmaf(r, e, e);
-}
-
 // DIM > 0: compile-time dimension; DIM == 0: runtime dim <= kKmvMaxDim.
 template <typename T, int KIND, int DIM, bool COINC>
 __global__ __launch_bounds__(kBlock) void kmv_partial_kernel(
@@ -104,6 +67,8 @@ __global__ __launch_bounds__(kBlock) void kmv_partial_kernel(
   const int dim = DIM > 0 ? DIM : dim_rt;
   __shared__ T lp[MD][kKmvChunk];
   __shared__ T lx[kKmvChunk];
+  __shared__ double exp_tab[Exp2Lds<T>::size];
+  if (sizeof(T) == 8) fill_exp2_table(exp_tab);  // first read after the chunk barrier below
 
   const int tid = threadIdx.x;
   const int64_t tile0 = (int64_t)blockIdx.x * kKmvTile;
@@ -163,7 +128,7 @@ __global__ __launch_bounds__(kBlock) void kmv_partial_kernel(
             s = fma(df, df, s);
           }
         }
-        T k = kernel_entry<T, KIND>(s);
+        T k = kernel_entry<T, KIND>(s, exp_tab);
         if (COINC) k += (s < thr) ? shift : T(0);
         cacc[t] = fma(k, xj, cacc[t]);
       }
@@ -372,6 +337,8 @@ __global__ __launch_bounds__(kBlock) void kmv_sym_kernel(
   __shared__ Q src[kGroups][kDup * kWave];
   __shared__ T gbox[kGroups][6];
   __shared__ T own_red[kHoldSums ? 1 : (kBlock / kWave) * kSymTile];
+  __shared__ double exp_tab[Exp2Lds<T>::size];
+  if (sizeof(T) == 8) fill_exp2_table(exp_tab);  // first read after the staging barrier
   __shared__ T nsrc[CENTER ? kGroups : 1][CENTER ? kDup * kWave : 1];  // centred source norms
   __shared__ T ibox[CENTER ? kBlock / kWave : 1][6];                 // target-tile box per wave
   T* red = kHoldSums ? reinterpret_cast<T*>(&src[0][0]) : own_red;
@@ -487,7 +454,7 @@ __global__ __launch_bounds__(kBlock) void kmv_sym_kernel(
       if (DIM > 1) s = fma(pa[h][1], q.y, s);
       if (DIM > 2) s = fma(pa[h][2], q.z, s);
       if (KIND == MPLAK_KERNEL_MATERN32) s = fmax(s, T(0));  // sqrt of a rounding-negative s
-      e[h] = kernel_entry<T, KIND>(s);
+      e[h] = kernel_entry<T, KIND>(s, exp_tab);
     }
   };
   auto entry2 = [&](const Q& q, T (&e)[2]) {
@@ -503,7 +470,7 @@ __global__ __launch_bounds__(kBlock) void kmv_sym_kernel(
         const T dz = pi[h][2] - q.z;
         s = fma(dz, dz, s);
       }
-      T k = kernel_entry<T, KIND>(s);
+      T k = kernel_entry<T, KIND>(s, exp_tab);
       if (COINC) k += (s < thr) ? shift : T(0);
       e[h] = k;
     }
@@ -708,7 +675,7 @@ int run_sym(const mplak_kernel_desc* d, int precision, int mode, double drop_s, 
             const int64_t* count, const uint32_t* partners, const int32_t* pcount, int64_t pcap, const double* x,
             double* y, void* work, const int32_t* skip_if, mplak_stream_t s) {
   SymArgs a;
-  a.scale = coord_scale(d->kind, precision, d->length_scale);
+  a.scale = coord_scale(d->kind, d->length_scale);
   const bool coinc = d->coincident_shift != 0.0;
   a.thr = (1e-14 * a.scale) * (1e-14 * a.scale);
   a.drop_s = drop_s;
@@ -744,8 +711,8 @@ int run_sym(const mplak_kernel_desc* d, int precision, int mode, double drop_s, 
 // Squared scaled distance beyond which every entry is below 2^-drop_log2
 // (in the coordinates coord_scale gives for this kind and precision).
 double drop_distance2(int kind, int precision, double drop_log2) {
-  if (kind == MPLAK_KERNEL_GAUSSIAN)
-    return precision == MPLAK_PREC_F32 ? drop_log2 : drop_log2 * 0.69314718055994530942;
+  (void)precision;
+  if (kind == MPLAK_KERNEL_GAUSSIAN) return drop_log2;  // entry 2^-s
   // (1 + r) exp(-r) is decreasing for r > 0: bisection for (1 + r) exp(-r) = 2^-drop_log2
   const double target = -drop_log2 * 0.69314718055994530942;
   double lo = 0.0, hi = 1.0;
@@ -1023,7 +990,7 @@ int mplak_screen_plan_build(const mplak_kernel_desc* d, int precision, double dr
   uint64_t* kout = reinterpret_cast<uint64_t*>(base + L.keys_out);
   double* pi = reinterpret_cast<double*>(base + L.part);
   double* pj = pi + (size_t)np * kSymTile;
-  const double scale = coord_scale(d->kind, precision, d->length_scale);
+  const double scale = coord_scale(d->kind, d->length_scale);
   const double drop_s = drop_distance2(d->kind, precision, drop_log2);
   hipStream_t hs = as_stream(s);
   if (precision == MPLAK_PREC_F64)
@@ -1096,7 +1063,7 @@ int mplak_kernel_matvec(const mplak_kernel_desc* d, int precision, const double*
   if (use_symmetric(d))
     return run_sym(d, precision, kSymExact, 0.0, nullptr, nullptr, nullptr, nullptr, 0, x, y, work, skip_if, s);
 
-  const double scale = coord_scale(d->kind, precision, d->length_scale);
+  const double scale = coord_scale(d->kind, d->length_scale);
   const bool coinc = d->coincident_shift != 0.0;
   const double thr = (1e-14 * scale) * (1e-14 * scale);
   double* part = static_cast<double*>(work);

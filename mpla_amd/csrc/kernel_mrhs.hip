@@ -27,6 +27,7 @@
 #include <algorithm>
 
 #include "common.hpp"
+#include "entry.hpp"
 
 namespace mplak {
 namespace {
@@ -34,43 +35,9 @@ namespace {
 constexpr int kChunk = 256;          // sources per LDS chunk
 constexpr int kMaxDim = 16;
 constexpr int kTargetBlocks = 2048;
-constexpr double kLog2e = 1.4426950408889634074;
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
-
-// ---------------------------------------------------------------------------
-// entry evaluation (same conventions as the single-vector product)
-
-inline double coord_scale(int kind, bool f32, double l) {
-  if (kind == MPLAK_KERNEL_GAUSSIAN) {
-    double c = 1.0 / (sqrt(2.0) * l);
-    if (f32) c *= sqrt(kLog2e);
-    return c;
-  }
-  return sqrt(3.0) / l;
-}
-
-template <typename T, int KIND>
-__device__ __forceinline__ T entry(T s);
-template <>
-__device__ __forceinline__ double entry<double, MPLAK_KERNEL_GAUSSIAN>(double s) { return exp(-s); }
-template <>
-__device__ __forceinline__ double entry<double, MPLAK_KERNEL_MATERN32>(double s) {
-  const double r = sqrt(s);
-  const double e = exp(-r);
-  return fma(r, e, e);
-}
-template <>
-__device__ __forceinline__ float entry<float, MPLAK_KERNEL_GAUSSIAN>(float s) {
-  return __builtin_amdgcn_exp2f(-s);
-}
-template <>
-__device__ __forceinline__ float entry<float, MPLAK_KERNEL_MATERN32>(float s) {
-  const float r = __builtin_amdgcn_sqrtf(s);
-  const float e = __builtin_amdgcn_exp2f(-r * (float)kLog2e);
-  return fmaf(r, e, e);
-}
 
 struct Slabs {
   int64_t tiles, slabs, slab_cols;
@@ -104,6 +71,8 @@ __global__ __launch_bounds__(kBlock) void mrhs_valu_kernel(
   const int dim = DIM > 0 ? DIM : dim_rt;
   __shared__ T lp[MD][kChunk];
   __shared__ T lv[RT][kChunk];
+  __shared__ double exp_tab[Exp2Lds<T>::size];
+  if (sizeof(T) == 8) fill_exp2_table(exp_tab);  // first read after the chunk barrier below
   const int tid = threadIdx.x;
   int64_t i = (int64_t)blockIdx.x * kBlock + tid;
   const bool active = i < rows;
@@ -137,7 +106,7 @@ __global__ __launch_bounds__(kBlock) void mrhs_valu_kernel(
           const T df = pi[d] - lp[d][jj];
           s = fma(df, df, s);
         }
-      const T k = entry<T, KIND>(s);
+      const T k = kernel_entry<T, KIND>(s, exp_tab);
 #pragma unroll
       for (int q = 0; q < RT; ++q) cacc[q] = fma(k, lv[q][jj], cacc[q]);
     }
@@ -263,7 +232,7 @@ __global__ __launch_bounds__(kBlock) void mrhs_mfma_kernel(
           for (int h = 0; h < 2; ++h) {
             const float dx = pa[h][0] - q.x, dy = pa[h][1] - q.y, dz = pa[h][2] - q.z, dw = pa[h][3] - q.w;
             const float s = fmaf(dw, dw, fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
-            e[h][t] = entry<float, KIND>(s);
+            e[h][t] = kernel_entry<float, KIND>(s, nullptr);
           }
         }
       } else {
@@ -274,7 +243,7 @@ __global__ __launch_bounds__(kBlock) void mrhs_mfma_kernel(
           for (int h = 0; h < 2; ++h) {
             const float dx = pa[h][0] - q.x, dy = pa[h][1] - q.y, dz = pa[h][2] - q.z, dw = pa[h][3] - q.w;
             const float s = fmaf(dw, dw, fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
-            e[h][t] = entry<float, KIND>(s);
+            e[h][t] = kernel_entry<float, KIND>(s, nullptr);
           }
         }
       }
@@ -508,8 +477,7 @@ int mplak_kernel_mrhs(const mplak_kernel_desc* d, int precision, int64_t nrhs, c
   if (V == nullptr || ldv < d->col_count) return MPLAK_ERR_ARG;
   if (work == nullptr || work_bytes < mplak_kernel_mrhs_workspace_bytes(d, precision, nrhs)) return MPLAK_ERR_WORKSPACE;
   hipStream_t hs = as_stream(s);
-  const bool f32 = precision != MPLAK_PREC_F64;
-  const double scale = coord_scale(d->kind, f32, d->length_scale);
+  const double scale = coord_scale(d->kind, d->length_scale);
   const MrhsPlan whole = make_plan(d, precision, nrhs);
   const int64_t group = whole.mfma ? kMfmaGroup : nrhs;
   for (int64_t r0 = 0; r0 < nrhs; r0 += group) {

@@ -18,6 +18,7 @@
 #include <math.h>
 
 #include "common.hpp"
+#include "entry.hpp"
 
 namespace mplak {
 namespace {
@@ -25,7 +26,6 @@ namespace {
 constexpr int kChunk = 256;
 constexpr int kRagTargetBlocks = 2048;
 constexpr int kMaxRaggedDim = 4;
-constexpr double kLog2e = 1.4426950408889634074;
 
 inline int vec_blocks_r(int64_t n) {
   int64_t b = (n + kBlock * 8 - 1) / (kBlock * 8);
@@ -36,23 +36,6 @@ inline int vec_blocks_r(int64_t n) {
 
 inline int rt_class(int64_t R) { return R <= 1 ? 1 : (R <= 2 ? 2 : (R <= 4 ? 4 : 8)); }
 
-template <typename T>
-__device__ __forceinline__ T rag_entry(int kind, T s);
-template <>
-__device__ __forceinline__ double rag_entry<double>(int kind, double s) {
-  if (kind == MPLAK_KERNEL_GAUSSIAN) return exp(-s);
-  const double r = sqrt(s);
-  const double e = exp(-r);
-  return fma(r, e, e);
-}
-template <>
-__device__ __forceinline__ float rag_entry<float>(int kind, float s) {
-  if (kind == MPLAK_KERNEL_GAUSSIAN) return __builtin_amdgcn_exp2f(-s);
-  const float r = __builtin_amdgcn_sqrtf(s);
-  const float e = __builtin_amdgcn_exp2f(-r * (float)kLog2e);
-  return fmaf(r, e, e);
-}
-
 template <typename T, int RT>
 __global__ __launch_bounds__(kBlock) void ragged_product_kernel(const mplak_ragged_system* __restrict__ systems,
                                                                 const mplak_ragged_item* __restrict__ items) {
@@ -61,6 +44,8 @@ __global__ __launch_bounds__(kBlock) void ragged_product_kernel(const mplak_ragg
   if (S.skip != nullptr && *S.skip != 0) return;
   __shared__ T lp[kMaxRaggedDim][kChunk];
   __shared__ T lv[RT][kChunk];
+  __shared__ double exp_tab[Exp2Lds<T>::size];
+  if (sizeof(T) == 8) fill_exp2_table(exp_tab);  // first read after the chunk barrier below
   const int tid = threadIdx.x;
   const int64_t n = S.n;
   const int dim = S.dim;
@@ -95,7 +80,7 @@ __global__ __launch_bounds__(kBlock) void ragged_product_kernel(const mplak_ragg
         const T df = pi[d] - lp[d][jj];
         s = fma(df, df, s);
       }
-      const T k = rag_entry<T>(S.kind, s);
+      const T k = kernel_entry_rt<T>(S.kind, s, exp_tab);
 #pragma unroll
       for (int q = 0; q < RT; ++q) cacc[q] = fma(k, lv[q][jj], cacc[q]);
     }
@@ -304,14 +289,6 @@ void launch_product(int rt, const mplak_ragged_system* sys, const mplak_ragged_i
   }
 }
 
-inline double coord_scale(int kind, bool f32, double l) {
-  if (kind == MPLAK_KERNEL_GAUSSIAN) {
-    double c = 1.0 / (sqrt(2.0) * l);
-    if (f32) c *= sqrt(kLog2e);
-    return c;
-  }
-  return sqrt(3.0) / l;
-}
 
 }  // namespace
 }  // namespace mplak
@@ -332,7 +309,7 @@ size_t mplak_ragged_plan(mplak_ragged_system* s) {
   const int64_t per = (chunks + ks - 1) / ks;
   s->slab_cols = per * kChunk;
   s->slabs = (int32_t)((s->n + s->slab_cols - 1) / s->slab_cols);
-  s->scale = coord_scale(s->kind, s->precision != MPLAK_PREC_F64, s->length_scale);
+  s->scale = coord_scale(s->kind, s->length_scale);
   return (size_t)s->slabs * (size_t)s->nrhs * (size_t)s->n * sizeof(double);
 }
 
