@@ -350,6 +350,31 @@ def test_mixed_precision_ir_reaches_fp64_residual(s, oracle, kind, ell, nugget):
     assert np.linalg.norm(xs - xo) <= 1e-6 * np.linalg.norm(xo)
 
 
+@pytest.mark.parametrize("n,dim,kind,ell,nugget", [(1, 3, 0, 0.05, 1.0), (2, 1, 1, 0.3, 0.5), (300, 2, 0, 0.05, 1.0),
+                                                   (513, 3, 1, 0.1, 0.5), (3000, 1, 0, 0.01, 1.0),
+                                                   (5000, 2, 1, 0.05, 0.75)])
+def test_mixed_precision_edge_sizes_and_dims(s, oracle, n, dim, kind, ell, nugget):
+    """The ordered, screened mixed path at sizes below one tile, ragged last
+    tiles, dims 1-3, both kernels and a non-zero initial guess: true FP64
+    residual below 1e-8 and the FP64 oracle solution to 1e-6."""
+    pts = oracle.uniform(dim * n, 900 + n, 0).reshape(dim, n)
+    bh = oracle.uniform(n, 900 + n, 1, -1, 1)
+    x0 = oracle.uniform(n, 900 + n, 2, -0.1, 0.1)
+    k = oracle.Kernel(pts, kind, ell, nugget=nugget)
+    P = Points(pts)
+    kd = mpla.mpla_kernel_matrix_data(P.ptr, dim, 0, kind, mpla.MPLA_PREC_F64, ell, nugget, 0.0)
+    A = s.generic(n, n, C.addressof(kd))
+    b = s.vector(n, bh)
+    x = s.vector(n, x0)
+    s.lib.mpla_kernel_conjugate_gradient_mixed(C.byref(b), C.byref(A), C.byref(x), 5000, 1e-8, mpla.MPLA_PREC_F32,
+                                               s.I)
+    xs = s.get(x)
+    assert s.inst.last_converged == 1
+    assert np.linalg.norm(bh - k.matvec(xs)) / np.linalg.norm(bh) < 1e-8 * 1.01
+    xo, _ = oracle.cg_relative(k.op(), bh, np.zeros(n), 5000, 1e-12)
+    assert np.linalg.norm(xs - xo) <= 1e-6 * np.linalg.norm(xo)
+
+
 def test_mixed_precision_ordered_screened_vs_exact_inner(s, oracle, monkeypatch):
     """The mixed solve runs on spatially ordered points with FP32 inner
     products that drop entries below 2^-30; with MPLA_IR_DROP_LOG2=0 it keeps
