@@ -1,0 +1,113 @@
+"""Empty inputs and argument errors of the kernel C ABI (include/mplak.h).
+
+These calls return before any device work, so they run without a GPU: empty
+problems succeed, malformed ones return MPLAK_ERR_ARG (the reference exits on
+size errors, src/mpla.cpp:160-175; the C ABI reports them instead), and the
+workspace queries of empty problems are 0.
+"""
+import ctypes as C
+
+import pytest
+
+from mpla_amd import _native
+
+OK, ARG, WS = _native.MPLAK_OK, _native.MPLAK_ERR_ARG, _native.MPLAK_ERR_WORKSPACE
+
+
+@pytest.fixture(scope="module")
+def lib():
+    return _native.mplak()
+
+
+def desc(rows=10, cols=10, dim=3, kind=0, ell=0.5, row_off=0, col_off=0):
+    d = _native.mplak_kernel_desc()
+    d.points = 0x1000  # never dereferenced by the paths under test
+    d.dim, d.kind, d.length_scale = dim, kind, ell
+    d.nugget, d.coincident_shift = 0.0, 0.0
+    d.row_offset, d.row_count, d.col_offset, d.col_count = row_off, rows, col_off, cols
+    return d
+
+
+def test_status_strings(lib):
+    for code in (OK, ARG, _native.MPLAK_ERR_HIP, WS, _native.MPLAK_ERR_UNSUPPORTED):
+        assert lib.mplak_status_string(code)
+    assert lib.mplak_status_string(12345)
+
+
+def test_empty_vectors_succeed(lib):
+    assert lib.mplak_fill_f64(None, 0, 1.0, None) == OK
+    assert lib.mplak_copy_f64(None, None, 0, None) == OK
+    assert lib.mplak_daxpy_f64(0, 2.0, None, None, None) == OK
+    assert lib.mplak_dwaxpby_f64(0, 1.0, None, 1.0, None, None, None) == OK
+    assert lib.mplak_fill_uniform_f64(None, 0, 1, 0, 0, 0.0, 1.0, None) == OK
+    assert lib.mplak_gather_f64(None, None, None, 0, None) == OK
+    assert lib.mplak_scatter_f64(None, None, None, 0, None) == OK
+
+
+def test_negative_sizes_and_null_pointers_rejected(lib):
+    assert lib.mplak_fill_f64(None, -1, 1.0, None) == ARG
+    assert lib.mplak_fill_f64(None, 5, 1.0, None) == ARG
+    assert lib.mplak_copy_f64(None, None, 3, None) == ARG
+    assert lib.mplak_daxpy_f64(-1, 1.0, None, None, None) == ARG
+    assert lib.mplak_ddot_f64(-1, None, None, None, None, 0, None) == ARG
+    assert lib.mplak_gather_f64(None, None, None, -2, None) == ARG
+    assert lib.mplak_cg_start(0, None, None, None, None, None, 1, 1e-8, 10, 1.0, None, 0, None) == ARG
+    assert lib.mplak_cg_iterate(0, None, None, None, None, None, None, None, 0, None) == ARG
+    assert lib.mplak_bicgstab_start(0, None, None, None, None, None, None, 1e-8, 10, 1, None, 0, None) == ARG
+
+
+def test_dgemv_shapes(lib):
+    assert lib.mplak_dgemv_workspace_bytes(0, 10) == 0
+    assert lib.mplak_dgemv_workspace_bytes(10, 0) == 0
+    assert lib.mplak_dgemv_f64(0, 10, None, 1, None, None, None, 0, None, None) == OK
+    assert lib.mplak_dgemv_f64(10, 10, None, 5, None, None, None, 0, None, None) == ARG  # lda < m
+    assert lib.mplak_dgemv_f64(-1, 10, None, 1, None, None, None, 0, None, None) == ARG
+
+
+def test_kernel_desc_validation(lib):
+    ok_empty = desc(rows=0)
+    assert lib.mplak_kernel_matvec(C.byref(ok_empty), 0, None, None, None, 0, None, None) == OK
+    assert lib.mplak_kernel_matvec_workspace_bytes(C.byref(ok_empty)) == 0
+    for bad in (desc(dim=0), desc(dim=17), desc(kind=7), desc(ell=0.0), desc(ell=-1.0), desc(rows=-1),
+                desc(row_off=-3)):
+        assert lib.mplak_kernel_matvec(C.byref(bad), 0, None, None, None, 0, None, None) == ARG
+    d = desc()
+    # precision: the single-vector product takes FP64 or FP32 only
+    assert lib.mplak_kernel_matvec(C.byref(d), _native.MPLAK_PREC_BF16, None, None, None, 0, None, None) == ARG
+    # missing output / input / workspace
+    assert lib.mplak_kernel_matvec(C.byref(d), 0, None, None, None, 0, None, None) == ARG
+    y = (C.c_double * 10)()
+    x = (C.c_double * 10)()
+    assert lib.mplak_kernel_matvec(C.byref(d), 0, C.addressof(x), C.addressof(y), None, 0, None, None) == WS
+    assert lib.mplak_kernel_matvec(None, 0, None, None, None, 0, None, None) == ARG
+
+
+def test_screened_and_planned_validation(lib):
+    d = desc()
+    for drop in (0.0, -5.0, 3000.0, float("nan")):
+        assert lib.mplak_kernel_matvec_screened(C.byref(d), 0, drop, None, None, None, 0, None, None) == ARG
+    rect = desc(rows=10, cols=20)
+    assert lib.mplak_screen_plan_bytes(C.byref(rect)) == 0
+    assert lib.mplak_screen_plan_build(C.byref(rect), 1, 20.0, None, 0, None) == ARG
+    assert lib.mplak_kernel_matvec_planned(C.byref(rect), 1, 20.0, None, 0, None, None, None, None) == ARG
+    four_d = desc(dim=4)
+    assert lib.mplak_screen_plan_bytes(C.byref(four_d)) == 0  # dim <= 3 only
+
+
+def test_spatial_order_validation(lib):
+    assert lib.mplak_spatial_order_workspace_bytes(0, 3) == 0
+    assert lib.mplak_spatial_order_workspace_bytes(10, 0) == 0
+    assert lib.mplak_spatial_order_workspace_bytes(10, 17) == 0
+    assert lib.mplak_spatial_order(0x1000, 3, 0, 0, None, None, 0, None) == OK
+    assert lib.mplak_spatial_order(0x1000, 3, 0, -1, None, None, 0, None) == ARG
+    assert lib.mplak_spatial_order(None, 3, 0, 10, None, None, 0, None) == ARG
+    assert lib.mplak_spatial_order(0x1000, 3, -1, 10, None, None, 0, None) == ARG
+    assert lib.mplak_gather_points(0x1000, 3, 0, None, 10, None, 5, None) == ARG  # ld < n
+
+
+def test_multi_rhs_validation(lib):
+    d = desc()
+    assert lib.mplak_kernel_mrhs_workspace_bytes(C.byref(d), 0, 0) == 0
+    assert lib.mplak_kernel_mrhs(C.byref(d), 0, 0, None, 10, None, 10, None, 0, None, None) == OK
+    assert lib.mplak_bcg_workspace_bytes(10, 0) >= 0
+    assert lib.mplak_bcg_start(10, 0, None, None, None, None, None, None, 1, 1e-8, 10, 1.0, None, 0, None) == ARG
