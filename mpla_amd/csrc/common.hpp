@@ -43,6 +43,20 @@ __device__ __forceinline__ T wave_sum(T v) {
   return v;
 }
 
+// Minimum / maximum over the wave (every lane gets the result).
+template <typename T>
+__device__ __forceinline__ T wave_min(T v) {
+#pragma unroll
+  for (int m = 1; m < kWave; m <<= 1) v = fmin(v, __shfl_xor(v, m));
+  return v;
+}
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+  for (int m = 1; m < kWave; m <<= 1) v = fmax(v, __shfl_xor(v, m));
+  return v;
+}
+
 // Sum over a block of NT threads; the result is valid in every thread.
 // `scratch` must hold NT / kWave elements of T and is reused after a barrier.
 template <int NT, typename T>

@@ -26,6 +26,24 @@ inline double coord_scale(int kind, double l) {
   return sqrt(3.0) / l;
 }
 
+// Squared scaled distance (in coord_scale coordinates) beyond which every
+// entry is below 2^-drop_log2; the entry at distance 0 is 1.
+inline double drop_distance2(int kind, double drop_log2) {
+  if (kind == MPLAK_KERNEL_GAUSSIAN) return drop_log2;  // entry 2^-s
+  // (1 + r) exp(-r) is decreasing for r > 0: bisection for (1 + r) exp(-r) = 2^-drop_log2
+  const double target = -drop_log2 * 0.69314718055994530942;
+  double lo = 0.0, hi = 1.0;
+  while (log1p(hi) - hi > target) hi *= 2.0;
+  for (int i = 0; i < 200; ++i) {
+    const double mid = 0.5 * (lo + hi);
+    if (log1p(mid) - mid > target)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return hi * hi;
+}
+
 // Fills tab[j] = 2^(j/256) (j < 256) with the threads of the calling block;
 // the caller synchronises the block before the first use.
 __device__ __forceinline__ void fill_exp2_table(double* tab) {

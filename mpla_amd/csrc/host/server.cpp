@@ -71,7 +71,7 @@ class DevicePool {
 enum Phase { kQueued, kResidual, kInner, kFinished };
 
 struct Layout {
-  size_t points, vec, part, cgpart, total;
+  size_t points, vec, part, cgpart, perm, total;
 };
 
 Layout layout_for(int64_t n, int dim, int nrhs) {
@@ -87,7 +87,8 @@ Layout layout_for(int64_t n, int dim, int nrhs) {
   s.precision = MPLAK_PREC_F64;
   L.part = align256(mplak_ragged_plan(&s));
   L.cgpart = align256(mplak_ragged_cg_part_bytes(n, nrhs));
-  L.total = L.points + 6 * L.vec + L.part + L.cgpart;  // points, B, X, R, D, Z, E
+  L.perm = align256(sizeof(int64_t) * (size_t)n);
+  L.total = L.points + 6 * L.vec + L.part + L.cgpart + L.perm;  // points, B, X, R, D, Z, E, spatial order
   return L;
 }
 
@@ -103,6 +104,8 @@ struct Request {
   size_t block_bytes = 0;
   double *points = nullptr, *B = nullptr, *X = nullptr, *R = nullptr, *D = nullptr, *Z = nullptr, *E = nullptr;
   double *part = nullptr, *cgpart = nullptr;
+  int64_t* perm = nullptr;  // spatial order (mixed mode): stored row k is request row perm[k]
+  bool ordered = false;
   double prev_max = INFINITY;
 };
 
@@ -151,6 +154,8 @@ struct mpla_server {
   char* dev_stage = nullptr;
   size_t dev_stage_bytes = 0;
   std::chrono::steady_clock::time_point t0;
+  // drop levels of the screened products (mixed mode; 0 = exact)
+  double inner_drop = 0.0, residual_drop = 0.0;
 
   double now() const {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -199,8 +204,10 @@ void submit_upload(mpla_server* s, Upload& u) {
   hip_ck(hipStreamSynchronize(s->stream), "server step");
 }
 
-mplak_ragged_system product_system(const Request& r, int precision, const double* V, double* Y, const int32_t* skip) {
+mplak_ragged_system product_system(const Request& r, int precision, const double* V, double* Y, const int32_t* skip,
+                                   double drop_log2) {
   mplak_ragged_system sy{};
+  sy.drop_log2 = r.ordered ? drop_log2 : 0.0;
   sy.points = r.points;
   sy.V = V;
   sy.Y = Y;
@@ -313,13 +320,46 @@ bool admit(mpla_server* s, Request* r) {
   r->part = reinterpret_cast<double*>(p);
   p += L.part;
   r->cgpart = reinterpret_cast<double*>(p);
+  p += L.cgpart;
+  r->perm = reinterpret_cast<int64_t*>(p);
   const int64_t n = r->req.n;
-  hip_ck(hipMemcpyAsync(r->points, r->req.points, sizeof(double) * (size_t)r->req.dim * (size_t)n,
-                        hipMemcpyHostToDevice, s->stream),
-         "server admit");
-  hip_ck(hipMemcpyAsync(r->B, r->req.rhs, sizeof(double) * (size_t)n * (size_t)r->req.nrhs, hipMemcpyHostToDevice,
-                        s->stream),
-         "server admit");
+  const int dim = r->req.dim;
+  r->ordered = s->cfg.mode == MPLA_SERVE_MIXED_BATCHED && (s->inner_drop > 0.0 || s->residual_drop > 0.0);
+  if (!r->ordered) {
+    hip_ck(hipMemcpyAsync(r->points, r->req.points, sizeof(double) * (size_t)dim * (size_t)n, hipMemcpyHostToDevice,
+                          s->stream),
+           "server admit");
+    hip_ck(hipMemcpyAsync(r->B, r->req.rhs, sizeof(double) * (size_t)n * (size_t)r->req.nrhs,
+                          hipMemcpyHostToDevice, s->stream),
+           "server admit");
+  } else {
+    // spatial (Morton) order of the request's points, so that the screened
+    // products can skip far blocks; the right-hand sides follow the points
+    const size_t ob = mplak_spatial_order_workspace_bytes(n, dim);
+    if (ob == 0) fail("server admit", "spatial order workspace query failed");
+    const size_t raw = align256(sizeof(double) * (size_t)dim * (size_t)n);
+    const size_t tmp_bytes = raw + 256 + align256(ob);
+    char* tmp = static_cast<char*>(s->pool.get(tmp_bytes));
+    double* raw_points = reinterpret_cast<double*>(tmp);
+    double** table = reinterpret_cast<double**>(tmp + raw);
+    double* host_table[MPLA_SERVER_MAX_DIM];
+    for (int d = 0; d < dim; ++d) host_table[d] = raw_points + (size_t)d * n;
+    hip_ck(hipMemcpyAsync(raw_points, r->req.points, sizeof(double) * (size_t)dim * (size_t)n, hipMemcpyHostToDevice,
+                          s->stream),
+           "server admit");
+    hip_ck(hipMemcpyAsync(table, host_table, sizeof(double*) * dim, hipMemcpyHostToDevice, s->stream),
+           "server admit");
+    ck(mplak_spatial_order(table, dim, 0, n, r->perm, tmp + raw + 256, ob, s->stream), "server admit order");
+    ck(mplak_gather_points(table, dim, 0, r->perm, n, r->points, n, s->stream), "server admit order");
+    hip_ck(hipMemcpyAsync(r->E, r->req.rhs, sizeof(double) * (size_t)n * (size_t)r->req.nrhs, hipMemcpyHostToDevice,
+                          s->stream),
+           "server admit");
+    for (int c = 0; c < r->req.nrhs; ++c)
+      ck(mplak_gather_f64(r->B + (size_t)c * n, r->E + (size_t)c * n, r->perm, n, s->stream), "server admit order");
+    // host_table and the temporary block must outlive the copies and kernels
+    hip_ck(hipStreamSynchronize(s->stream), "server admit");
+    s->pool.put(tmp, tmp_bytes);
+  }
   ck(mplak_fill_f64(r->X, n * r->req.nrhs, 0.0, s->stream), "server admit");
   r->phase = kResidual;
   r->x_zero = true;
@@ -341,7 +381,7 @@ void mpla_server_default_config(mpla_server_config* cfg) {
   cfg->max_active = 64;
   cfg->mode = MPLA_SERVE_MIXED_BATCHED;
   cfg->iterations_per_step = 4;
-  cfg->inner_tol_floor = 1e-5;
+  cfg->inner_tol_floor = 1e-4;
   cfg->max_iterations = 20000;
 }
 
@@ -368,6 +408,15 @@ mpla_server* mpla_server_create(const mpla_server_config* cfg) {
   hip_ck(hipHostMalloc(reinterpret_cast<void**>(&s->host_groups), gb, hipHostMallocDefault), "mpla_server_create");
   for (int k = s->cfg.max_active - 1; k >= 0; --k) s->free_slots.push_back(k);
   ensure_stage(s, size_t(4) << 20);
+  if (s->cfg.mode == MPLA_SERVE_MIXED_BATCHED) {
+    // as the single-system mixed solver: FP32 inner products drop entries
+    // 6 bits below the inner tolerance floor (MPLA_IR_DROP_LOG2 overrides;
+    // <= 0 keeps every entry), FP64 products entries below 2^-80
+    const double floor = s->cfg.inner_tol_floor > 0.0 ? s->cfg.inner_tol_floor : 1e-4;
+    s->inner_drop = env_or("MPLA_IR_DROP_LOG2", std::ceil(-std::log2(floor)) + 6.0);
+    if (s->inner_drop > 0.0) s->residual_drop = 80.0;
+    if (s->inner_drop < 0.0) s->inner_drop = 0.0;
+  }
   s->t0 = std::chrono::steady_clock::now();
   return s;
 }
@@ -446,7 +495,7 @@ int mpla_server_step(mpla_server* s) {
     Upload u;
     std::vector<mplak_ragged_system> prod;
     for (Request* r : resid)
-      if (!r->x_zero) prod.push_back(product_system(*r, MPLAK_PREC_F64, r->X, r->Z, nullptr));
+      if (!r->x_zero) prod.push_back(product_system(*r, MPLAK_PREC_F64, r->X, r->Z, nullptr, s->residual_drop));
     add_products(u, prod, MPLAK_PREC_F64);
     // Z = 0 for requests whose X is still zero (the fill op writes the X field)
     const size_t zb = u.csys.size();
@@ -528,7 +577,8 @@ int mpla_server_step(mpla_server* s) {
     for (Request* r : inner) {
       const bool f64 = serial || r->escalated;
       (f64 ? p64 : p32).push_back(product_system(*r, f64 ? MPLAK_PREC_F64 : MPLAK_PREC_F32, r->D, r->Z,
-                                                 &s->groups[r->slot].all_done));
+                                                 &s->groups[r->slot].all_done,
+                                                 f64 ? s->residual_drop : s->inner_drop));
     }
     const size_t base = u.csys.size();
     for (Request* r : inner) u.csys.push_back(cg_system(s, *r, r->E, r->X));
@@ -577,11 +627,19 @@ int mpla_server_step(mpla_server* s) {
 
   // 4. retire
   if (!finished.empty()) {
-    for (Request* r : finished)
-      if (r->req.solution)
-        hip_ck(hipMemcpyAsync(r->req.solution, r->X, sizeof(double) * (size_t)r->req.n * (size_t)r->req.nrhs,
-                              hipMemcpyDeviceToHost, s->stream),
-               "server retire");
+    for (Request* r : finished) {
+      if (!r->req.solution) continue;
+      const double* out = r->X;
+      if (r->ordered) {  // back to the request's row order
+        const int64_t n = r->req.n;
+        for (int c = 0; c < r->req.nrhs; ++c)
+          ck(mplak_scatter_f64(r->Z + (size_t)c * n, r->X + (size_t)c * n, r->perm, n, s->stream), "server retire");
+        out = r->Z;
+      }
+      hip_ck(hipMemcpyAsync(r->req.solution, out, sizeof(double) * (size_t)r->req.n * (size_t)r->req.nrhs,
+                            hipMemcpyDeviceToHost, s->stream),
+             "server retire");
+    }
     hip_ck(hipStreamSynchronize(s->stream), "server retire");
     const double t = s->now();
     for (Request* r : finished) {

@@ -262,19 +262,6 @@ __device__ __forceinline__ double wave_rotate(double v) {
 
 __device__ __forceinline__ int64_t sym_pair_index(int64_t I, int64_t J) { return J * (J + 1) / 2 + I; }
 
-template <typename T>
-__device__ __forceinline__ T wave_min(T v) {
-#pragma unroll
-  for (int m = 1; m < kWave; m <<= 1) v = fmin(v, __shfl_xor(v, m));
-  return v;
-}
-template <typename T>
-__device__ __forceinline__ T wave_max(T v) {
-#pragma unroll
-  for (int m = 1; m < kWave; m <<= 1) v = fmax(v, __shfl_xor(v, m));
-  return v;
-}
-
 // Tile pair p = J (J + 1) / 2 + I, I <= J.
 __device__ __forceinline__ void decode_pair(int64_t p, int64_t& I, int64_t& J) {
   J = (int64_t)((sqrt(8.0 * (double)p + 1.0) - 1.0) * 0.5);
@@ -708,22 +695,6 @@ int run_sym(const mplak_kernel_desc* d, int precision, int mode, double drop_s, 
   return launch_status();
 }
 
-// Squared scaled distance beyond which every entry is below 2^-drop_log2
-// (in the coordinates coord_scale gives for this kind and precision).
-double drop_distance2(int kind, int precision, double drop_log2) {
-  (void)precision;
-  if (kind == MPLAK_KERNEL_GAUSSIAN) return drop_log2;  // entry 2^-s
-  // (1 + r) exp(-r) is decreasing for r > 0: bisection for (1 + r) exp(-r) = 2^-drop_log2
-  const double target = -drop_log2 * 0.69314718055994530942;
-  double lo = 0.0, hi = 1.0;
-  while (log1p(hi) - hi > target) hi *= 2.0;
-  for (int i = 0; i < 200; ++i) {
-    const double mid = 0.5 * (lo + hi);
-    if (log1p(mid) - mid > target) lo = mid;
-    else hi = mid;
-  }
-  return hi * hi;
-}
 
 
 // ---------------------------------------------------------------------------
@@ -963,7 +934,7 @@ int mplak_kernel_matvec_screened(const mplak_kernel_desc* d, int precision, doub
   if (d->row_count == 0) return MPLAK_OK;
   if (x == nullptr || y == nullptr) return MPLAK_ERR_ARG;
   if (work == nullptr || work_bytes < sym_workspace_bytes(d->row_count)) return MPLAK_ERR_WORKSPACE;
-  return run_sym(d, precision, kSymScreened, drop_distance2(d->kind, precision, drop_log2), nullptr, nullptr,
+  return run_sym(d, precision, kSymScreened, drop_distance2(d->kind, drop_log2), nullptr, nullptr,
                  nullptr, nullptr, 0, x, y, work, skip_if, s);
 }
 
@@ -991,7 +962,7 @@ int mplak_screen_plan_build(const mplak_kernel_desc* d, int precision, double dr
   double* pi = reinterpret_cast<double*>(base + L.part);
   double* pj = pi + (size_t)np * kSymTile;
   const double scale = coord_scale(d->kind, d->length_scale);
-  const double drop_s = drop_distance2(d->kind, precision, drop_log2);
+  const double drop_s = drop_distance2(d->kind, drop_log2);
   hipStream_t hs = as_stream(s);
   if (precision == MPLAK_PREC_F64)
     launch_plan_geometry<double>(hs, d, scale, drop_s, boxes, kin);
@@ -1043,7 +1014,7 @@ int mplak_kernel_matvec_planned(const mplak_kernel_desc* d, int precision, doubl
   if (L.total == 0) return MPLAK_ERR_HIP;
   if (plan == nullptr || plan_bytes < L.total) return MPLAK_ERR_WORKSPACE;
   char* base = static_cast<char*>(plan);
-  return run_sym(d, precision, kSymPlanned, drop_distance2(d->kind, precision, drop_log2),
+  return run_sym(d, precision, kSymPlanned, drop_distance2(d->kind, drop_log2),
                  reinterpret_cast<const uint32_t*>(base + L.list), reinterpret_cast<const int64_t*>(base + L.count),
                  reinterpret_cast<const uint32_t*>(base + L.partners), reinterpret_cast<const int32_t*>(base + L.pcount),
                  L.pcap, x, y, base + L.part, skip_if, s);

@@ -59,6 +59,20 @@ __global__ __launch_bounds__(kBlock) void ragged_product_kernel(const mplak_ragg
   for (int d = 0; d < kMaxRaggedDim; ++d) pi[d] = d < dim ? (T)(S.points[d * n + i] * S.scale) : T(0);
   const int64_t c0 = (int64_t)it.slab * S.slab_cols;
   const int64_t c1 = (c0 + S.slab_cols < n) ? c0 + S.slab_cols : n;
+  // Screening (drop_s > 0): a wave skips a chunk when the bounding boxes of
+  // its 64 targets and of the chunk's sources are further apart than drop_s,
+  // i.e. every entry of the block is below the system's drop level.
+  const bool screen = S.drop_s > 0.0;
+  const int lane = tid & (kWave - 1), w = tid / kWave;
+  __shared__ T cbox[kBlock / kWave][2 * kMaxRaggedDim];
+  T tbox[2 * kMaxRaggedDim];
+  if (screen) {
+#pragma unroll
+    for (int d = 0; d < kMaxRaggedDim; ++d) {
+      tbox[d] = wave_min(pi[d]);
+      tbox[kMaxRaggedDim + d] = wave_max(pi[d]);
+    }
+  }
   double acc[RT];
 #pragma unroll
   for (int q = 0; q < RT; ++q) acc[q] = 0.0;
@@ -69,7 +83,33 @@ __global__ __launch_bounds__(kBlock) void ragged_product_kernel(const mplak_ragg
     for (int d = 0; d < kMaxRaggedDim; ++d) lp[d][tid] = (j < c1 && d < dim) ? (T)(S.points[d * n + j] * S.scale) : T(0);
 #pragma unroll
     for (int q = 0; q < RT; ++q) lv[q][tid] = (j < c1 && r0 + q < R) ? (T)S.V[(r0 + q) * n + j] : T(0);
+    if (screen) {  // this wave's part of the chunk box; sources past the slab do not count
+#pragma unroll
+      for (int d = 0; d < kMaxRaggedDim; ++d) {
+        const T lo = wave_min(j < c1 ? lp[d][tid] : (T)INFINITY);
+        const T hi = wave_max(j < c1 ? lp[d][tid] : (T)-INFINITY);
+        if (lane == 0) {
+          cbox[w][d] = lo;
+          cbox[w][kMaxRaggedDim + d] = hi;
+        }
+      }
+    }
     __syncthreads();
+    if (screen) {
+      T d2 = T(0);
+#pragma unroll
+      for (int d = 0; d < kMaxRaggedDim; ++d) {
+        T lo = cbox[0][d], hi = cbox[0][kMaxRaggedDim + d];
+#pragma unroll
+        for (int q = 1; q < kBlock / kWave; ++q) {
+          lo = fmin(lo, cbox[q][d]);
+          hi = fmax(hi, cbox[q][kMaxRaggedDim + d]);
+        }
+        const T gap = fmax(T(0), fmax(lo - tbox[kMaxRaggedDim + d], tbox[d] - hi));
+        d2 = fma(gap, gap, d2);
+      }
+      if (__builtin_amdgcn_readfirstlane((int)(d2 > (T)S.drop_s)) != 0) continue;
+    }
     T cacc[RT];
 #pragma unroll
     for (int q = 0; q < RT; ++q) cacc[q] = T(0);
@@ -310,6 +350,7 @@ size_t mplak_ragged_plan(mplak_ragged_system* s) {
   s->slab_cols = per * kChunk;
   s->slabs = (int32_t)((s->n + s->slab_cols - 1) / s->slab_cols);
   s->scale = coord_scale(s->kind, s->length_scale);
+  s->drop_s = s->drop_log2 > 0.0 ? drop_distance2(s->kind, s->drop_log2) : 0.0;
   return (size_t)s->slabs * (size_t)s->nrhs * (size_t)s->n * sizeof(double);
 }
 

@@ -102,3 +102,21 @@ def test_mixed_and_fp64_serial_agree(oracle):
         _, plain = srv.run_trace(reqs)
     for a, b in zip(mixed, plain):
         assert np.linalg.norm(a - b) <= 1e-6 * np.linalg.norm(b)
+
+
+def test_server_screened_ordered_vs_exact(oracle, monkeypatch):
+    """Mixed mode orders each request's points and screens its products by
+    default; MPLA_IR_DROP_LOG2=0 keeps every entry in the request's order.
+    Both meet every request's tolerance and agree to the solution bar."""
+    require_gpu()
+    reqs = serving.request_stream(6, 3000.0, seed=23, n_min=2000, n_max=20000)
+    with serving.Server() as srv:
+        res_s, sol_s = srv.run_trace(reqs)
+    monkeypatch.setenv("MPLA_IR_DROP_LOG2", "0")
+    with serving.Server() as srv:
+        res_e, sol_e = srv.run_trace(reqs)
+    for r, a, b, ra, rb in zip(reqs, sol_s, sol_e, res_s, res_e):
+        assert ra.status == rb.status == mpla.MPLA_REQUEST_CONVERGED
+        _, _, worst = host_check(oracle, r, a)
+        assert worst < r["tolerance"] * 1.01
+        assert np.linalg.norm(a - b) <= 1e-6 * np.linalg.norm(b)
