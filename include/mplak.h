@@ -354,9 +354,11 @@ int64_t mplak_ragged_finish_items(const mplak_ragged_system* s, mplak_ragged_ite
 
 /* Product partials for `count` items whose systems share rt and precision
  * (systems and items are device arrays); then mplak_ragged_finish adds the
- * slabs in order and the nugget term: Y = (K + nugget I) V per system. */
+ * slabs in order and the nugget term: Y = (K + nugget I) V per system.
+ * dim: 1..4 when every system of the launch has that dimension (a kernel
+ * specialised for it), 0 for any mix of dimensions. */
 int mplak_ragged_product(const mplak_ragged_system* systems, const mplak_ragged_item* items, int64_t count, int rt,
-                         int precision, mplak_stream_t stream);
+                         int dim, int precision, mplak_stream_t stream);
 int mplak_ragged_finish(const mplak_ragged_system* systems, const mplak_ragged_item* items, int64_t count,
                         mplak_stream_t stream);
 

@@ -144,7 +144,7 @@ MPLAK_SIGNATURES = {
     "mplak_ragged_plan": (_sz, [_vp]),
     "mplak_ragged_product_items": (_i64, [_vp, _vp, _i32]),
     "mplak_ragged_finish_items": (_i64, [_vp, _vp, _i32]),
-    "mplak_ragged_product": (C.c_int, [_vp, _vp, _i64, C.c_int, C.c_int, _vp]),
+    "mplak_ragged_product": (C.c_int, [_vp, _vp, _i64, C.c_int, C.c_int, C.c_int, _vp]),
     "mplak_ragged_finish": (C.c_int, [_vp, _vp, _i64, _vp]),
     "mplak_ragged_cg_blocks": (_i32, [_i64]),
     "mplak_ragged_cg_part_bytes": (_sz, [_i64, _i32]),

@@ -117,7 +117,8 @@ struct Upload {
   std::vector<mplak_ragged_item> items;
   struct Launch {
     int kind;  // 0 product, 1 finish, 2 cg op
-    int op, rt, precision, parity;
+    int op;    // cg op; for a product, the systems' dimension
+    int rt, precision, parity;
     double value;
     size_t first;
     int64_t count;
@@ -194,7 +195,7 @@ void submit_upload(mpla_server* s, Upload& u) {
   for (const auto& L : u.launches) {
     const mplak_ragged_item* it = dit + L.first;
     if (L.kind == 0)
-      ck(mplak_ragged_product(dps, it, L.count, L.rt, L.precision, s->stream), "server product");
+      ck(mplak_ragged_product(dps, it, L.count, L.rt, L.op, L.precision, s->stream), "server product");
     else if (L.kind == 1)
       ck(mplak_ragged_finish(dps, it, L.count, s->stream), "server finish");
     else
@@ -247,18 +248,20 @@ void add_products(Upload& u, const std::vector<mplak_ragged_system>& systems, in
   if (systems.empty()) return;
   const size_t base = u.psys.size();
   for (const auto& sy : systems) u.psys.push_back(sy);
-  for (int rt : {1, 2, 4, 8}) {
-    const size_t first = u.items.size();
-    for (size_t k = 0; k < systems.size(); ++k) {
-      if (systems[k].rt != rt) continue;
-      const int64_t c = mplak_ragged_product_items(&systems[k], nullptr, 0);
-      const size_t at = u.items.size();
-      u.items.resize(at + (size_t)c);
-      mplak_ragged_product_items(&systems[k], u.items.data() + at, (int32_t)(base + k));
+  // one launch per (rt, dim) class: kernels specialised for both
+  for (int rt : {1, 2, 4, 8})
+    for (int dim = 1; dim <= MPLA_SERVER_MAX_DIM; ++dim) {
+      const size_t first = u.items.size();
+      for (size_t k = 0; k < systems.size(); ++k) {
+        if (systems[k].rt != rt || systems[k].dim != dim) continue;
+        const int64_t c = mplak_ragged_product_items(&systems[k], nullptr, 0);
+        const size_t at = u.items.size();
+        u.items.resize(at + (size_t)c);
+        mplak_ragged_product_items(&systems[k], u.items.data() + at, (int32_t)(base + k));
+      }
+      const int64_t count = (int64_t)(u.items.size() - first);
+      if (count > 0) u.launches.push_back({0, dim, rt, precision, 0, 0.0, first, count});
     }
-    const int64_t count = (int64_t)(u.items.size() - first);
-    if (count > 0) u.launches.push_back({0, 0, rt, precision, 0, 0.0, first, count});
-  }
   const size_t first = u.items.size();
   for (size_t k = 0; k < systems.size(); ++k) {
     const int64_t c = mplak_ragged_finish_items(&systems[k], nullptr, 0);

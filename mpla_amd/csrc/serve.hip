@@ -36,13 +36,16 @@ inline int vec_blocks_r(int64_t n) {
 
 inline int rt_class(int64_t R) { return R <= 1 ? 1 : (R <= 2 ? 2 : (R <= 4 ? 4 : 8)); }
 
-template <typename T, int RT>
+// DIM > 0: every system of the launch has that dimension; DIM == 0: any
+// dimension <= kMaxRaggedDim (coordinates past it are zero).
+template <typename T, int RT, int DIM>
 __global__ __launch_bounds__(kBlock) void ragged_product_kernel(const mplak_ragged_system* __restrict__ systems,
                                                                 const mplak_ragged_item* __restrict__ items) {
+  constexpr int MD = DIM > 0 ? DIM : kMaxRaggedDim;
   const mplak_ragged_item it = items[blockIdx.x];
   const mplak_ragged_system S = systems[it.sys];
   if (S.skip != nullptr && *S.skip != 0) return;
-  __shared__ T lp[kMaxRaggedDim][kChunk];
+  __shared__ T lp[MD][kChunk];
   __shared__ T lv[RT][kChunk];
   __shared__ double exp_tab[Exp2Lds<T>::size];
   if (sizeof(T) == 8) fill_exp2_table(exp_tab);  // first read after the chunk barrier below
@@ -54,9 +57,9 @@ __global__ __launch_bounds__(kBlock) void ragged_product_kernel(const mplak_ragg
   const bool active = i < n;
   if (!active) i = n - 1;
   const int64_t r0 = (int64_t)it.ctile * RT;
-  T pi[kMaxRaggedDim];
+  T pi[MD];
 #pragma unroll
-  for (int d = 0; d < kMaxRaggedDim; ++d) pi[d] = d < dim ? (T)(S.points[d * n + i] * S.scale) : T(0);
+  for (int d = 0; d < MD; ++d) pi[d] = d < dim ? (T)(S.points[d * n + i] * S.scale) : T(0);
   const int64_t c0 = (int64_t)it.slab * S.slab_cols;
   const int64_t c1 = (c0 + S.slab_cols < n) ? c0 + S.slab_cols : n;
   // Screening (drop_s > 0): a wave skips a chunk when the bounding boxes of
@@ -64,13 +67,13 @@ __global__ __launch_bounds__(kBlock) void ragged_product_kernel(const mplak_ragg
   // i.e. every entry of the block is below the system's drop level.
   const bool screen = S.drop_s > 0.0;
   const int lane = tid & (kWave - 1), w = tid / kWave;
-  __shared__ T cbox[kBlock / kWave][2 * kMaxRaggedDim];
-  T tbox[2 * kMaxRaggedDim];
+  __shared__ T cbox[kBlock / kWave][2 * MD];
+  T tbox[2 * MD];
   if (screen) {
 #pragma unroll
-    for (int d = 0; d < kMaxRaggedDim; ++d) {
+    for (int d = 0; d < MD; ++d) {
       tbox[d] = wave_min(pi[d]);
-      tbox[kMaxRaggedDim + d] = wave_max(pi[d]);
+      tbox[MD + d] = wave_max(pi[d]);
     }
   }
   double acc[RT];
@@ -80,17 +83,17 @@ __global__ __launch_bounds__(kBlock) void ragged_product_kernel(const mplak_ragg
     __syncthreads();
     const int64_t j = cb + tid;
 #pragma unroll
-    for (int d = 0; d < kMaxRaggedDim; ++d) lp[d][tid] = (j < c1 && d < dim) ? (T)(S.points[d * n + j] * S.scale) : T(0);
+    for (int d = 0; d < MD; ++d) lp[d][tid] = (j < c1 && d < dim) ? (T)(S.points[d * n + j] * S.scale) : T(0);
 #pragma unroll
     for (int q = 0; q < RT; ++q) lv[q][tid] = (j < c1 && r0 + q < R) ? (T)S.V[(r0 + q) * n + j] : T(0);
     if (screen) {  // this wave's part of the chunk box; sources past the slab do not count
 #pragma unroll
-      for (int d = 0; d < kMaxRaggedDim; ++d) {
+      for (int d = 0; d < MD; ++d) {
         const T lo = wave_min(j < c1 ? lp[d][tid] : (T)INFINITY);
         const T hi = wave_max(j < c1 ? lp[d][tid] : (T)-INFINITY);
         if (lane == 0) {
           cbox[w][d] = lo;
-          cbox[w][kMaxRaggedDim + d] = hi;
+          cbox[w][MD + d] = hi;
         }
       }
     }
@@ -98,14 +101,14 @@ __global__ __launch_bounds__(kBlock) void ragged_product_kernel(const mplak_ragg
     if (screen) {
       T d2 = T(0);
 #pragma unroll
-      for (int d = 0; d < kMaxRaggedDim; ++d) {
-        T lo = cbox[0][d], hi = cbox[0][kMaxRaggedDim + d];
+      for (int d = 0; d < MD; ++d) {
+        T lo = cbox[0][d], hi = cbox[0][MD + d];
 #pragma unroll
         for (int q = 1; q < kBlock / kWave; ++q) {
           lo = fmin(lo, cbox[q][d]);
-          hi = fmax(hi, cbox[q][kMaxRaggedDim + d]);
+          hi = fmax(hi, cbox[q][MD + d]);
         }
-        const T gap = fmax(T(0), fmax(lo - tbox[kMaxRaggedDim + d], tbox[d] - hi));
+        const T gap = fmax(T(0), fmax(lo - tbox[MD + d], tbox[d] - hi));
         d2 = fma(gap, gap, d2);
       }
       if (__builtin_amdgcn_readfirstlane((int)(d2 > (T)S.drop_s)) != 0) continue;
@@ -116,7 +119,7 @@ __global__ __launch_bounds__(kBlock) void ragged_product_kernel(const mplak_ragg
     for (int jj = 0; jj < kChunk; ++jj) {
       T s = T(0);
 #pragma unroll
-      for (int d = 0; d < kMaxRaggedDim; ++d) {
+      for (int d = 0; d < MD; ++d) {
         const T df = pi[d] - lp[d][jj];
         s = fma(df, df, s);
       }
@@ -318,14 +321,27 @@ __global__ __launch_bounds__(kBlock) void ragged_fill_kernel(const mplak_ragged_
   for (int64_t i = (int64_t)it.tile * kBlock + threadIdx.x; i < n; i += (int64_t)S.G * kBlock) S.X[off + i] = value;
 }
 
-template <typename T>
-void launch_product(int rt, const mplak_ragged_system* sys, const mplak_ragged_item* items, int64_t count,
-                    hipStream_t st) {
+template <typename T, int DIM>
+void launch_product_dim(int rt, const mplak_ragged_system* sys, const mplak_ragged_item* items, int64_t count,
+                        hipStream_t st) {
+  const dim3 g((unsigned)count), b(kBlock);
   switch (rt) {
-    case 1: hipLaunchKernelGGL((ragged_product_kernel<T, 1>), dim3((unsigned)count), dim3(kBlock), 0, st, sys, items); break;
-    case 2: hipLaunchKernelGGL((ragged_product_kernel<T, 2>), dim3((unsigned)count), dim3(kBlock), 0, st, sys, items); break;
-    case 4: hipLaunchKernelGGL((ragged_product_kernel<T, 4>), dim3((unsigned)count), dim3(kBlock), 0, st, sys, items); break;
-    default: hipLaunchKernelGGL((ragged_product_kernel<T, 8>), dim3((unsigned)count), dim3(kBlock), 0, st, sys, items); break;
+    case 1: hipLaunchKernelGGL((ragged_product_kernel<T, 1, DIM>), g, b, 0, st, sys, items); break;
+    case 2: hipLaunchKernelGGL((ragged_product_kernel<T, 2, DIM>), g, b, 0, st, sys, items); break;
+    case 4: hipLaunchKernelGGL((ragged_product_kernel<T, 4, DIM>), g, b, 0, st, sys, items); break;
+    default: hipLaunchKernelGGL((ragged_product_kernel<T, 8, DIM>), g, b, 0, st, sys, items); break;
+  }
+}
+
+template <typename T>
+void launch_product(int rt, int dim, const mplak_ragged_system* sys, const mplak_ragged_item* items, int64_t count,
+                    hipStream_t st) {
+  switch (dim) {
+    case 1: launch_product_dim<T, 1>(rt, sys, items, count, st); break;
+    case 2: launch_product_dim<T, 2>(rt, sys, items, count, st); break;
+    case 3: launch_product_dim<T, 3>(rt, sys, items, count, st); break;
+    case 4: launch_product_dim<T, 4>(rt, sys, items, count, st); break;
+    default: launch_product_dim<T, 0>(rt, sys, items, count, st); break;
   }
 }
 
@@ -375,14 +391,14 @@ int64_t mplak_ragged_finish_items(const mplak_ragged_system* s, mplak_ragged_ite
 }
 
 int mplak_ragged_product(const mplak_ragged_system* systems, const mplak_ragged_item* items, int64_t count, int rt,
-                         int precision, mplak_stream_t s) {
+                         int dim, int precision, mplak_stream_t s) {
   if (count <= 0) return MPLAK_OK;
-  if (!systems || !items || count > 0x7fffffff) return MPLAK_ERR_ARG;
+  if (!systems || !items || count > 0x7fffffff || dim < 0 || dim > kMaxRaggedDim) return MPLAK_ERR_ARG;
   hipStream_t hs = as_stream(s);
   if (precision == MPLAK_PREC_F64)
-    launch_product<double>(rt, systems, items, count, hs);
+    launch_product<double>(rt, dim, systems, items, count, hs);
   else
-    launch_product<float>(rt, systems, items, count, hs);
+    launch_product<float>(rt, dim, systems, items, count, hs);
   return launch_status();
 }
 
