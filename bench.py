@@ -285,6 +285,59 @@ def bench_block(K, L, inst, stream_ptr, args):
     return out
 
 
+PUBLISHED_KERNEL_CG_S = 364.6644  # reference benchmark_kernel_cg, N = 2^17, 5 trials, P100-class GPU (2017)
+
+
+def bench_reference_kernel_cg(K, L, inst, stream_ptr, ext):
+    """The reference's published benchmark on the drop-in API: matrix-free
+    kernel CG with the reference core (exp(-r^2), +1e-8 on coincident
+    points) at N = 131072 points U[0,1]^3, b = A x_true, 5 trials of
+    mpla_generic_conjugate_gradient(iter_max 50, eps 1e-15) from the same
+    random x0 (restored after each trial): 250 FP64 matrix-free products, as
+    src/benchmark_kernel_cg.cpp:40-140 with trials = 5."""
+    import torch
+    from mpla_amd import _native, mpla
+    n, dim, trials = 131072, 3, 5
+    P = torch.empty(dim, n, dtype=torch.float64, device="cuda")
+    _native.check(K.mplak_fill_uniform_f64(P.data_ptr(), dim * n, 1235, 30, 0, 0.0, 1.0, stream_ptr))
+    table = torch.tensor([P[d].data_ptr() for d in range(dim)], dtype=torch.int64, device="cuda")
+    kd = mpla.kernel_matrix_data(table.data_ptr(), dim, n)
+    A = mpla.mpla_generic_matrix()
+    L.mpla_init_generic_matrix(C.byref(A), C.byref(inst), n, n)
+    A.data = C.addressof(kd)
+    x, x0, b = mpla.mpla_vector(), mpla.mpla_vector(), mpla.mpla_vector()
+    for v in (x, x0, b):
+        L.mpla_init_vector(C.byref(v), C.byref(inst), n)
+    core = mpla.core("mpla_dgemv_core_kernel_streamed_cublas")
+    _native.check(K.mplak_fill_uniform_f64(x.data, n, 1235, 31, 0, 0.0, 1.0, stream_ptr))
+    L.mpla_generic_dgemv(C.byref(b), C.byref(A), C.byref(x), core, C.byref(inst))
+    _native.check(K.mplak_fill_uniform_f64(x0.data, n, 1235, 32, 0, 0.0, 1.0, stream_ptr))
+    _native.check(K.mplak_copy_f64(x.data, x0.data, n, stream_ptr))
+    torch.cuda.synchronize()
+    products = 0
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(ext)
+    for _ in range(trials):
+        L.mpla_generic_conjugate_gradient(C.byref(b), C.byref(A), C.byref(x), 50, 1.0e-15, core, C.byref(inst))
+        products += inst.last_matvecs
+        _native.check(K.mplak_copy_f64(x.data, x0.data, n, stream_ptr))
+    e1.record(ext)
+    e1.synchronize()
+    t = e0.elapsed_time(e1) * 1e-3
+    for v in (x, x0, b):
+        L.mpla_free_vector(C.byref(v), C.byref(inst))
+    L.mpla_free_generic_matrix(C.byref(A), C.byref(inst))
+    del P, table
+    torch.cuda.empty_cache()
+    log(f"reference benchmark_kernel_cg n={n}: {t:.3f} s for {products} FP64 products "
+        f"(published: {PUBLISHED_KERNEL_CG_S} s on a P100-class GPU)")
+    return {"n": n, "trials": trials, "seconds": t, "fp64_products": products, "ms_per_product": t / products * 1e3,
+            "entries_per_s": float(n) * n * products / t, "published_seconds": PUBLISHED_KERNEL_CG_S,
+            "published_hardware": "P100-class GPU (Piz Daint, 2017), different hardware",
+            "time_ratio_published_over_this": PUBLISHED_KERNEL_CG_S / t}
+
+
 def bench_served(args):
     """Served stream: Poisson arrivals of kernel-interpolation solves (N
     log-uniform in [1024, 65536], d in {2,3}, Gaussian l=0.05 or Matern-3/2
@@ -643,6 +696,7 @@ def main():
         configs["dense_dgemv_bicgstab"] = bench_dense_large(K, L, inst, stream_ptr, ext)
         configs["matern_block_n131072"] = bench_block(K, L, inst, stream_ptr, args)
         configs["served_stream"] = bench_served(args)
+        configs["reference_benchmark_kernel_cg_n131072"] = bench_reference_kernel_cg(K, L, inst, stream_ptr, ext)
 
     value = args.steps / wall
     out = {
