@@ -25,6 +25,7 @@
 #include <math.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "common.hpp"
 #include "entry.hpp"
@@ -170,7 +171,28 @@ __global__ __launch_bounds__(kBlock) void pack_v_kernel(const double* __restrict
   }
 }
 
-template <int KIND, int MODE, int CB>
+// DIM: the points' dimension (1..4); coordinates past it are not evaluated.
+// Squared distance over the first DIM coordinates.
+template <int DIM>
+__device__ __forceinline__ float dist2(const float (&p)[4], const float4& q) {
+  const float dx = p[0] - q.x;
+  float s = dx * dx;
+  if (DIM > 1) {
+    const float dy = p[1] - q.y;
+    s = fmaf(dy, dy, s);
+  }
+  if (DIM > 2) {
+    const float dz = p[2] - q.z;
+    s = fmaf(dz, dz, s);
+  }
+  if (DIM > 3) {
+    const float dw = p[3] - q.w;
+    s = fmaf(dw, dw, s);
+  }
+  return s;
+}
+
+template <int KIND, int MODE, int CB, int DIM>
 __global__ __launch_bounds__(kBlock) void mrhs_mfma_kernel(
     const double* const* __restrict__ points, int dim, int64_t row_off, int64_t rows, int64_t col_off,
     int64_t cols, int64_t slab_cols, double scale, int64_t R, const void* __restrict__ vpk, int64_t vpk_total,
@@ -189,7 +211,7 @@ __global__ __launch_bounds__(kBlock) void mrhs_mfma_kernel(
     int64_t i = base + h * 32 + (lane % 32);
     if (i >= rows) i = rows - 1;
 #pragma unroll
-    for (int d = 0; d < 4; ++d) pa[h][d] = d < dim ? (float)(points[d][row_off + i] * scale) : 0.0f;
+    for (int d = 0; d < 4; ++d) pa[h][d] = d < DIM ? (float)(points[d][row_off + i] * scale) : 0.0f;
   }
 
   f32x16 acc[2][CB];
@@ -209,9 +231,9 @@ __global__ __launch_bounds__(kBlock) void mrhs_mfma_kernel(
       float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
       if (j < c1) {
         q.x = (float)(points[0][col_off + j] * scale);
-        if (dim > 1) q.y = (float)(points[1][col_off + j] * scale);
-        if (dim > 2) q.z = (float)(points[2][col_off + j] * scale);
-        if (dim > 3) q.w = (float)(points[3][col_off + j] * scale);
+        if (DIM > 1) q.y = (float)(points[1][col_off + j] * scale);
+        if (DIM > 2) q.z = (float)(points[2][col_off + j] * scale);
+        if (DIM > 3) q.w = (float)(points[3][col_off + j] * scale);
       } else {
         q.x = 1.0e4f;  // far away: entries of padded sources underflow to 0
       }
@@ -230,9 +252,7 @@ __global__ __launch_bounds__(kBlock) void mrhs_mfma_kernel(
           const float4 q = lp[st * 16 + 2 * t + lane / 32];
 #pragma unroll
           for (int h = 0; h < 2; ++h) {
-            const float dx = pa[h][0] - q.x, dy = pa[h][1] - q.y, dz = pa[h][2] - q.z, dw = pa[h][3] - q.w;
-            const float s = fmaf(dw, dw, fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
-            e[h][t] = kernel_entry<float, KIND>(s, nullptr);
+            e[h][t] = kernel_entry<float, KIND>(dist2<DIM>(pa[h], q), nullptr);
           }
         }
       } else {
@@ -241,9 +261,7 @@ __global__ __launch_bounds__(kBlock) void mrhs_mfma_kernel(
           const float4 q = lp[st * 16 + 8 * (lane / 32) + t];
 #pragma unroll
           for (int h = 0; h < 2; ++h) {
-            const float dx = pa[h][0] - q.x, dy = pa[h][1] - q.y, dz = pa[h][2] - q.z, dw = pa[h][3] - q.w;
-            const float s = fmaf(dw, dw, fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
-            e[h][t] = kernel_entry<float, KIND>(s, nullptr);
+            e[h][t] = kernel_entry<float, KIND>(dist2<DIM>(pa[h], q), nullptr);
           }
         }
       }
@@ -409,18 +427,24 @@ void launch_mfma_mode(const mplak_kernel_desc* d, const MrhsPlan& p, hipStream_t
                       const void* vpk, double* part, const int32_t* skip) {
   const dim3 grid((unsigned)p.s.tiles, (unsigned)p.s.slabs);
   const int64_t total = p.steps * p.cblocks * kWave;
-#define MPLAK_MFMA_CASE(CBV)                                                                                   \
-  case CBV:                                                                                                    \
-    hipLaunchKernelGGL((mrhs_mfma_kernel<KIND, MODE, CBV>), grid, dim3(kBlock), 0, st, d->points, d->dim,       \
-                       d->row_offset, d->row_count, d->col_offset, d->col_count, p.s.slab_cols, scale, R, vpk, \
-                       total, part, skip);                                                                     \
-    break;
-  switch (p.cblocks) {
-    MPLAK_MFMA_CASE(1)
-    default:
-      MPLAK_MFMA_CASE(2)
-  }
-#undef MPLAK_MFMA_CASE
+  auto go = [&](auto cb_tag, auto dim_tag) {
+    constexpr int CBV = decltype(cb_tag)::value, DIMV = decltype(dim_tag)::value;
+    hipLaunchKernelGGL((mrhs_mfma_kernel<KIND, MODE, CBV, DIMV>), grid, dim3(kBlock), 0, st, d->points, d->dim,
+                       d->row_offset, d->row_count, d->col_offset, d->col_count, p.s.slab_cols, scale, R, vpk,
+                       total, part, skip);
+  };
+  auto with_dim = [&](auto cb_tag) {
+    switch (d->dim) {
+      case 1: go(cb_tag, std::integral_constant<int, 1>{}); break;
+      case 2: go(cb_tag, std::integral_constant<int, 2>{}); break;
+      case 3: go(cb_tag, std::integral_constant<int, 3>{}); break;
+      default: go(cb_tag, std::integral_constant<int, 4>{}); break;
+    }
+  };
+  if (p.cblocks == 1)
+    with_dim(std::integral_constant<int, 1>{});
+  else
+    with_dim(std::integral_constant<int, 2>{});
 }
 
 template <int KIND>

@@ -73,6 +73,22 @@ def test_mrhs_vs_oracle(lib, oracle, prec, R, kind, ell):
     assert max(errs) <= TOL[prec], errs
 
 
+@pytest.mark.parametrize("dim", [1, 2, 4])
+@pytest.mark.parametrize("prec,R", [(0, 3), (1, 16), (2, 32), (3, 20)])
+def test_mrhs_dims(lib, oracle, dim, prec, R):
+    """Every dimension the MFMA kernels specialise (and FP64) against the oracle."""
+    n = 1100
+    pts, V = case(oracle, n, dim, R, 300 + dim + R)
+    P = dev(pts)
+    T = point_table(P)
+    ell = 0.05 if dim > 1 else 0.005
+    Y = mrhs(lib, desc_for(T, dim, n, 0, ell, 0.5), prec, V)
+    k = oracle.Kernel(pts, 0, ell, nugget=0.5)
+    cols = [0, R // 2, R - 1]
+    ref = np.stack([k.matvec(V[:, r], mode=1) for r in cols], axis=1)
+    assert max(col_errors(Y[:, cols], ref)) <= TOL[prec]
+
+
 def test_mrhs_f64_matches_single_vector_product(lib, oracle):
     n, R = 2000, 6
     pts, V = case(oracle, n, 3, R, 7)
