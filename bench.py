@@ -409,8 +409,9 @@ def ordered_problem(K, table, stream_ptr):
 def measure_inner_product(K, table, xv, yv, stream_ptr, ext, reps=20):
     """Launch time (HIP events on the library stream) of the planned screened
     FP32 product, the kernel that dominates a mixed solve, and its executed
-    work: every kept 128 x 64 block is 8192 pair evaluations of 15 FLOP
-    (13 + the second accumulation) off the diagonal tiles, 13 FLOP on them."""
+    work: the plan's count of evaluated entries (256 per kept 16 x 16 tile on
+    compact target tiles, 8192 per kept 128 x 64 block on the others), 15 FLOP
+    each (13 + the second accumulation) off the diagonal tiles, 13 on them."""
     import torch
     from mpla_amd import _native
     op = ordered_problem(K, table, stream_ptr)
@@ -430,15 +431,16 @@ def measure_inner_product(K, table, xv, yv, stream_ptr, ext, reps=20):
     e1.record(ext)
     e1.synchronize()
     t = e0.elapsed_time(e1) / reps * 1e-3
-    counts = torch.zeros(3, dtype=torch.int64, device="cuda")
+    counts = torch.zeros(5, dtype=torch.int64, device="cuda")
     _native.check(K.mplak_screen_plan_counts(C.byref(d), plan.data_ptr(), pb, counts.data_ptr(), stream_ptr))
     torch.cuda.synchronize()
-    kept_pairs, kept, diag = (int(v) for v in counts.cpu())
+    kept_pairs, kept, diag, evals_off, evals_diag = (int(v) for v in counts.cpu())
     nt = (N + 511) // 512
-    flop = 8192.0 * ((kept - diag) * (FLOP_PER_ENTRY + 2) + diag * FLOP_PER_ENTRY)
-    return {"kernel": "kmv_sym_kernel<float, GAUSSIAN, 3, planned> + kmv_sym_finish_kernel",
+    flop = evals_off * (FLOP_PER_ENTRY + 2.0) + evals_diag * FLOP_PER_ENTRY
+    return {"kernel": "kmv_sym_planned_kernel<GAUSSIAN, 3> + kmv_sym_finish_kernel",
             "launch_ms": t * 1e3, "drop_log2": DROP_LOG2, "kept_pairs": kept_pairs,
             "all_pairs": nt * (nt + 1) // 2, "kept_blocks": kept, "all_blocks": 32 * nt * (nt + 1) // 2,
+            "evaluated_entries": evals_off + evals_diag,
             "executed_flop": flop, "executed_tflops": flop / t / 1e12,
             "reference_equivalent_tflops": FLOP_PER_ENTRY * float(N) * N / t / 1e12}
 
@@ -726,8 +728,9 @@ def main():
                      "unit": "TFLOP/s", "frac": inner["executed_tflops"] * 1e12 / PEAK_FP32,
                      "traffic": committed_traffic("planned_inner_product_n65536"),
                      "kernel": inner["kernel"], "launch_ms": inner["launch_ms"],
-                     "basis": "FLOP of the evaluated entries (kept 128x64 blocks x 8192 pairs x 15 FLOP off the "
-                              "diagonal tiles, 13 on them) per launch / launch time (HIP events, library stream)",
+                     "basis": "FLOP of the evaluated entries (the plan's count: 256 per kept 16x16 tile, 8192 per "
+                              "kept 128x64 block of non-compact tiles; 15 FLOP each off the diagonal tiles, 13 on "
+                              "them) per launch / launch time (HIP events, library stream)",
                      "frac_of_nonpacked_fp32": inner["executed_tflops"] * 1e12 / PEAK_FP32_NONPACKED,
                      "inner_product": inner},
         "roofline_exact_fp32": {"bound": "valu", "achieved": ach32 / 1e12, "peak": PEAK_FP32 / 1e12,

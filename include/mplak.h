@@ -156,23 +156,32 @@ int mplak_kernel_matvec_screened(const mplak_kernel_desc* desc, int precision, d
 
 /* No reference counterpart.  A screening plan for repeated screened products
  * with the same points, precision and drop level (an inner solve): built once,
- * it lists the 512 x 512 tile pairs that hold at least one kept block,
- * heaviest first, and owns the product's partial sums (so one planned product
- * at a time per plan).  mplak_kernel_matvec_planned gives bitwise the result
- * of mplak_kernel_matvec_screened with the same arguments, without launching
- * work for tile pairs whose entries are all dropped.  The desc, precision and
- * drop_log2 passed to the product must be the ones the plan was built with.
- * Square blocks with row range == column range and dim <= 3 only
- * (mplak_screen_plan_bytes returns 0 otherwise). */
+ * it lists the 512 x 512 tile pairs that hold kept entries, heaviest first,
+ * and owns the product's partial sums (so one planned product at a time per
+ * plan).  FP64 (or with a coincidence shift): mplak_kernel_matvec_planned
+ * gives bitwise the result of mplak_kernel_matvec_screened with the same
+ * arguments, without launching work for tile pairs whose entries are all
+ * dropped.  FP32 with an FP32 plan and no coincidence shift: on target tiles
+ * whose bounding box is compact (half-diagonal <= 7 in the scaled
+ * coordinates, the usual case after mplak_spatial_order) the squared
+ * distances come from the matrix cores in tile-centred form and the screen
+ * is applied per 16 x 16 block of points, so fewer entries are evaluated and
+ * the result agrees with the screened product to FP32 accuracy (still
+ * deterministic); other tiles are computed as by the screened product.  The
+ * desc, precision and drop_log2 passed to the product must be the ones the
+ * plan was built with.  Square blocks with row range == column range and
+ * dim <= 3 only (mplak_screen_plan_bytes returns 0 otherwise). */
 size_t mplak_screen_plan_bytes(const mplak_kernel_desc* desc);
 int mplak_screen_plan_build(const mplak_kernel_desc* desc, int precision, double drop_log2, void* plan,
                             size_t plan_bytes, mplak_stream_t stream);
 int mplak_kernel_matvec_planned(const mplak_kernel_desc* desc, int precision, double drop_log2, void* plan,
                                 size_t plan_bytes, const double* x, double* y, const int32_t* skip_if,
                                 mplak_stream_t stream);
-/* Writes counts[0] = tile pairs the plan keeps, counts[1] = 128 x 64 blocks
+/* Writes counts[0] = tile pairs the plan lists, counts[1] = 128 x 64 blocks
  * they hold that pass the screen, counts[2] = those of counts[1] on diagonal
- * tile pairs (counts: 3 int64 in device memory). */
+ * tile pairs, counts[3] and counts[4] = kernel entries the planned FP32
+ * product (no coincidence shift) evaluates on off-diagonal and on diagonal
+ * tile pairs (counts: 5 int64 in device memory). */
 int mplak_screen_plan_counts(const mplak_kernel_desc* desc, const void* plan, size_t plan_bytes, int64_t* counts,
                              mplak_stream_t stream);
 

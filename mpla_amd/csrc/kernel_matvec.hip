@@ -284,53 +284,34 @@ __device__ __forceinline__ T box_gap2(const T* a, const T* b) {
 
 enum SymMode { kSymExact = 0, kSymScreened = 1, kSymPlanned = 2 };
 
-// kSymScreened: a wave skips a 64-source group when the bounding boxes of its
-// 128 targets and of the group are further apart (squared, in the scaled
-// coordinates) than drop_s, i.e. when every entry of that 128 x 64 block is
-// below the caller's drop level.  The test uses wave-reduced boxes, so the
-// branch is uniform across the wave.
-// kSymPlanned: as kSymScreened, over the tile pairs list[0 .. *count) of a
-// screening plan (pairs with at least one kept block, heaviest first); the
-// partials of the other pairs were zeroed when the plan was built.
-template <typename T, int KIND, int DIM, bool COINC, int MODE>
-__global__ __launch_bounds__(kBlock) void kmv_sym_kernel(
-    const double* const* __restrict__ points, int64_t off, int64_t n, double scale, double coinc_thr,
-    double coinc_shift, double drop_s, const double* __restrict__ x, double* __restrict__ part_i,
-    double* __restrict__ part_j, const uint32_t* __restrict__ list, const int64_t* __restrict__ count,
-    const int32_t* __restrict__ skip) {
-  constexpr bool SCREEN = MODE != kSymExact;
-  // CENTER: on compact target tiles (spatially ordered points) the screened
-  // FP32 product evaluates |p - q|^2 as n_p + n_q - 2 p.q in coordinates
-  // centred on the target tile (n = squared norm; one add and DIM FMAs
-  // instead of 2 DIM subtractions and DIM products).  The tile-centred norms
-  // are small, so the cancellation error stays at the level of the FP32
-  // coordinates themselves; tiles wider than kCenterRadius2 (unordered
-  // points) and the coincidence test (which needs s == 0 exactly) use the
-  // difference form.
-  constexpr bool CENTER = SCREEN && !COINC && sizeof(T) == 4;
-  constexpr float kCenterRadius2 = 49.0f;
-  if (skip != nullptr && *skip != 0) return;
-  if (MODE == kSymPlanned && (int64_t)blockIdx.x >= *count) return;
+// One tile pair (I, J) of the symmetric product on the vector unit.
+// Screening (SCREEN): a wave skips a 64-source group when every entry of
+// that 128 x 64 block is below the drop level.  MASKED takes that decision
+// from bit 8 w + g of `wmask` (a screening plan); otherwise it compares the
+// wave-reduced bounding boxes of the wave's 128 targets and of the group
+// (squared gap in scaled coordinates > drop_s), so the branch is uniform
+// across the wave.  src, gbox, own_red and exp_tab are the caller's LDS.
+// One staged 64-source group: stored twice in a row in FP32 (the rotated
+// read (l + s) mod 64 becomes an immediate offset), once in FP64.
+template <typename T>
+using SymSrc = typename Quad<T>::type[(sizeof(T) == 4 ? 2 : 1) * kWave];
+
+template <typename T, int KIND, int DIM, bool COINC, bool SCREEN, bool MASKED>
+__device__ __forceinline__ void sym_valu_pair(const double* const* __restrict__ points, int64_t off, int64_t n,
+                                              double scale, double coinc_thr, double coinc_shift, double drop_s,
+                                              const double* __restrict__ x, double* __restrict__ part_i,
+                                              double* __restrict__ part_j, int64_t p, uint32_t pmask,
+                                              SymSrc<T>* src, T (*gbox)[6], T* own_red, const double* exp_tab) {
   typedef typename Quad<T>::type Q;
   constexpr int kDup = sizeof(T) == 4 ? 2 : 1;  // copies of each 64-source group
   constexpr int kGroups = kSymTile / kWave;
   // Column-side sums of the four waves, red[w * kSymTile + k].  In FP32 each
   // wave holds its 8 group sums in registers and they reuse the source buffer
-  // once every wave is done with it (16 KB of LDS per workgroup, 8 resident
-  // workgroups per CU); in FP64 registers are the scarcer resource, and the
-  // sums go to a buffer of their own as each group completes.
+  // once every wave is done with it; in FP64 registers are the scarcer
+  // resource, and the sums go to a buffer of their own as each group
+  // completes.
   constexpr bool kHoldSums = sizeof(T) == 4;
-  static_assert(sizeof(Q) * kGroups * kDup * kWave >= sizeof(T) * (kBlock / kWave) * kSymTile, "LDS reuse");
-  __shared__ Q src[kGroups][kDup * kWave];
-  __shared__ T gbox[kGroups][6];
-  __shared__ T own_red[kHoldSums ? 1 : (kBlock / kWave) * kSymTile];
-  __shared__ double exp_tab[Exp2Lds<T>::size];
-  if (sizeof(T) == 8) fill_exp2_table(exp_tab);  // first read after the staging barrier
-  __shared__ T nsrc[CENTER ? kGroups : 1][CENTER ? kDup * kWave : 1];  // centred source norms
-  __shared__ T ibox[CENTER ? kBlock / kWave : 1][6];                 // target-tile box per wave
   T* red = kHoldSums ? reinterpret_cast<T*>(&src[0][0]) : own_red;
-
-  const int64_t p = MODE == kSymPlanned ? (int64_t)list[blockIdx.x] : (int64_t)blockIdx.x;
   int64_t I, J;
   decode_pair(p, I, J);
 
@@ -338,6 +319,7 @@ __global__ __launch_bounds__(kBlock) void kmv_sym_kernel(
   const int lane = tid & (kWave - 1);
   const int w = tid / kWave;
   const T thr = (T)coinc_thr, shift = (T)coinc_shift;
+  const uint32_t wmask = MASKED ? (uint32_t)__builtin_amdgcn_readfirstlane((int)((pmask >> (8 * w)) & 0xffu)) : 0u;
 
   // two targets per lane: tile offsets w*128 + lane and w*128 + 64 + lane
   T pi[2][DIM], xi[2];
@@ -349,37 +331,14 @@ __global__ __launch_bounds__(kBlock) void kmv_sym_kernel(
     for (int d = 0; d < DIM; ++d) pi[h][d] = (T)(points[d][off + ic] * scale);
     xi[h] = i < n ? (T)x[i] : T(0);
   }
+  constexpr bool kBoxes = SCREEN && !MASKED;
   T tbox[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
-  if (SCREEN) {
+  if (kBoxes) {
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
       tbox[d] = wave_min(fmin(pi[0][d], pi[1][d]));
       tbox[3 + d] = wave_max(fmax(pi[0][d], pi[1][d]));
     }
-  }
-  // centring: the target tile's box from the four wave boxes
-  T ctr[3] = {T(0), T(0), T(0)};
-  bool centered = false;
-  if (CENTER) {
-    if (lane == 0) {
-#pragma unroll
-      for (int q = 0; q < 6; ++q) ibox[w][q] = tbox[q];
-    }
-    __syncthreads();
-    T r2 = T(0);
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) {
-      T lo = ibox[0][d], hi = ibox[0][3 + d];
-#pragma unroll
-      for (int q = 1; q < kBlock / kWave; ++q) {
-        lo = fmin(lo, ibox[q][d]);
-        hi = fmax(hi, ibox[q][3 + d]);
-      }
-      ctr[d] = T(0.5) * (lo + hi);
-      const T half = T(0.5) * (hi - lo);
-      r2 = fma(half, half, r2);
-    }
-    centered = __builtin_amdgcn_readfirstlane((int)(r2 <= (T)kCenterRadius2)) != 0;
   }
   for (int k = tid; k < kSymTile; k += kBlock) {
     const int64_t j = J * kSymTile + k;
@@ -389,7 +348,7 @@ __global__ __launch_bounds__(kBlock) void kmv_sym_kernel(
     q.y = DIM > 1 ? (T)(points[1][off + jc] * scale) : T(0);
     q.z = DIM > 2 ? (T)(points[2][off + jc] * scale) : T(0);
     q.w = j < n ? (T)x[j] : T(0);
-    if (SCREEN) {  // k / kWave is uniform across the wave: one group per wave and pass
+    if (kBoxes) {  // k / kWave is uniform across the wave: one group per wave and pass
       const T lo0 = wave_min(q.x), lo1 = wave_min(q.y), lo2 = wave_min(q.z);
       const T hi0 = wave_max(q.x), hi1 = wave_max(q.y), hi2 = wave_max(q.z);
       if (lane == 0) {
@@ -397,53 +356,19 @@ __global__ __launch_bounds__(kBlock) void kmv_sym_kernel(
         b[0] = lo0, b[1] = lo1, b[2] = lo2, b[3] = hi0, b[4] = hi1, b[5] = hi2;
       }
     }
-    if (CENTER && centered) {
-      q.x -= ctr[0];
-      if (DIM > 1) q.y -= ctr[1];
-      if (DIM > 2) q.z -= ctr[2];
-      T nq = q.x * q.x;
-      if (DIM > 1) nq = fma(q.y, q.y, nq);
-      if (DIM > 2) nq = fma(q.z, q.z, nq);
-#pragma unroll
-      for (int c = 0; c < kDup; ++c) nsrc[k / kWave][c * kWave + (k & (kWave - 1))] = nq;
-    }
 #pragma unroll
     for (int c = 0; c < kDup; ++c) src[k / kWave][c * kWave + (k & (kWave - 1))] = q;
-  }
-  // centred targets: pa = -2 (p - c), np = |p - c|^2
-  T pa[2][DIM], np_[2] = {T(0), T(0)};
-  if (CENTER && centered) {
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-#pragma unroll
-      for (int d = 0; d < DIM; ++d) {
-        const T v = pi[h][d] - ctr[d];
-        pa[h][d] = T(-2) * v;
-        np_[h] = d == 0 ? v * v : fma(v, v, np_[h]);
-      }
-    }
   }
   // true when every entry between this wave's targets and group g is dropped
   auto far_group = [&](int g) -> bool {
     if (!SCREEN) return false;
+    if (MASKED) return ((wmask >> g) & 1u) == 0;
     const T d2 = box_gap2<T, DIM>(tbox, gbox[g]);
     return __builtin_amdgcn_readfirstlane((int)(d2 > (T)drop_s)) != 0;
   };
   __syncthreads();
 
   T accI[2] = {T(0), T(0)};
-  // centred form: s = (n_p + n_q) + pa . q
-  auto entry2c = [&](const Q& q, T nq, T (&e)[2]) {
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      T s = np_[h] + nq;
-      s = fma(pa[h][0], q.x, s);
-      if (DIM > 1) s = fma(pa[h][1], q.y, s);
-      if (DIM > 2) s = fma(pa[h][2], q.z, s);
-      if (KIND == MPLAK_KERNEL_MATERN32) s = fmax(s, T(0));  // sqrt of a rounding-negative s
-      e[h] = kernel_entry<T, KIND>(s, exp_tab);
-    }
-  };
   auto entry2 = [&](const Q& q, T (&e)[2]) {
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
@@ -464,39 +389,28 @@ __global__ __launch_bounds__(kBlock) void kmv_sym_kernel(
   };
 
   if (I == J) {
-    auto diag_group = [&](int g, auto centred) {
+#pragma unroll 1
+    for (int g = 0; g < kGroups; ++g) {
+      if (far_group(g)) continue;
 #pragma unroll 4
       for (int s = 0; s < kWave; ++s) {
         const Q q = src[g][s];
         T e[2];
-        if constexpr (decltype(centred)::value)
-          entry2c(q, nsrc[g][s], e);
-        else
-          entry2(q, e);
+        entry2(q, e);
         accI[0] = fma(e[0], q.w, accI[0]);
         accI[1] = fma(e[1], q.w, accI[1]);
       }
-    };
-#pragma unroll 1
-    for (int g = 0; g < kGroups; ++g) {
-      if (far_group(g)) continue;
-      if (CENTER && centered)
-        diag_group(g, std::true_type{});
-      else
-        diag_group(g, std::false_type{});
     }
   } else {
-    auto rotate_group = [&](int g, auto centred) -> T {
+    auto group_sum = [&](int g) -> T {
+      if (far_group(g)) return T(0);
       T t = T(0);
 #pragma unroll 4
       for (int s = 0; s < kWave; ++s) {
         const int jj = kDup == 2 ? lane + s : (lane + s) & (kWave - 1);
         const Q q = src[g][jj];
         T e[2];
-        if constexpr (decltype(centred)::value)
-          entry2c(q, nsrc[g][jj], e);
-        else
-          entry2(q, e);
+        entry2(q, e);
         accI[0] = fma(e[0], q.w, accI[0]);
         accI[1] = fma(e[1], q.w, accI[1]);
         t = fma(e[0], xi[0], t);
@@ -504,11 +418,6 @@ __global__ __launch_bounds__(kBlock) void kmv_sym_kernel(
         t = wave_rotate(t);
       }
       return t;
-    };
-    auto group_sum = [&](int g) -> T {
-      if (far_group(g)) return T(0);
-      if (CENTER && centered) return rotate_group(g, std::true_type{});
-      return rotate_group(g, std::false_type{});
     };
     if constexpr (kHoldSums) {
       T tg[kGroups];
@@ -535,6 +444,260 @@ __global__ __launch_bounds__(kBlock) void kmv_sym_kernel(
       for (int q = 1; q < kBlock / kWave; ++q) s += (double)red[q * kSymTile + k];
       part_j[p * kSymTile + k] = s;
     }
+  }
+}
+
+// Symmetric product kernel on the vector unit: kSymExact / kSymScreened over
+// all tile pairs (workgroup p = pair p), kSymPlanned over the tile pairs
+// list[0 .. *count) of a screening plan (pairs with a kept block; the
+// partials of the others were zeroed when the plan was built).
+template <typename T, int KIND, int DIM, bool COINC, int MODE>
+__global__ __launch_bounds__(kBlock) void kmv_sym_kernel(
+    const double* const* __restrict__ points, int64_t off, int64_t n, double scale, double coinc_thr,
+    double coinc_shift, double drop_s, const double* __restrict__ x, double* __restrict__ part_i,
+    double* __restrict__ part_j, const uint32_t* __restrict__ list, const int64_t* __restrict__ count,
+    const int32_t* __restrict__ skip) {
+  if (skip != nullptr && *skip != 0) return;
+  if (MODE == kSymPlanned && (int64_t)blockIdx.x >= *count) return;
+  typedef typename Quad<T>::type Q;
+  constexpr int kDup = sizeof(T) == 4 ? 2 : 1;
+  constexpr int kGroups = kSymTile / kWave;
+  constexpr bool kHoldSums = sizeof(T) == 4;
+  static_assert(sizeof(Q) * kGroups * kDup * kWave >= sizeof(T) * (kBlock / kWave) * kSymTile, "LDS reuse");
+  __shared__ Q src[kGroups][kDup * kWave];
+  static_assert(sizeof(SymSrc<T>) == sizeof(src[0]), "staged group layout");
+  __shared__ T gbox[kGroups][6];
+  __shared__ T own_red[kHoldSums ? 1 : (kBlock / kWave) * kSymTile];
+  __shared__ double exp_tab[Exp2Lds<T>::size];
+  if (sizeof(T) == 8) fill_exp2_table(exp_tab);  // first read after the staging barrier
+  const int64_t p = MODE == kSymPlanned ? (int64_t)list[blockIdx.x] : (int64_t)blockIdx.x;
+  sym_valu_pair<T, KIND, DIM, COINC, MODE != kSymExact, false>(
+      points, off, n, scale, coinc_thr, coinc_shift, drop_s, x, part_i, part_j, p, 0u,
+      reinterpret_cast<SymSrc<T>*>(&src[0][0]), gbox, own_red, exp_tab);
+}
+
+// ---------------------------------------------------------------------------
+// Planned FP32 product (plan built for FP32, no coincidence shift): one
+// workgroup per listed tile pair, heaviest first.
+//
+// Compact target tile: matrix-core form.  In coordinates centred on the
+// target tile, s = |p - q|^2 = n_p + n_q - 2 p.q (n = squared norm), and
+// v_mfma_f32_16x16x4f32 with A = [p, n_p] (16 targets), B = [-2 q, 1] (16
+// sources) and C = n_q (one value per lane) gives s for a 16 x 16 tile in
+// the accumulator layout: lane 16 r + c holds targets 4 r + v (v < 4)
+// against source c.  The vector unit is left with the exponential and the
+// two accumulations.  The tile-centred norms are small (half-diagonal^2 <=
+// kCenterRadius2), so the cancellation error stays at the level of the FP32
+// coordinates themselves.  Both tiles are staged in LDS in operand layout.
+// Wave w takes targets w*128 .. +127 (8 slabs of 16) against the 16 x 16
+// tiles the plan keeps (tile mask of (w, group g): bit 8 b + a for target
+// slab a and source slab b of the group); the branches are on scalar masks.
+// Row sums accR[a][v] collect over the sources of the lane's column and are
+// added over the 16 lanes of a DPP row once per tile pair; column sums
+// collect over the lane's 4 targets of every slab and are added over the four
+// lane rows once per group, then over the four waves in wave order.
+//
+// Other target tiles: the vector unit's difference form over the plan's kept
+// 128 x 64 blocks.  All orders are fixed, so the product is deterministic.
+// ---------------------------------------------------------------------------
+constexpr float kCenterRadius2 = 49.0f;
+constexpr int kSymGroups = kSymTile / kWave;
+constexpr int kSlabs = kSymTile / 16;                      // 16-point slabs per tile
+constexpr int kTmaskWords = (kBlock / kWave) * kSymGroups;  // tile masks per pair
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// Sum over the 16 lanes of a DPP row (butterfly; every lane of the row ends
+// with the same bits).
+__device__ __forceinline__ float row16_sum(float v) {
+  v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));   // quad_perm [1,0,3,2]
+  v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, false));   // quad_perm [2,3,0,1]
+  v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x141, 0xF, 0xF, false));  // row_half_mirror
+  v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x140, 0xF, 0xF, false));  // row_mirror
+  return v;
+}
+
+// LDS layout of the matrix-core form (floats): operands [32 slabs][64 lanes]
+// of the targets (lane 16 k + c: coordinate k of target 16 slab + c, k < 3,
+// and its norm at k = 3) and of the sources (-2 x coordinate, and 1), then
+// the source norms, source values and target values.
+constexpr int kMfOp = kSlabs * kWave;
+constexpr int kMfB = kMfOp;
+constexpr int kMfNorm = 2 * kMfOp;
+constexpr int kMfXs = kMfNorm + kSymTile;
+constexpr int kMfXt = kMfXs + kSymTile;
+constexpr int kMfFloats = kMfXt + kSymTile;
+
+template <int KIND, int DIM>
+__device__ __forceinline__ void sym_mfma_pair(const double* const* __restrict__ points, int64_t off, int64_t n,
+                                              double scale, const double* __restrict__ x,
+                                              double* __restrict__ part_i, double* __restrict__ part_j, int64_t p,
+                                              int64_t I, int64_t J, float4 centre,
+                                              const uint32_t* __restrict__ tmask, float* mf) {
+  const int tid = threadIdx.x, lane = tid & (kWave - 1);
+  const int w = __builtin_amdgcn_readfirstlane(tid / kWave);
+  const int c16 = lane & 15, r4 = lane >> 4;
+  const float ctr[3] = {centre.x, centre.y, centre.z};
+
+  // staging: centred coordinates c, norm |c|^2, value x (0 past the end)
+  auto stage = [&](int64_t X, int k, bool source, bool target) {
+    const int64_t j = X * kSymTile + k;
+    const int64_t jc = j < n ? j : n - 1;
+    float c[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) c[d] = (float)(points[d][off + jc] * scale) - ctr[d];
+    float nc = c[0] * c[0];
+    if (DIM > 1) nc = fmaf(c[1], c[1], nc);
+    if (DIM > 2) nc = fmaf(c[2], c[2], nc);
+    const float xv = j < n ? (float)x[j] : 0.0f;
+    const int o = (k >> 4) * kWave + (k & 15);
+    if (source) {
+      mf[kMfB + o] = -2.0f * c[0];
+      mf[kMfB + o + 16] = -2.0f * c[1];
+      mf[kMfB + o + 32] = -2.0f * c[2];
+      mf[kMfB + o + 48] = 1.0f;
+      mf[kMfNorm + k] = nc;
+      mf[kMfXs + k] = xv;
+    }
+    if (target) {
+      mf[o] = c[0];
+      mf[o + 16] = c[1];
+      mf[o + 32] = c[2];
+      mf[o + 48] = nc;
+      mf[kMfXt + k] = xv;
+    }
+  };
+  for (int k = tid; k < kSymTile; k += kBlock) {
+    if (I == J) {
+      stage(I, k, true, true);
+    } else {
+      stage(J, k, true, false);
+      stage(I, k, false, true);
+    }
+  }
+  const uint32_t* tm = tmask + p * kTmaskWords + w * kSymGroups;
+  __syncthreads();
+
+  float A[8], xm[8][4], accR[8][4];
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    A[a] = mf[(w * 8 + a) * kWave + lane];
+    const f32x4 xv = *reinterpret_cast<const f32x4*>(&mf[kMfXt + w * 2 * kWave + 16 * a + 4 * r4]);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      xm[a][v] = xv[v];
+      accR[a][v] = 0.0f;
+    }
+  }
+  auto mfma_group = [&](int g, uint32_t gm, auto diagonal) -> float {
+    constexpr bool DIAG = decltype(diagonal)::value;
+    float accC[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const uint32_t bm = (gm >> (8 * b)) & 0xffu;
+      if (bm == 0) continue;
+      const float bv = mf[kMfB + (g * 4 + b) * kWave + lane];
+      const float nq = mf[kMfNorm + g * kWave + 16 * b + c16];
+      const float xn = mf[kMfXs + g * kWave + 16 * b + c16];
+      const f32x4 cq = {nq, nq, nq, nq};
+#pragma unroll
+      for (int a = 0; a < 8; ++a) {
+        if (((bm >> a) & 1u) == 0) continue;
+        const f32x4 s = __builtin_amdgcn_mfma_f32_16x16x4f32(A[a], bv, cq, 0, 0, 0);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          float sv = s[v];
+          if (KIND == MPLAK_KERNEL_MATERN32) sv = fmaxf(sv, 0.0f);  // sqrt of a rounding-negative s
+          const float e = kernel_entry<float, KIND>(sv, nullptr);
+          accR[a][v] = fmaf(e, xn, accR[a][v]);
+          if (!DIAG) accC[b] = fmaf(e, xm[a][v], accC[b]);
+        }
+      }
+    }
+    if (DIAG) return 0.0f;
+    // lane 16 r + c keeps the sum of source 16 r + c of the group
+    float mine = 0.0f;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      float t = accC[b];
+      t += __shfl_xor(t, 16);
+      t += __shfl_xor(t, 32);
+      if (r4 == b) mine = t;
+    }
+    return mine;
+  };
+  float tg[kSymGroups];
+#pragma unroll
+  for (int g = 0; g < kSymGroups; ++g) tg[g] = 0.0f;
+  if (I == J) {
+#pragma unroll 1
+    for (int g = 0; g < kSymGroups; ++g) {
+      const uint32_t gm = (uint32_t)__builtin_amdgcn_readfirstlane((int)tm[g]);
+      if (gm != 0) mfma_group(g, gm, std::true_type{});
+    }
+  } else {
+#pragma unroll 1
+    for (int g = 0; g < kSymGroups; ++g) {
+      const uint32_t gm = (uint32_t)__builtin_amdgcn_readfirstlane((int)tm[g]);
+      if (gm == 0) continue;
+      const float v = mfma_group(g, gm, std::false_type{});
+#pragma unroll
+      for (int q = 0; q < kSymGroups; ++q)
+        if (q == g) tg[q] = v;
+    }
+  }
+  // row sums: lane 16 r + c writes rows q = c and q = c + 16 of its lane
+  // row's 32 (q = 4 a + v, tile row w * 128 + 16 a + 4 r + v)
+  float out[2] = {0.0f, 0.0f};
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const float s = row16_sum(accR[a][v]);
+      if (4 * a + v == c16) out[0] = s;
+      if (4 * a + v == c16 + 16) out[1] = s;
+    }
+  }
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int k = w * 2 * kWave + 64 * h + 16 * (c16 >> 2) + 4 * r4 + (c16 & 3);
+    part_i[p * kSymTile + k] = (double)out[h];
+  }
+  if (I != J) {
+    __syncthreads();  // every wave is done reading the staged tiles
+#pragma unroll
+    for (int g = 0; g < kSymGroups; ++g) mf[w * kSymTile + g * kWave + lane] = tg[g];
+    __syncthreads();
+    for (int k = tid; k < kSymTile; k += kBlock) {
+      double s = (double)mf[k];
+#pragma unroll
+      for (int q = 1; q < kBlock / kWave; ++q) s += (double)mf[q * kSymTile + k];
+      part_j[p * kSymTile + k] = s;
+    }
+  }
+}
+
+template <int KIND, int DIM>
+__global__ __launch_bounds__(kBlock) void kmv_sym_planned_kernel(
+    const double* const* __restrict__ points, int64_t off, int64_t n, double scale, const double* __restrict__ x,
+    double* __restrict__ part_i, double* __restrict__ part_j, const uint32_t* __restrict__ list,
+    const int64_t* __restrict__ count, const float4* __restrict__ tinfo, const uint32_t* __restrict__ bmask,
+    const uint32_t* __restrict__ tmask, const int32_t* __restrict__ skip) {
+  if (skip != nullptr && *skip != 0) return;
+  if ((int64_t)blockIdx.x >= *count) return;
+  __shared__ __attribute__((aligned(16))) float mf[kMfFloats];
+  static_assert((kBlock / kWave) * kSymTile <= kMfFloats, "column sums reuse the staging buffer");
+  static_assert(sizeof(SymSrc<float>) * kSymGroups <= sizeof(mf), "vector-unit staging fits");
+  const int64_t p = (int64_t)list[blockIdx.x];
+  int64_t I, J;
+  decode_pair(p, I, J);
+  const float4 centre = tinfo[I];
+  if (centre.w != 0.0f) {
+    sym_mfma_pair<KIND, DIM>(points, off, n, scale, x, part_i, part_j, p, I, J, centre, tmask, mf);
+  } else {
+    sym_valu_pair<float, KIND, DIM, false, true, true>(points, off, n, scale, 0.0, 0.0, 0.0, x, part_i, part_j, p,
+                                                       bmask[p], reinterpret_cast<SymSrc<float>*>(mf), nullptr,
+                                                       nullptr, nullptr);
   }
 }
 
@@ -604,6 +767,10 @@ struct SymArgs {
   double *pi, *pj;
   const uint32_t* list;
   const int64_t* count;
+  // planned FP32 without coincidence shift: the plan's tile data, for the
+  // planned kernel (matrix-core form on compact target tiles)
+  const float4* tinfo;
+  const uint32_t *bmask, *tmask;
   const int32_t* skip;
 };
 
@@ -611,6 +778,13 @@ template <typename T, int KIND, int DIM, int MODE>
 void launch_sym_dim(hipStream_t st, bool coinc, const mplak_kernel_desc* d, const SymArgs& a) {
   const int64_t nt = sym_tiles(d->row_count);
   const dim3 grid((unsigned)(nt * (nt + 1) / 2));
+  if constexpr (MODE == kSymPlanned && sizeof(T) == 4) {
+    if (a.tinfo != nullptr && !coinc) {
+      hipLaunchKernelGGL((kmv_sym_planned_kernel<KIND, DIM>), grid, dim3(kBlock), 0, st, d->points, d->row_offset,
+                         d->row_count, a.scale, a.x, a.pi, a.pj, a.list, a.count, a.tinfo, a.bmask, a.tmask, a.skip);
+      return;
+    }
+  }
   if (coinc)
     hipLaunchKernelGGL((kmv_sym_kernel<T, KIND, DIM, true, MODE>), grid, dim3(kBlock), 0, st, d->points,
                        d->row_offset, d->row_count, a.scale, a.thr, d->coincident_shift, a.drop_s, a.x, a.pi, a.pj,
@@ -659,9 +833,14 @@ size_t sym_workspace_bytes(int64_t n) {
 // Symmetric product: tile-pair kernel (exact, screened or planned) and the
 // finish.  `work` holds part_i then part_j ([pairs][512] doubles each).
 int run_sym(const mplak_kernel_desc* d, int precision, int mode, double drop_s, const uint32_t* list,
-            const int64_t* count, const uint32_t* partners, const int32_t* pcount, int64_t pcap, const double* x,
-            double* y, void* work, const int32_t* skip_if, mplak_stream_t s) {
+            const int64_t* count, const float4* tinfo, const uint32_t* bmask, const uint32_t* tmask,
+            const uint32_t* partners, const int32_t* pcount, int64_t pcap, const double* x, double* y, void* work,
+            const int32_t* skip_if, mplak_stream_t s) {
   SymArgs a;
+  const bool tiles = mode == kSymPlanned && precision == MPLAK_PREC_F32;
+  a.tinfo = tiles ? tinfo : nullptr;
+  a.bmask = bmask;
+  a.tmask = tmask;
   a.scale = coord_scale(d->kind, d->length_scale);
   const bool coinc = d->coincident_shift != 0.0;
   a.thr = (1e-14 * a.scale) * (1e-14 * a.scale);
@@ -698,19 +877,28 @@ int run_sym(const mplak_kernel_desc* d, int precision, int mode, double drop_s, 
 
 
 // ---------------------------------------------------------------------------
-// Screening plan: which tile pairs of the symmetric product have at least one
-// kept 128 x 64 block (same test as the screened kernel, on the same
-// precision-cast coordinates), ordered by kept block count, heaviest first
-// (ties by pair index), so that the planned product launches no all-dropped
-// pair and starts the longest workgroups first.  The partials of the dropped
-// pairs are zeroed once at build time and never written again.
+// Screening plan.  For every tile pair p = (I, J) of the symmetric product:
+//  - bmask[p]: bit 8 w + g set when the 128 x 64 block of wave w's targets
+//    of tile I and source group g of tile J is kept (the screened kernel's
+//    test, on the same precision-cast coordinates);
+//  - tmask[p][w][g]: bit 8 b + a set when the 16 x 16 tile of target slab a
+//    of wave w and source slab b of group g is kept (the matrix-core form's
+//    granularity; a kept tile lies in a kept block);
+// and for every tile the centre of its box and whether it is compact
+// (half-diagonal^2 <= kCenterRadius2, FP32 plans only).  Pairs are listed by
+// estimated cost, heaviest first (ties by pair index): kept 16 x 16 tiles /
+// 32 on compact target tiles, 3 x kept blocks on the others (the vector-unit
+// form costs about three times as much per entry).  Pairs with nothing kept
+// are left out, and their partials are zeroed once at build time.
 // ---------------------------------------------------------------------------
 
-constexpr int kSymGroups = kSymTile / kWave;
-constexpr uint64_t kPlanDropped = 63;  // key class of pairs with no kept block
+constexpr uint64_t kPlanDropped = 127;  // key class of pairs with nothing kept
+constexpr int kPlanKeyBits = 7;         // key classes: 96 - cost (cost 1 .. 96) and kPlanDropped
+constexpr int kPlanCounts = 5;          // entries written by mplak_screen_plan_counts
 
 struct PlanLayout {
-  size_t count, boxes, list, keys_in, keys_out, tmp, tmp_bytes, pcount, partners, part, total;
+  size_t count, tinfo, boxes, sboxes, list, bmask, tmask, keys_in, keys_out, tmp, tmp_bytes, pcount, partners, part,
+      total;
   int64_t pcap;  // partner list capacity per (tile, slice)
 };
 
@@ -719,14 +907,18 @@ PlanLayout plan_layout(int64_t n) {
   const int64_t nt = sym_tiles(n);
   const size_t np = (size_t)(nt * (nt + 1) / 2);
   L.count = 0;
-  L.boxes = 256;
-  L.list = L.boxes + align_up(sizeof(double) * 6 * kSymGroups * (size_t)nt, 256);
-  L.keys_in = L.list + align_up(sizeof(uint32_t) * np, 256);
+  L.tinfo = 256;
+  L.boxes = L.tinfo + align_up(sizeof(float4) * (size_t)nt, 256);
+  L.sboxes = L.boxes + align_up(sizeof(double) * 6 * kSymGroups * (size_t)nt, 256);
+  L.list = L.sboxes + align_up(sizeof(double) * 6 * kSlabs * (size_t)nt, 256);
+  L.bmask = L.list + align_up(sizeof(uint32_t) * np, 256);
+  L.tmask = L.bmask + align_up(sizeof(uint32_t) * np, 256);
+  L.keys_in = L.tmask + align_up(sizeof(uint32_t) * kTmaskWords * np, 256);
   L.keys_out = L.keys_in + align_up(sizeof(uint64_t) * np, 256);
   L.tmp = L.keys_out + align_up(sizeof(uint64_t) * np, 256);
   size_t tb = 0;
   if (rocprim::radix_sort_keys(nullptr, tb, static_cast<uint64_t*>(nullptr), static_cast<uint64_t*>(nullptr),
-                               (unsigned)np, 32, 38) != hipSuccess)
+                               (unsigned)np, 32, 32 + kPlanKeyBits) != hipSuccess)
     return PlanLayout{};
   L.tmp_bytes = tb;
   L.pcap = (nt + kFinishSlices - 1) / kFinishSlices;
@@ -737,12 +929,24 @@ PlanLayout plan_layout(int64_t n) {
   return L;
 }
 
-// Bounding box of every 64-point group of every tile, from the same
-// precision-cast scaled coordinates the product kernel uses (padding points
-// are clamped to the last point, as there).
+template <typename T>
+__device__ __forceinline__ T min_over_lanes(T v, int from, int to) {
+  for (int m = from; m < to; m <<= 1) v = fmin(v, __shfl_xor(v, m));
+  return v;
+}
+template <typename T>
+__device__ __forceinline__ T max_over_lanes(T v, int from, int to) {
+  for (int m = from; m < to; m <<= 1) v = fmax(v, __shfl_xor(v, m));
+  return v;
+}
+
+// Bounding boxes of every 16-point slab and 64-point group of every tile,
+// from the same precision-cast scaled coordinates the product kernels use
+// (padding points are clamped to the last point, as there).
 template <typename T, int DIM>
 __global__ __launch_bounds__(kBlock) void plan_box_kernel(const double* const* __restrict__ points, int64_t off,
-                                                          int64_t n, double scale, double* __restrict__ boxes) {
+                                                          int64_t n, double scale, double* __restrict__ boxes,
+                                                          double* __restrict__ sboxes) {
   const int lane = threadIdx.x & (kWave - 1);
   for (int k = threadIdx.x; k < kSymTile; k += kBlock) {
     const int64_t j = (int64_t)blockIdx.x * kSymTile + k;
@@ -751,8 +955,20 @@ __global__ __launch_bounds__(kBlock) void plan_box_kernel(const double* const* _
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
       const T v = (T)(points[d][off + jc] * scale);
-      lo[d] = wave_min(v);
-      hi[d] = wave_max(v);
+      lo[d] = min_over_lanes(v, 1, 16);
+      hi[d] = max_over_lanes(v, 1, 16);
+    }
+    if ((lane & 15) == 0) {
+      double* b = sboxes + ((size_t)blockIdx.x * kSlabs + k / 16) * 6;
+      for (int d = 0; d < 3; ++d) {
+        b[d] = (double)lo[d];
+        b[3 + d] = (double)hi[d];
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      lo[d] = min_over_lanes(lo[d], 16, kWave);
+      hi[d] = max_over_lanes(hi[d], 16, kWave);
     }
     if (lane == 0) {
       double* b = boxes + ((size_t)blockIdx.x * kSymGroups + k / kWave) * 6;
@@ -770,15 +986,72 @@ __device__ __forceinline__ void load_box(const double* b, T* out) {
   for (int d = 0; d < 6; ++d) out[d] = (T)b[d];
 }
 
-// key[p] = (32 - kept blocks) << 32 | p, or kPlanDropped << 32 | p.
+// tinfo[t] = (centre of tile t's box, 1 if the tile is compact else 0).
 template <typename T, int DIM>
-__global__ __launch_bounds__(kBlock) void plan_pair_kernel(const double* __restrict__ boxes, int64_t np,
-                                                           double drop_s, uint64_t* __restrict__ keys) {
+__global__ __launch_bounds__(kBlock) void plan_tile_kernel(const double* __restrict__ boxes, int64_t nt,
+                                                           float4* __restrict__ tinfo) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= nt) return;
+  T lo[3], hi[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = (T)boxes[(size_t)t * kSymGroups * 6 + d];
+    hi[d] = (T)boxes[(size_t)t * kSymGroups * 6 + 3 + d];
+  }
+  for (int g = 1; g < kSymGroups; ++g) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = fmin(lo[d], (T)boxes[((size_t)t * kSymGroups + g) * 6 + d]);
+      hi[d] = fmax(hi[d], (T)boxes[((size_t)t * kSymGroups + g) * 6 + 3 + d]);
+    }
+  }
+  T r2 = T(0);
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    const T half = T(0.5) * (hi[d] - lo[d]);
+    r2 = fma(half, half, r2);
+  }
+  const bool compact = sizeof(T) == 4 && r2 <= (T)kCenterRadius2;
+  tinfo[t] = make_float4((float)(T(0.5) * (lo[0] + hi[0])), (float)(T(0.5) * (lo[1] + hi[1])),
+                         (float)(T(0.5) * (lo[2] + hi[2])), compact ? 1.0f : 0.0f);
+}
+
+// tmask[p][w][g] (one thread each).
+template <typename T, int DIM>
+__global__ __launch_bounds__(kBlock) void plan_tmask_kernel(const double* __restrict__ sboxes, int64_t np,
+                                                            double drop_s, uint32_t* __restrict__ tmask) {
+  const int64_t id = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (id >= np * kTmaskWords) return;
+  const int64_t p = id / kTmaskWords;
+  const int w = (int)(id % kTmaskWords) / kSymGroups, g = (int)(id % kSymGroups);
+  int64_t I, J;
+  decode_pair(p, I, J);
+  uint32_t bits = 0;
+  for (int b = 0; b < 4; ++b) {
+    T sb[6];
+    load_box(sboxes + ((size_t)J * kSlabs + 4 * g + b) * 6, sb);
+    for (int a = 0; a < 8; ++a) {
+      T tb[6];
+      load_box(sboxes + ((size_t)I * kSlabs + 8 * w + a) * 6, tb);
+      if (box_gap2<T, DIM>(tb, sb) <= (T)drop_s) bits |= 1u << (8 * b + a);
+    }
+  }
+  tmask[id] = bits;
+}
+
+// key[p] = class << 32 | p and bmask[p].
+template <typename T, int DIM>
+__global__ __launch_bounds__(kBlock) void plan_pair_kernel(const double* __restrict__ boxes,
+                                                           const float4* __restrict__ tinfo,
+                                                           const uint32_t* __restrict__ tmask, int64_t np,
+                                                           double drop_s, uint64_t* __restrict__ keys,
+                                                           uint32_t* __restrict__ bmask) {
   const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (p >= np) return;
   int64_t I, J;
   decode_pair(p, I, J);
   int kept = 0;
+  uint32_t bits = 0;
   for (int w = 0; w < kBlock / kWave; ++w) {
     T a[6], b[6], t[6];
     load_box(boxes + ((size_t)I * kSymGroups + 2 * w) * 6, a);
@@ -791,10 +1064,23 @@ __global__ __launch_bounds__(kBlock) void plan_pair_kernel(const double* __restr
     for (int g = 0; g < kSymGroups; ++g) {
       T gb[6];
       load_box(boxes + ((size_t)J * kSymGroups + g) * 6, gb);
-      kept += box_gap2<T, DIM>(t, gb) > (T)drop_s ? 0 : 1;
+      if (box_gap2<T, DIM>(t, gb) <= (T)drop_s) {
+        kept += 1;
+        bits |= 1u << (8 * w + g);
+      }
     }
   }
-  keys[p] = ((kept > 0 ? (uint64_t)(32 - kept) : kPlanDropped) << 32) | (uint64_t)p;
+  int cost;
+  if (tinfo[I].w != 0.0f) {
+    int tiles = 0;
+    for (int q = 0; q < kTmaskWords; ++q) tiles += __popc(tmask[p * kTmaskWords + q]);
+    cost = (tiles + 31) / 32;
+  } else {
+    cost = 3 * kept;
+  }
+  const uint64_t cls = cost == 0 ? kPlanDropped : (uint64_t)(96 - cost);
+  keys[p] = (cls << 32) | (uint64_t)p;
+  bmask[p] = bits;
 }
 
 __global__ __launch_bounds__(kBlock) void plan_zero_kernel(const uint64_t* __restrict__ keys, int64_t np,
@@ -808,6 +1094,7 @@ __global__ __launch_bounds__(kBlock) void plan_zero_kernel(const uint64_t* __res
   }
 }
 
+// list = pair indices in key order; *count = pairs with something kept.
 __global__ __launch_bounds__(kBlock) void plan_list_kernel(const uint64_t* __restrict__ keys, int64_t np,
                                                            uint32_t* __restrict__ list, int64_t* __restrict__ count) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -818,41 +1105,48 @@ __global__ __launch_bounds__(kBlock) void plan_list_kernel(const uint64_t* __res
   if (i == 0 && !kept) *count = 0;
 }
 
-// counts[0] = tile pairs with a kept block, counts[1] = kept 128 x 64 blocks,
-// counts[2] = kept blocks on diagonal pairs (I == J) (one workgroup, fixed order).
-__global__ __launch_bounds__(kBlock) void plan_count_kernel(const uint64_t* __restrict__ keys, int64_t np,
+// counts[0] = listed tile pairs, counts[1] = their kept 128 x 64 blocks,
+// counts[2] = kept blocks on diagonal pairs (I == J), counts[3] and
+// counts[4] = kernel entries the planned FP32 product evaluates off and on
+// the diagonal pairs (256 per kept 16 x 16 tile on compact target tiles,
+// 8192 per kept block on the others).  One workgroup, fixed order.
+__global__ __launch_bounds__(kBlock) void plan_count_kernel(const uint64_t* __restrict__ keys,
+                                                            const float4* __restrict__ tinfo,
+                                                            const uint32_t* __restrict__ bmask,
+                                                            const uint32_t* __restrict__ tmask, int64_t np,
                                                             int64_t* __restrict__ counts) {
-  __shared__ int64_t red[3][kBlock];
-  int64_t pairs = 0, blocks = 0, diag = 0;
+  __shared__ int64_t red[kPlanCounts][kBlock];
+  int64_t c[kPlanCounts] = {0, 0, 0, 0, 0};
   for (int64_t i = threadIdx.x; i < np; i += kBlock) {
-    const uint64_t c = keys[i] >> 32;
-    if (c != kPlanDropped) {
-      int64_t I, J;
-      decode_pair((int64_t)(keys[i] & 0xffffffffull), I, J);
-      pairs += 1;
-      blocks += 32 - (int64_t)c;
-      if (I == J) diag += 32 - (int64_t)c;
+    if ((keys[i] >> 32) == kPlanDropped) continue;
+    const int64_t p = (int64_t)(keys[i] & 0xffffffffull);
+    int64_t I, J;
+    decode_pair(p, I, J);
+    const int64_t blocks = __popc(bmask[p]);
+    int64_t evals = 8192 * blocks;
+    if (tinfo[I].w != 0.0f) {
+      evals = 0;
+      for (int q = 0; q < kTmaskWords; ++q) evals += 256 * __popc(tmask[p * kTmaskWords + q]);
     }
+    c[0] += 1;
+    c[1] += blocks;
+    c[2] += I == J ? blocks : 0;
+    c[I == J ? 4 : 3] += evals;
   }
-  red[0][threadIdx.x] = pairs;
-  red[1][threadIdx.x] = blocks;
-  red[2][threadIdx.x] = diag;
+  for (int q = 0; q < kPlanCounts; ++q) red[q][threadIdx.x] = c[q];
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int k = 1; k < kBlock; ++k) {
-      pairs += red[0][k];
-      blocks += red[1][k];
-      diag += red[2][k];
+    for (int q = 0; q < kPlanCounts; ++q) {
+      int64_t s = 0;
+      for (int k = 0; k < kBlock; ++k) s += red[q][k];
+      counts[q] = s;
     }
-    counts[0] = pairs;
-    counts[1] = blocks;
-    counts[2] = diag;
   }
 }
 
 // Partner lists of the finish: for tile X and slice k, the tiles P = k (mod 4)
-// in ascending order whose pair with X holds a kept block (one thread per
-// (tile, slice); keys are the unsorted per-pair keys).
+// in ascending order whose pair with X is listed (one thread per (tile,
+// slice); keys are the unsorted per-pair keys).
 __global__ __launch_bounds__(kBlock) void plan_partner_kernel(const uint64_t* __restrict__ keys, int64_t nt,
                                                               int64_t pcap, uint32_t* __restrict__ partners,
                                                               int32_t* __restrict__ pcount) {
@@ -868,28 +1162,35 @@ __global__ __launch_bounds__(kBlock) void plan_partner_kernel(const uint64_t* __
   pcount[id] = c;
 }
 
-template <typename T>
-void launch_plan_geometry(hipStream_t st, const mplak_kernel_desc* d, double scale, double drop_s, double* boxes,
-                          uint64_t* keys) {
+struct PlanPtrs {
+  double *boxes, *sboxes;
+  float4* tinfo;
+  uint32_t *bmask, *tmask;
+  uint64_t* keys;
+};
+
+template <typename T, int DIM>
+void launch_plan_geometry_dim(hipStream_t st, const mplak_kernel_desc* d, double scale, double drop_s,
+                              const PlanPtrs& q) {
   const int64_t nt = sym_tiles(d->row_count);
   const int64_t np = nt * (nt + 1) / 2;
-  const unsigned pg = (unsigned)((np + kBlock - 1) / kBlock);
+  auto groups = [](int64_t threads) { return dim3((unsigned)((threads + kBlock - 1) / kBlock)); };
+  hipLaunchKernelGGL((plan_box_kernel<T, DIM>), dim3((unsigned)nt), dim3(kBlock), 0, st, d->points, d->row_offset,
+                     d->row_count, scale, q.boxes, q.sboxes);
+  hipLaunchKernelGGL((plan_tile_kernel<T, DIM>), groups(nt), dim3(kBlock), 0, st, q.boxes, nt, q.tinfo);
+  hipLaunchKernelGGL((plan_tmask_kernel<T, DIM>), groups(np * kTmaskWords), dim3(kBlock), 0, st, q.sboxes, np,
+                     drop_s, q.tmask);
+  hipLaunchKernelGGL((plan_pair_kernel<T, DIM>), groups(np), dim3(kBlock), 0, st, q.boxes, q.tinfo, q.tmask, np,
+                     drop_s, q.keys, q.bmask);
+}
+
+template <typename T>
+void launch_plan_geometry(hipStream_t st, const mplak_kernel_desc* d, double scale, double drop_s,
+                          const PlanPtrs& q) {
   switch (d->dim) {
-    case 1:
-      hipLaunchKernelGGL((plan_box_kernel<T, 1>), dim3((unsigned)nt), dim3(kBlock), 0, st, d->points, d->row_offset,
-                         d->row_count, scale, boxes);
-      hipLaunchKernelGGL((plan_pair_kernel<T, 1>), dim3(pg), dim3(kBlock), 0, st, boxes, np, drop_s, keys);
-      break;
-    case 2:
-      hipLaunchKernelGGL((plan_box_kernel<T, 2>), dim3((unsigned)nt), dim3(kBlock), 0, st, d->points, d->row_offset,
-                         d->row_count, scale, boxes);
-      hipLaunchKernelGGL((plan_pair_kernel<T, 2>), dim3(pg), dim3(kBlock), 0, st, boxes, np, drop_s, keys);
-      break;
-    default:
-      hipLaunchKernelGGL((plan_box_kernel<T, 3>), dim3((unsigned)nt), dim3(kBlock), 0, st, d->points, d->row_offset,
-                         d->row_count, scale, boxes);
-      hipLaunchKernelGGL((plan_pair_kernel<T, 3>), dim3(pg), dim3(kBlock), 0, st, boxes, np, drop_s, keys);
-      break;
+    case 1: launch_plan_geometry_dim<T, 1>(st, d, scale, drop_s, q); break;
+    case 2: launch_plan_geometry_dim<T, 2>(st, d, scale, drop_s, q); break;
+    default: launch_plan_geometry_dim<T, 3>(st, d, scale, drop_s, q); break;
   }
 }
 
@@ -934,8 +1235,8 @@ int mplak_kernel_matvec_screened(const mplak_kernel_desc* d, int precision, doub
   if (d->row_count == 0) return MPLAK_OK;
   if (x == nullptr || y == nullptr) return MPLAK_ERR_ARG;
   if (work == nullptr || work_bytes < sym_workspace_bytes(d->row_count)) return MPLAK_ERR_WORKSPACE;
-  return run_sym(d, precision, kSymScreened, drop_distance2(d->kind, drop_log2), nullptr, nullptr,
-                 nullptr, nullptr, 0, x, y, work, skip_if, s);
+  return run_sym(d, precision, kSymScreened, drop_distance2(d->kind, drop_log2), nullptr, nullptr, nullptr, nullptr,
+                 nullptr, nullptr, nullptr, 0, x, y, work, skip_if, s);
 }
 
 size_t mplak_screen_plan_bytes(const mplak_kernel_desc* d) {
@@ -964,10 +1265,17 @@ int mplak_screen_plan_build(const mplak_kernel_desc* d, int precision, double dr
   const double scale = coord_scale(d->kind, d->length_scale);
   const double drop_s = drop_distance2(d->kind, drop_log2);
   hipStream_t hs = as_stream(s);
+  PlanPtrs q;
+  q.boxes = boxes;
+  q.sboxes = reinterpret_cast<double*>(base + L.sboxes);
+  q.tinfo = reinterpret_cast<float4*>(base + L.tinfo);
+  q.bmask = reinterpret_cast<uint32_t*>(base + L.bmask);
+  q.tmask = reinterpret_cast<uint32_t*>(base + L.tmask);
+  q.keys = kin;
   if (precision == MPLAK_PREC_F64)
-    launch_plan_geometry<double>(hs, d, scale, drop_s, boxes, kin);
+    launch_plan_geometry<double>(hs, d, scale, drop_s, q);
   else
-    launch_plan_geometry<float>(hs, d, scale, drop_s, boxes, kin);
+    launch_plan_geometry<float>(hs, d, scale, drop_s, q);
   st = launch_status();
   if (st != MPLAK_OK) return st;
   hipLaunchKernelGGL(plan_zero_kernel, dim3((unsigned)std::min<int64_t>(np, 4096)), dim3(kBlock), 0, hs, kin, np, pi,
@@ -980,7 +1288,7 @@ int mplak_screen_plan_build(const mplak_kernel_desc* d, int precision, double dr
   st = launch_status();
   if (st != MPLAK_OK) return st;
   size_t tb = L.tmp_bytes;
-  if (rocprim::radix_sort_keys(base + L.tmp, tb, kin, kout, (unsigned)np, 32, 38, hs) != hipSuccess)
+  if (rocprim::radix_sort_keys(base + L.tmp, tb, kin, kout, (unsigned)np, 32, 32 + kPlanKeyBits, hs) != hipSuccess)
     return MPLAK_ERR_HIP;
   hipLaunchKernelGGL(plan_list_kernel, dim3((unsigned)((np + kBlock - 1) / kBlock)), dim3(kBlock), 0, hs, kout, np,
                      reinterpret_cast<uint32_t*>(base + L.list), reinterpret_cast<int64_t*>(base + L.count));
@@ -997,7 +1305,9 @@ int mplak_screen_plan_counts(const mplak_kernel_desc* d, const void* plan, size_
   const int64_t nt = sym_tiles(d->row_count);
   const char* base = static_cast<const char*>(plan);
   hipLaunchKernelGGL(plan_count_kernel, dim3(1), dim3(kBlock), 0, as_stream(s),
-                     reinterpret_cast<const uint64_t*>(base + L.keys_out), nt * (nt + 1) / 2, counts);
+                     reinterpret_cast<const uint64_t*>(base + L.keys_out),
+                     reinterpret_cast<const float4*>(base + L.tinfo), reinterpret_cast<const uint32_t*>(base + L.bmask),
+                     reinterpret_cast<const uint32_t*>(base + L.tmask), nt * (nt + 1) / 2, counts);
   return launch_status();
 }
 
@@ -1016,7 +1326,8 @@ int mplak_kernel_matvec_planned(const mplak_kernel_desc* d, int precision, doubl
   char* base = static_cast<char*>(plan);
   return run_sym(d, precision, kSymPlanned, drop_distance2(d->kind, drop_log2),
                  reinterpret_cast<const uint32_t*>(base + L.list), reinterpret_cast<const int64_t*>(base + L.count),
-                 reinterpret_cast<const uint32_t*>(base + L.partners), reinterpret_cast<const int32_t*>(base + L.pcount),
+                 reinterpret_cast<const float4*>(base + L.tinfo), reinterpret_cast<const uint32_t*>(base + L.bmask),
+                 reinterpret_cast<const uint32_t*>(base + L.tmask), reinterpret_cast<const uint32_t*>(base + L.partners), reinterpret_cast<const int32_t*>(base + L.pcount),
                  L.pcap, x, y, base + L.part, skip_if, s);
 }
 
@@ -1032,7 +1343,8 @@ int mplak_kernel_matvec(const mplak_kernel_desc* d, int precision, const double*
   const size_t need = mplak_kernel_matvec_workspace_bytes(d);
   if (work == nullptr || work_bytes < need) return MPLAK_ERR_WORKSPACE;
   if (use_symmetric(d))
-    return run_sym(d, precision, kSymExact, 0.0, nullptr, nullptr, nullptr, nullptr, 0, x, y, work, skip_if, s);
+    return run_sym(d, precision, kSymExact, 0.0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0, x,
+                   y, work, skip_if, s);
 
   const double scale = coord_scale(d->kind, d->length_scale);
   const bool coinc = d->coincident_shift != 0.0;

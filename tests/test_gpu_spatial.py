@@ -6,8 +6,12 @@ the screened product with nothing to drop is bitwise the exact symmetric
 product; with a drop level of 2^-30 on ordered points every row differs from
 the oracle's exact FP64 product by at most the dropped mass bound
 n * 2^-30 * max|x| plus the FP32 product tolerance; the planned product
-(screening plan built once, all-dropped tile pairs never launched) is bitwise
-the screened product, also when the plan is reused for new vectors.
+(screening plan built once, all-dropped tile pairs never launched) is, in
+FP64, bitwise the screened product, also when the plan is reused for new
+vectors.  In FP32 the planned product takes the squared distances of compact
+tiles from the matrix cores (tile-centred form), so it agrees with the
+screened product row by row within 2e-5 of sum_j |K_ij x_j| and with the FP64
+oracle within the same bar, and it is bitwise reproducible.
 """
 import ctypes as C
 import math
@@ -119,10 +123,10 @@ def ordered_points(lib, oracle, n, dim, seed):
 @pytest.mark.parametrize("kind,l", [(0, 0.05), (1, 0.1)])
 @pytest.mark.parametrize("ordered", [True, False])
 def test_screened_without_drops_is_the_exact_symmetric_product(lib, oracle, monkeypatch, kind, l, ordered):
-    """FP64: bitwise the exact symmetric product.  FP32: on compact (ordered)
-    tiles the screened kernel uses the tile-centred distance form, so it
-    agrees with the exact FP32 product to FP32 accuracy and with the FP64
-    oracle to the FP32 product bar; on unordered points it is bitwise."""
+    """Screened with nothing to drop: bitwise the exact symmetric product in
+    both precisions.  Planned FP32 with nothing to drop (matrix-core form on
+    the compact tiles of ordered points): within the FP32 bar of the FP64
+    oracle, row by row."""
     monkeypatch.setenv("MPLAK_KMV_SYMMETRIC", "1")
     n = 3000
     pts = ordered_points(lib, oracle, n, 3, 51) if ordered else oracle.uniform(3 * n, 51, 0).reshape(3, n)
@@ -134,14 +138,14 @@ def test_screened_without_drops_is_the_exact_symmetric_product(lib, oracle, monk
     assert np.array_equal(screened(lib, d, X, 0, 2000.0), host(kmv_gpu(lib, d, X, precision=0)))
     exact32 = host(kmv_gpu(lib, d, X, precision=1))
     y32 = screened(lib, d, X, 1, 2000.0)
+    assert np.array_equal(y32, exact32)
     if ordered:
-        assert rel_err(y32, exact32) <= 3e-6
+        plan, wb = build_plan(lib, d, 1, 2000.0)
+        yp = planned(lib, d, 1, 2000.0, plan, wb, X)
         k = oracle.Kernel(pts, kind, l, nugget=0.5)
         ref = k.matvec(x, mode=1)
-        assert rel_err(y32, ref) <= 2e-5
-        assert np.all(np.abs(y32 - ref) <= 2e-5 * k.matvec(np.abs(x), mode=1))
-    else:
-        assert np.array_equal(y32, exact32)
+        assert rel_err(yp, ref) <= 2e-5
+        assert np.all(np.abs(yp - ref) <= 2e-5 * k.matvec(np.abs(x), mode=1))
 
 
 @pytest.mark.parametrize("kind,l,dim", [(0, 0.05, 3), (0, 0.02, 2), (1, 0.05, 3), (0, 0.01, 1)])
@@ -199,7 +203,7 @@ def planned(lib, desc, prec, drop, plan, wb, X):
 @pytest.mark.parametrize("kind,l,dim,n", [(0, 0.05, 3, 20000), (1, 0.1, 3, 7000), (0, 0.02, 2, 9000),
                                           (0, 0.01, 1, 5000), (0, 0.05, 3, 511)])
 @pytest.mark.parametrize("drop", [20.0, 30.0])
-def test_planned_product_equals_screened_bitwise(lib, oracle, kind, l, dim, n, drop):
+def test_planned_product_matches_screened(lib, oracle, kind, l, dim, n, drop):
     pts = ordered_points(lib, oracle, n, dim, 70 + dim)
     P = dev(pts)
     T = point_table(P)
@@ -208,8 +212,40 @@ def test_planned_product_equals_screened_bitwise(lib, oracle, kind, l, dim, n, d
         plan, wb = build_plan(lib, d, prec, drop)
         # the plan is reused across products with different vectors
         for seed in (1, 2):
-            X = dev(oracle.uniform(n, 71, seed, -1, 1))
-            assert np.array_equal(planned(lib, d, prec, drop, plan, wb, X), screened(lib, d, X, prec, drop))
+            x = oracle.uniform(n, 71, seed, -1, 1)
+            X = dev(x)
+            yp = planned(lib, d, prec, drop, plan, wb, X)
+            ys = screened(lib, d, X, prec, drop)
+            if prec == 0:
+                assert np.array_equal(yp, ys)
+            else:
+                mass = screened(lib, d, dev(np.abs(x)), 0, drop)
+                assert np.all(np.abs(yp - ys) <= 2e-5 * mass)
+                assert np.array_equal(planned(lib, d, prec, drop, plan, wb, X), yp)
+
+
+@pytest.mark.parametrize("kind,l", [(0, 0.05), (1, 0.1)])
+def test_planned_fp32_matrix_core_form_against_oracle(lib, oracle, kind, l):
+    """20000 ordered points: every target tile is compact, so the planned FP32
+    product takes all its tile pairs on the matrix cores.  With nothing
+    dropped, sampled rows are within 2e-5 of sum_j |K_ij x_j| of the FP64
+    oracle; the result is not bitwise the vector-unit screened product (the
+    other arithmetic path)."""
+    n = 20000
+    pts = ordered_points(lib, oracle, n, 3, 80 + kind)
+    x = oracle.uniform(n, 81, 1, -1, 1)
+    P = dev(pts)
+    T = point_table(P)
+    d = kernel_desc(P, T, kind, l, nugget=0.5)
+    X = dev(x)
+    plan, wb = build_plan(lib, d, 1, 2000.0)
+    yp = planned(lib, d, 1, 2000.0, plan, wb, X)
+    assert not np.array_equal(yp, screened(lib, d, X, 1, 2000.0))
+    k = oracle.Kernel(pts, kind, l, nugget=0.5)
+    rows = np.arange(0, n, 61)
+    ref = np.array([k.matvec(x, mode=1, row_off=int(i), rows=1)[0] for i in rows])
+    mass = np.array([k.matvec(np.abs(x), mode=1, row_off=int(i), rows=1)[0] for i in rows])
+    assert np.all(np.abs(yp[rows] - ref) <= 2e-5 * mass)
 
 
 def test_planned_product_unordered_points_and_errors(lib, oracle):
@@ -229,6 +265,56 @@ def test_planned_product_unordered_points_and_errors(lib, oracle):
     assert lib.mplak_screen_plan_bytes(C.byref(rect)) == 0
 
 
+def plan_counts_numpy(pts, l, drop):
+    """The plan's screen restated in numpy on the same FP32-cast scaled
+    coordinates (Gaussian): boxes of 16-point slabs, 64-point groups and
+    128-point wave target sets per 512-point tile (padding clamped to the last
+    point), squared box gaps with FP32 rounding (fma emulated in FP64, exact
+    for these products), compact tiles (half-diagonal^2 <= 49), and per tile
+    pair I <= J the kept 128 x 64 blocks and kept 16 x 16 tiles.  Returns the
+    five plan counts."""
+    dim, n = pts.shape
+    scale = math.sqrt(math.log2(math.e)) / (math.sqrt(2.0) * l)
+    f = np.float32
+    nt = (n + 511) // 512
+    P = (pts * scale).astype(f)[:, np.minimum(np.arange(nt * 512), n - 1)]
+
+    def boxes(size):
+        Q = P.reshape(dim, -1, size)
+        return Q.min(axis=2).T, Q.max(axis=2).T  # (count, dim)
+
+    def gap2(alo, ahi, blo, bhi):
+        g = np.maximum(f(0), np.maximum(blo - ahi, alo - bhi)).astype(f)
+        d2 = (g[..., 0].astype(np.float64) ** 2).astype(f)
+        for d in range(1, dim):
+            d2 = (g[..., d].astype(np.float64) ** 2 + d2.astype(np.float64)).astype(f)
+        return d2
+
+    slo, shi = boxes(16)
+    glo, ghi = boxes(64)
+    wlo, whi = boxes(128)
+    tlo, thi = boxes(512)
+    half = (f(0.5) * (thi - tlo)).astype(f)
+    r2 = (half[:, 0].astype(np.float64) ** 2).astype(f)
+    for d in range(1, dim):
+        r2 = (half[:, d].astype(np.float64) ** 2 + r2.astype(np.float64)).astype(f)
+    compact = r2 <= f(49.0)
+    dt = gap2(slo[:, None, :], shi[:, None, :], slo[None, :, :], shi[None, :, :]) <= f(drop)  # slab x slab
+    db = gap2(wlo[:, None, :], whi[:, None, :], glo[None, :, :], ghi[None, :, :]) <= f(drop)  # wave x group
+    c = [0, 0, 0, 0, 0]
+    for J in range(nt):
+        for I in range(J + 1):
+            blocks = int(db[4 * I:4 * I + 4, 8 * J:8 * J + 8].sum())
+            tiles = int(dt[32 * I:32 * I + 32, 32 * J:32 * J + 32].sum())
+            if (tiles if compact[I] else blocks) == 0:
+                continue
+            c[0] += 1
+            c[1] += blocks
+            c[2] += blocks if I == J else 0
+            c[4 if I == J else 3] += 256 * tiles if compact[I] else 8192 * blocks
+    return c
+
+
 def test_plan_counts(lib, oracle):
     import torch
     n = 3000  # 6 tiles, 21 pairs
@@ -236,15 +322,18 @@ def test_plan_counts(lib, oracle):
     P = dev(pts)
     T = point_table(P)
     d = kernel_desc(P, T, 0, 0.05, nugget=1.0)
-    counts = torch.zeros(3, dtype=torch.int64, device="cuda")
-    plan, wb = build_plan(lib, d, 1, 20.0)  # unordered points: every block is kept
+    counts = torch.zeros(5, dtype=torch.int64, device="cuda")
+    plan, wb = build_plan(lib, d, 1, 20.0)  # unordered points: every block is kept, no tile is compact
     ck(lib.mplak_screen_plan_counts(C.byref(d), plan.data_ptr(), wb, counts.data_ptr(), stream()))
-    assert host(counts).tolist() == [21, 21 * 32, 6 * 32]
-    po = ordered_points(lib, oracle, 20000, 3, 5)
-    Po = dev(po)
-    To = point_table(Po)
-    do = kernel_desc(Po, To, 0, 0.05, nugget=1.0)
-    plan, wb = build_plan(lib, do, 1, 20.0)
-    ck(lib.mplak_screen_plan_counts(C.byref(do), plan.data_ptr(), wb, counts.data_ptr(), stream()))
-    pairs, blocks, diag = host(counts).tolist()
-    assert 40 <= pairs < 40 * 41 // 2 and pairs * 1 <= blocks < 32 * pairs and diag <= 40 * 32
+    assert host(counts).tolist() == [21, 21 * 32, 6 * 32, 8192 * 15 * 32, 8192 * 6 * 32]
+    for dim, l, n in ((3, 0.05, 20000), (2, 0.02, 9000), (1, 0.01, 5000)):
+        po = ordered_points(lib, oracle, n, dim, 5)
+        Po = dev(po)
+        To = point_table(Po)
+        do = kernel_desc(Po, To, 0, l, nugget=1.0)
+        plan, wb = build_plan(lib, do, 1, 20.0)
+        ck(lib.mplak_screen_plan_counts(C.byref(do), plan.data_ptr(), wb, counts.data_ptr(), stream()))
+        got = host(counts).tolist()
+        assert got == plan_counts_numpy(po, l, 20.0), (dim, got)
+        nt = (n + 511) // 512
+        assert nt <= got[0] < nt * (nt + 1) // 2 and got[3] + got[4] < 8192 * got[1]
