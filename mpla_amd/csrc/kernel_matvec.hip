@@ -296,11 +296,11 @@ enum SymMode { kSymExact = 0, kSymScreened = 1, kSymPlanned = 2 };
 template <typename T>
 using SymSrc = typename Quad<T>::type[(sizeof(T) == 4 ? 2 : 1) * kWave];
 
-template <typename T, int KIND, int DIM, bool COINC, bool SCREEN, bool MASKED>
+template <typename T, int KIND, int DIM, bool COINC, bool SCREEN, bool MASKED, typename P>
 __device__ __forceinline__ void sym_valu_pair(const double* const* __restrict__ points, int64_t off, int64_t n,
                                               double scale, double coinc_thr, double coinc_shift, double drop_s,
-                                              const double* __restrict__ x, double* __restrict__ part_i,
-                                              double* __restrict__ part_j, int64_t p, uint32_t pmask,
+                                              const double* __restrict__ x, P* __restrict__ part_i,
+                                              P* __restrict__ part_j, int64_t p, uint32_t pmask,
                                               SymSrc<T>* src, T (*gbox)[6], T* own_red, const double* exp_tab) {
   typedef typename Quad<T>::type Q;
   constexpr int kDup = sizeof(T) == 4 ? 2 : 1;  // copies of each 64-source group
@@ -434,7 +434,7 @@ __device__ __forceinline__ void sym_valu_pair(const double* const* __restrict__ 
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     const int k = w * 2 * kWave + h * kWave + lane;
-    part_i[p * kSymTile + k] = (double)accI[h];
+    part_i[p * kSymTile + k] = (P)accI[h];
   }
   if (I != J) {
     __syncthreads();
@@ -442,7 +442,7 @@ __device__ __forceinline__ void sym_valu_pair(const double* const* __restrict__ 
       double s = (double)red[k];
 #pragma unroll
       for (int q = 1; q < kBlock / kWave; ++q) s += (double)red[q * kSymTile + k];
-      part_j[p * kSymTile + k] = s;
+      part_j[p * kSymTile + k] = (P)s;
     }
   }
 }
@@ -471,7 +471,7 @@ __global__ __launch_bounds__(kBlock) void kmv_sym_kernel(
   __shared__ double exp_tab[Exp2Lds<T>::size];
   if (sizeof(T) == 8) fill_exp2_table(exp_tab);  // first read after the staging barrier
   const int64_t p = MODE == kSymPlanned ? (int64_t)list[blockIdx.x] : (int64_t)blockIdx.x;
-  sym_valu_pair<T, KIND, DIM, COINC, MODE != kSymExact, false>(
+  sym_valu_pair<T, KIND, DIM, COINC, MODE != kSymExact, false, double>(
       points, off, n, scale, coinc_thr, coinc_shift, drop_s, x, part_i, part_j, p, 0u,
       reinterpret_cast<SymSrc<T>*>(&src[0][0]), gbox, own_red, exp_tab);
 }
@@ -518,22 +518,31 @@ __device__ __forceinline__ float row16_sum(float v) {
 }
 
 // LDS layout of the matrix-core form (floats): operands [32 slabs][64 lanes]
-// of the targets (lane 16 k + c: coordinate k of target 16 slab + c, k < 3,
-// and its norm at k = 3) and of the sources (-2 x coordinate, and 1), then
-// the source norms, source values and target values.
+// of the targets (lane 16 k + c: coordinate k of target 16 slab + c for
+// k < 3, at k = 3 its norm (Matern) or 1 (Gaussian)) and of the sources
+// (-2 x coordinate k; at k = 3: 1 (Matern) or the source norm (Gaussian)),
+// then source norms (Matern), source values, target values (Gaussian: times
+// 2^-n_p) and, Gaussian, the target row scales 2^-n_p.
 constexpr int kMfOp = kSlabs * kWave;
 constexpr int kMfB = kMfOp;
 constexpr int kMfNorm = 2 * kMfOp;
 constexpr int kMfXs = kMfNorm + kSymTile;
 constexpr int kMfXt = kMfXs + kSymTile;
-constexpr int kMfFloats = kMfXt + kSymTile;
+constexpr int kMfScale = kMfXt + kSymTile;
+constexpr int kMfFloats = kMfScale + kSymTile;
 
-template <int KIND, int DIM>
+// Gaussian entries factor as 2^-s = 2^-n_p 2^-t with t = n_q - 2 p.q, so the
+// matrix cores give t with no C operand (A = [p, 1], B = [-2 q, n_q]); the
+// factor 2^-n_p goes into the target values (column side) and into the row
+// sums once per target.  t >= -n_p >= -kCenterRadius2 on compact tiles, so
+// 2^-t stays in range.  Matern entries are not a product of that form and
+// take s = t + n_p with C = n_q.
+template <int KIND, int DIM, typename P>
 __device__ __forceinline__ void sym_mfma_pair(const double* const* __restrict__ points, int64_t off, int64_t n,
-                                              double scale, const double* __restrict__ x,
-                                              double* __restrict__ part_i, double* __restrict__ part_j, int64_t p,
-                                              int64_t I, int64_t J, float4 centre,
-                                              const uint32_t* __restrict__ tmask, float* mf) {
+                                              double scale, const double* __restrict__ x, P* __restrict__ part_i,
+                                              P* __restrict__ part_j, int64_t p, int64_t I, int64_t J,
+                                              float4 centre, const uint32_t* __restrict__ tmask, float* mf) {
+  constexpr bool FACTOR = KIND == MPLAK_KERNEL_GAUSSIAN;
   const int tid = threadIdx.x, lane = tid & (kWave - 1);
   const int w = __builtin_amdgcn_readfirstlane(tid / kWave);
   const int c16 = lane & 15, r4 = lane >> 4;
@@ -555,16 +564,18 @@ __device__ __forceinline__ void sym_mfma_pair(const double* const* __restrict__ 
       mf[kMfB + o] = -2.0f * c[0];
       mf[kMfB + o + 16] = -2.0f * c[1];
       mf[kMfB + o + 32] = -2.0f * c[2];
-      mf[kMfB + o + 48] = 1.0f;
-      mf[kMfNorm + k] = nc;
+      mf[kMfB + o + 48] = FACTOR ? nc : 1.0f;
+      if (!FACTOR) mf[kMfNorm + k] = nc;
       mf[kMfXs + k] = xv;
     }
     if (target) {
       mf[o] = c[0];
       mf[o + 16] = c[1];
       mf[o + 32] = c[2];
-      mf[o + 48] = nc;
-      mf[kMfXt + k] = xv;
+      mf[o + 48] = FACTOR ? 1.0f : nc;
+      const float sc = FACTOR ? __builtin_amdgcn_exp2f(-nc) : 1.0f;
+      mf[kMfXt + k] = xv * sc;
+      if (FACTOR) mf[kMfScale + k] = sc;
     }
   };
   for (int k = tid; k < kSymTile; k += kBlock) {
@@ -575,20 +586,21 @@ __device__ __forceinline__ void sym_mfma_pair(const double* const* __restrict__ 
       stage(I, k, false, true);
     }
   }
+  // the wave's 8 group masks (scalar loads, before the barrier)
+  uint32_t gmask[kSymGroups];
   const uint32_t* tm = tmask + p * kTmaskWords + w * kSymGroups;
+#pragma unroll
+  for (int g = 0; g < kSymGroups; ++g) gmask[g] = (uint32_t)__builtin_amdgcn_readfirstlane((int)tm[g]);
   __syncthreads();
 
-  float A[8], xm[8][4], accR[8][4];
+  float A[8], accR[8][4];
 #pragma unroll
   for (int a = 0; a < 8; ++a) {
     A[a] = mf[(w * 8 + a) * kWave + lane];
-    const f32x4 xv = *reinterpret_cast<const f32x4*>(&mf[kMfXt + w * 2 * kWave + 16 * a + 4 * r4]);
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      xm[a][v] = xv[v];
-      accR[a][v] = 0.0f;
-    }
+    for (int v = 0; v < 4; ++v) accR[a][v] = 0.0f;
   }
+  const float* xt = &mf[kMfXt + w * 2 * kWave + 4 * r4];  // xt[16 a + v]: target 16 a + 4 r + v
   auto mfma_group = [&](int g, uint32_t gm, auto diagonal) -> float {
     constexpr bool DIAG = decltype(diagonal)::value;
     float accC[4] = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -597,12 +609,16 @@ __device__ __forceinline__ void sym_mfma_pair(const double* const* __restrict__ 
       const uint32_t bm = (gm >> (8 * b)) & 0xffu;
       if (bm == 0) continue;
       const float bv = mf[kMfB + (g * 4 + b) * kWave + lane];
-      const float nq = mf[kMfNorm + g * kWave + 16 * b + c16];
       const float xn = mf[kMfXs + g * kWave + 16 * b + c16];
-      const f32x4 cq = {nq, nq, nq, nq};
+      f32x4 cq = {0.0f, 0.0f, 0.0f, 0.0f};
+      if (!FACTOR) {
+        const float nq = mf[kMfNorm + g * kWave + 16 * b + c16];
+        cq = f32x4{nq, nq, nq, nq};
+      }
 #pragma unroll
       for (int a = 0; a < 8; ++a) {
         if (((bm >> a) & 1u) == 0) continue;
+        const f32x4 xm = DIAG ? f32x4{0.0f, 0.0f, 0.0f, 0.0f} : *reinterpret_cast<const f32x4*>(&xt[16 * a]);
         const f32x4 s = __builtin_amdgcn_mfma_f32_16x16x4f32(A[a], bv, cq, 0, 0, 0);
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
@@ -610,7 +626,7 @@ __device__ __forceinline__ void sym_mfma_pair(const double* const* __restrict__ 
           if (KIND == MPLAK_KERNEL_MATERN32) sv = fmaxf(sv, 0.0f);  // sqrt of a rounding-negative s
           const float e = kernel_entry<float, KIND>(sv, nullptr);
           accR[a][v] = fmaf(e, xn, accR[a][v]);
-          if (!DIAG) accC[b] = fmaf(e, xm[a][v], accC[b]);
+          if (!DIAG) accC[b] = fmaf(e, xm[v], accC[b]);
         }
       }
     }
@@ -629,16 +645,24 @@ __device__ __forceinline__ void sym_mfma_pair(const double* const* __restrict__ 
   float tg[kSymGroups];
 #pragma unroll
   for (int g = 0; g < kSymGroups; ++g) tg[g] = 0.0f;
+  // group g's mask: the uniform select over gmask keeps it in scalar registers
+  auto group_mask = [&](int g) -> uint32_t {
+    uint32_t m = 0;
+#pragma unroll
+    for (int q = 0; q < kSymGroups; ++q)
+      if (q == g) m = gmask[q];
+    return m;
+  };
   if (I == J) {
 #pragma unroll 1
     for (int g = 0; g < kSymGroups; ++g) {
-      const uint32_t gm = (uint32_t)__builtin_amdgcn_readfirstlane((int)tm[g]);
+      const uint32_t gm = group_mask(g);
       if (gm != 0) mfma_group(g, gm, std::true_type{});
     }
   } else {
 #pragma unroll 1
     for (int g = 0; g < kSymGroups; ++g) {
-      const uint32_t gm = (uint32_t)__builtin_amdgcn_readfirstlane((int)tm[g]);
+      const uint32_t gm = group_mask(g);
       if (gm == 0) continue;
       const float v = mfma_group(g, gm, std::false_type{});
 #pragma unroll
@@ -661,7 +685,8 @@ __device__ __forceinline__ void sym_mfma_pair(const double* const* __restrict__ 
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     const int k = w * 2 * kWave + 64 * h + 16 * (c16 >> 2) + 4 * r4 + (c16 & 3);
-    part_i[p * kSymTile + k] = (double)out[h];
+    const float v = FACTOR ? out[h] * mf[kMfScale + k] : out[h];
+    part_i[p * kSymTile + k] = (P)v;
   }
   if (I != J) {
     __syncthreads();  // every wave is done reading the staged tiles
@@ -672,7 +697,7 @@ __device__ __forceinline__ void sym_mfma_pair(const double* const* __restrict__ 
       double s = (double)mf[k];
 #pragma unroll
       for (int q = 1; q < kBlock / kWave; ++q) s += (double)mf[q * kSymTile + k];
-      part_j[p * kSymTile + k] = s;
+      part_j[p * kSymTile + k] = (P)s;
     }
   }
 }
@@ -680,7 +705,7 @@ __device__ __forceinline__ void sym_mfma_pair(const double* const* __restrict__ 
 template <int KIND, int DIM>
 __global__ __launch_bounds__(kBlock) void kmv_sym_planned_kernel(
     const double* const* __restrict__ points, int64_t off, int64_t n, double scale, const double* __restrict__ x,
-    double* __restrict__ part_i, double* __restrict__ part_j, const uint32_t* __restrict__ list,
+    float* __restrict__ part_i, float* __restrict__ part_j, const uint32_t* __restrict__ list,
     const int64_t* __restrict__ count, const float4* __restrict__ tinfo, const uint32_t* __restrict__ bmask,
     const uint32_t* __restrict__ tmask, const int32_t* __restrict__ skip) {
   if (skip != nullptr && *skip != 0) return;
@@ -693,9 +718,9 @@ __global__ __launch_bounds__(kBlock) void kmv_sym_planned_kernel(
   decode_pair(p, I, J);
   const float4 centre = tinfo[I];
   if (centre.w != 0.0f) {
-    sym_mfma_pair<KIND, DIM>(points, off, n, scale, x, part_i, part_j, p, I, J, centre, tmask, mf);
+    sym_mfma_pair<KIND, DIM, float>(points, off, n, scale, x, part_i, part_j, p, I, J, centre, tmask, mf);
   } else {
-    sym_valu_pair<float, KIND, DIM, false, true, true>(points, off, n, scale, 0.0, 0.0, 0.0, x, part_i, part_j, p,
+    sym_valu_pair<float, KIND, DIM, false, true, true, float>(points, off, n, scale, 0.0, 0.0, 0.0, x, part_i, part_j, p,
                                                        bmask[p], reinterpret_cast<SymSrc<float>*>(mf), nullptr,
                                                        nullptr, nullptr);
   }
@@ -707,18 +732,22 @@ __global__ __launch_bounds__(kBlock) void kmv_sym_planned_kernel(
 // order, then the four slice sums are added in slice order.  With a partner
 // list (planned product) a wave visits only the listed partners of its slice,
 // in the same order; the partials it leaves out are zero, so the result is
-// the same.
+// the same.  The wave reads its list with one load per lane and takes the
+// partners from it with readlane, four independent partial loads at a time.
 constexpr int kFinishRows = kWave;
 constexpr int kFinishSlices = kBlock / kWave;
 
-__device__ __forceinline__ double sym_partial(const double* __restrict__ part_i, const double* __restrict__ part_j,
+template <typename PT>
+__device__ __forceinline__ double sym_partial(const PT* __restrict__ part_i, const PT* __restrict__ part_j,
                                               int64_t X, int64_t P, int64_t t) {
-  return P < X ? part_j[sym_pair_index(P, X) * kSymTile + t] : part_i[sym_pair_index(X, P) * kSymTile + t];
+  return P < X ? (double)part_j[sym_pair_index(P, X) * kSymTile + t] : (double)part_i[sym_pair_index(X, P) * kSymTile + t];
 }
 
-template <bool LIST>
+// Partials in FP64, or in FP32 where the tile sums are FP32 sums anyway (the
+// planned FP32 kernel); the finish adds them in FP64 either way.
+template <bool LIST, typename PT>
 __global__ __launch_bounds__(kBlock) void kmv_sym_finish_kernel(
-    int64_t n, int64_t ntiles, const double* __restrict__ part_i, const double* __restrict__ part_j, double nugget,
+    int64_t n, int64_t ntiles, const PT* __restrict__ part_i, const PT* __restrict__ part_j, double nugget,
     const double* __restrict__ x, double* __restrict__ y, const uint32_t* __restrict__ partners,
     const int32_t* __restrict__ pcount, int64_t pcap, const int32_t* __restrict__ skip) {
   if (skip != nullptr && *skip != 0) return;
@@ -726,16 +755,29 @@ __global__ __launch_bounds__(kBlock) void kmv_sym_finish_kernel(
   const int r = threadIdx.x % kFinishRows, k = threadIdx.x / kFinishRows;
   for (int64_t row0 = (int64_t)blockIdx.x * kFinishRows; row0 < n; row0 += (int64_t)gridDim.x * kFinishRows) {
     const int64_t i = row0 + r;
-    const int64_t X = row0 / kSymTile, t = i % kSymTile;
+    const int64_t X = row0 / kSymTile, t = i % kSymTile;  // rows past n read their tile's padding partials
     double s = 0.0;
-    if (i < n) {
-      if (LIST) {
-        const uint32_t* lst = partners + (X * kFinishSlices + k) * pcap;
-        const int cnt = pcount[X * kFinishSlices + k];
-        for (int q = 0; q < cnt; ++q) s += sym_partial(part_i, part_j, X, (int64_t)lst[q], t);
-      } else {
-        for (int64_t P = k; P < ntiles; P += kFinishSlices) s += sym_partial(part_i, part_j, X, P, t);
+    if (LIST) {
+      const uint32_t* lst = partners + (X * kFinishSlices + k) * pcap;
+      const int cnt = __builtin_amdgcn_readfirstlane(pcount[X * kFinishSlices + k]);
+      for (int q0 = 0; q0 < cnt; q0 += kWave) {
+        const int m = cnt - q0 < kWave ? cnt - q0 : kWave;
+        const int mine = r < m ? (int)lst[q0 + r] : 0;
+        int q = 0;
+        for (; q + 4 <= m; q += 4) {
+          const double v0 = sym_partial(part_i, part_j, X, (int64_t)__builtin_amdgcn_readlane(mine, q), t);
+          const double v1 = sym_partial(part_i, part_j, X, (int64_t)__builtin_amdgcn_readlane(mine, q + 1), t);
+          const double v2 = sym_partial(part_i, part_j, X, (int64_t)__builtin_amdgcn_readlane(mine, q + 2), t);
+          const double v3 = sym_partial(part_i, part_j, X, (int64_t)__builtin_amdgcn_readlane(mine, q + 3), t);
+          s += v0;
+          s += v1;
+          s += v2;
+          s += v3;
+        }
+        for (; q < m; ++q) s += sym_partial(part_i, part_j, X, (int64_t)__builtin_amdgcn_readlane(mine, q), t);
       }
+    } else {
+      for (int64_t P = k; P < ntiles; P += kFinishSlices) s += sym_partial(part_i, part_j, X, P, t);
     }
     __syncthreads();
     red[k][r] = s;
@@ -780,8 +822,10 @@ void launch_sym_dim(hipStream_t st, bool coinc, const mplak_kernel_desc* d, cons
   const dim3 grid((unsigned)(nt * (nt + 1) / 2));
   if constexpr (MODE == kSymPlanned && sizeof(T) == 4) {
     if (a.tinfo != nullptr && !coinc) {
+      float* fi = reinterpret_cast<float*>(a.pi);
+      float* fj = fi + (size_t)(nt * (nt + 1) / 2) * kSymTile;
       hipLaunchKernelGGL((kmv_sym_planned_kernel<KIND, DIM>), grid, dim3(kBlock), 0, st, d->points, d->row_offset,
-                         d->row_count, a.scale, a.x, a.pi, a.pj, a.list, a.count, a.tinfo, a.bmask, a.tmask, a.skip);
+                         d->row_count, a.scale, a.x, fi, fj, a.list, a.count, a.tinfo, a.bmask, a.tmask, a.skip);
       return;
     }
   }
@@ -837,12 +881,13 @@ int run_sym(const mplak_kernel_desc* d, int precision, int mode, double drop_s, 
             const uint32_t* partners, const int32_t* pcount, int64_t pcap, const double* x, double* y, void* work,
             const int32_t* skip_if, mplak_stream_t s) {
   SymArgs a;
-  const bool tiles = mode == kSymPlanned && precision == MPLAK_PREC_F32;
+  const bool coinc = d->coincident_shift != 0.0;
+  // planned FP32 without coincidence shift: the planned kernel, FP32 partials
+  const bool tiles = mode == kSymPlanned && precision == MPLAK_PREC_F32 && tinfo != nullptr && !coinc;
   a.tinfo = tiles ? tinfo : nullptr;
   a.bmask = bmask;
   a.tmask = tmask;
   a.scale = coord_scale(d->kind, d->length_scale);
-  const bool coinc = d->coincident_shift != 0.0;
   a.thr = (1e-14 * a.scale) * (1e-14 * a.scale);
   a.drop_s = drop_s;
   a.x = x;
@@ -865,12 +910,18 @@ int run_sym(const mplak_kernel_desc* d, int precision, int mode, double drop_s, 
   if (st != MPLAK_OK) return st;
   int64_t fb = (d->row_count + kFinishRows - 1) / kFinishRows;
   if (fb > 65536) fb = 65536;
-  if (partners != nullptr)
-    hipLaunchKernelGGL(kmv_sym_finish_kernel<true>, dim3((unsigned)fb), dim3(kBlock), 0, hs, d->row_count, nt, pi,
-                       pj, d->nugget, x, y, partners, pcount, pcap, skip_if);
-  else
-    hipLaunchKernelGGL(kmv_sym_finish_kernel<false>, dim3((unsigned)fb), dim3(kBlock), 0, hs, d->row_count, nt, pi,
-                       pj, d->nugget, x, y, nullptr, nullptr, 0, skip_if);
+  if (tiles) {
+    const float* fi = reinterpret_cast<const float*>(pi);
+    const float* fj = fi + (size_t)(nt * (nt + 1) / 2) * kSymTile;
+    hipLaunchKernelGGL((kmv_sym_finish_kernel<true, float>), dim3((unsigned)fb), dim3(kBlock), 0, hs, d->row_count,
+                       nt, fi, fj, d->nugget, x, y, partners, pcount, pcap, skip_if);
+  } else if (partners != nullptr) {
+    hipLaunchKernelGGL((kmv_sym_finish_kernel<true, double>), dim3((unsigned)fb), dim3(kBlock), 0, hs, d->row_count,
+                       nt, pi, pj, d->nugget, x, y, partners, pcount, pcap, skip_if);
+  } else {
+    hipLaunchKernelGGL((kmv_sym_finish_kernel<false, double>), dim3((unsigned)fb), dim3(kBlock), 0, hs,
+                       d->row_count, nt, pi, pj, d->nugget, x, y, nullptr, nullptr, 0, skip_if);
+  }
   return launch_status();
 }
 
