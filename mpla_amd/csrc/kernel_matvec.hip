@@ -282,6 +282,14 @@ __device__ __forceinline__ T box_gap2(const T* a, const T* b) {
   return d2;
 }
 
+// Load through the global address space (pointers read from memory are
+// generic to the compiler, and flat loads also count against the LDS
+// counter, which forces waits on LDS traffic).
+__device__ __forceinline__ double global_load(const double* ptr) {
+  typedef const __attribute__((address_space(1))) double* global_ptr;
+  return *(global_ptr)ptr;
+}
+
 enum SymMode { kSymExact = 0, kSymScreened = 1, kSymPlanned = 2 };
 
 // One tile pair (I, J) of the symmetric product on the vector unit.
@@ -321,15 +329,33 @@ __device__ __forceinline__ void sym_valu_pair(const double* const* __restrict__ 
   const T thr = (T)coinc_thr, shift = (T)coinc_shift;
   const uint32_t wmask = MASKED ? (uint32_t)__builtin_amdgcn_readfirstlane((int)((pmask >> (8 * w)) & 0xffu)) : 0u;
 
-  // two targets per lane: tile offsets w*128 + lane and w*128 + 64 + lane
-  T pi[2][DIM], xi[2];
+  // two targets per lane: tile offsets w*128 + lane and w*128 + 64 + lane;
+  // every global load of the staging is issued before the first use
+  constexpr int kPer = kSymTile / kBlock;
+  double ti[2][DIM], tv[2], sc_[kPer][DIM], sv[kPer];
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     const int64_t i = I * kSymTile + w * 2 * kWave + h * kWave + lane;
     const int64_t ic = i < n ? i : n - 1;
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) pi[h][d] = (T)(points[d][off + ic] * scale);
-    xi[h] = i < n ? (T)x[i] : T(0);
+    for (int d = 0; d < DIM; ++d) ti[h][d] = global_load(points[d] + off + ic);
+    tv[h] = global_load(x + ic);
+  }
+#pragma unroll
+  for (int it = 0; it < kPer; ++it) {
+    const int64_t j = J * kSymTile + tid + it * kBlock;
+    const int64_t jc = j < n ? j : n - 1;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) sc_[it][d] = global_load(points[d] + off + jc);
+    sv[it] = global_load(x + jc);
+  }
+  T pi[2][DIM], xi[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int64_t i = I * kSymTile + w * 2 * kWave + h * kWave + lane;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) pi[h][d] = (T)(ti[h][d] * scale);
+    xi[h] = i < n ? (T)tv[h] : T(0);
   }
   constexpr bool kBoxes = SCREEN && !MASKED;
   T tbox[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
@@ -340,14 +366,17 @@ __device__ __forceinline__ void sym_valu_pair(const double* const* __restrict__ 
       tbox[3 + d] = wave_max(fmax(pi[0][d], pi[1][d]));
     }
   }
-  for (int k = tid; k < kSymTile; k += kBlock) {
+#pragma unroll
+  for (int it = 0; it < kPer; ++it) {
+    const int k = tid + it * kBlock;
     const int64_t j = J * kSymTile + k;
-    const int64_t jc = j < n ? j : n - 1;
     Q q;
-    q.x = (T)(points[0][off + jc] * scale);
-    q.y = DIM > 1 ? (T)(points[1][off + jc] * scale) : T(0);
-    q.z = DIM > 2 ? (T)(points[2][off + jc] * scale) : T(0);
-    q.w = j < n ? (T)x[j] : T(0);
+    q.x = (T)(sc_[it][0] * scale);
+    q.y = T(0);
+    q.z = T(0);
+    if constexpr (DIM > 1) q.y = (T)(sc_[it][1] * scale);
+    if constexpr (DIM > 2) q.z = (T)(sc_[it][2] * scale);
+    q.w = j < n ? (T)sv[it] : T(0);
     if (kBoxes) {  // k / kWave is uniform across the wave: one group per wave and pass
       const T lo0 = wave_min(q.x), lo1 = wave_min(q.y), lo2 = wave_min(q.z);
       const T hi0 = wave_max(q.x), hi1 = wave_max(q.y), hi2 = wave_max(q.z);
@@ -374,11 +403,11 @@ __device__ __forceinline__ void sym_valu_pair(const double* const* __restrict__ 
     for (int h = 0; h < 2; ++h) {
       const T dx = pi[h][0] - q.x;
       T s = dx * dx;
-      if (DIM > 1) {
+      if constexpr (DIM > 1) {
         const T dy = pi[h][1] - q.y;
         s = fma(dy, dy, s);
       }
-      if (DIM > 2) {
+      if constexpr (DIM > 2) {
         const T dz = pi[h][2] - q.z;
         s = fma(dz, dz, s);
       }
@@ -489,13 +518,15 @@ __global__ __launch_bounds__(kBlock) void kmv_sym_kernel(
 // two accumulations.  The tile-centred norms are small (half-diagonal^2 <=
 // kCenterRadius2), so the cancellation error stays at the level of the FP32
 // coordinates themselves.  Both tiles are staged in LDS in operand layout.
-// Wave w takes targets w*128 .. +127 (8 slabs of 16) against the 16 x 16
-// tiles the plan keeps (tile mask of (w, group g): bit 8 b + a for target
-// slab a and source slab b of the group); the branches are on scalar masks.
-// Row sums accR[a][v] collect over the sources of the lane's column and are
-// added over the 16 lanes of a DPP row once per tile pair; column sums
-// collect over the lane's 4 targets of every slab and are added over the four
-// lane rows once per group, then over the four waves in wave order.
+// Wave w takes the target slabs 4 a + w (a < 8; interleaved, which balances
+// the waves' kept tiles on spatially ordered points) against the 16 x 16
+// tiles the plan keeps (tile mask of (w, group g): bit 8 b + a for the
+// wave's slab a and source slab b of the group); the branches are on scalar
+// masks.  Row sums accR[a][v] collect over the sources of the lane's column
+// and are reduce-scattered over the 16 lanes of a DPP row once per tile pair;
+// column sums collect over the lane's 4 targets of every slab and are
+// reduce-scattered over the four lane rows once per group, then added over
+// the four waves in wave order.
 //
 // Other target tiles: the vector unit's difference form over the plan's kept
 // 128 x 64 blocks.  All orders are fixed, so the product is deterministic.
@@ -507,14 +538,62 @@ constexpr int kTmaskWords = (kBlock / kWave) * kSymGroups;  // tile masks per pa
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// Sum over the 16 lanes of a DPP row (butterfly; every lane of the row ends
-// with the same bits).
-__device__ __forceinline__ float row16_sum(float v) {
-  v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));   // quad_perm [1,0,3,2]
-  v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, false));   // quad_perm [2,3,0,1]
-  v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x141, 0xF, 0xF, false));  // row_half_mirror
-  v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x140, 0xF, 0xF, false));  // row_mirror
-  return v;
+template <int CTRL>
+__device__ __forceinline__ float dpp_f32(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+
+// Reduce-scatter over the 16 lanes of a DPP row: v[q] (q < 32) of every lane
+// in, out[h] = sum over the row's lanes of v[16 h + c] for lane c.  Round k
+// pairs lane c with c ^ 15, c ^ 7, c ^ 2, c ^ 1 (row_mirror,
+// row_half_mirror, quad_perm) and keeps the values whose index bit 3, 2, 1,
+// 0 equals the lane's, adding the partner's copy of them.  Fixed order.
+__device__ __forceinline__ void row16_reduce_scatter(const float (&v)[32], int c16, float (&out)[2]) {
+  float r1[16], r2[8], r3[4];
+  const bool b3 = (c16 & 8) != 0, b2 = (c16 & 4) != 0, b1 = (c16 & 2) != 0, b0 = (c16 & 1) != 0;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {  // index 16 h + q (bit 3 clear) pairs with + 8
+      const float lo = v[16 * h + q], hi = v[16 * h + q + 8];
+      r1[8 * h + q] = (b3 ? hi : lo) + dpp_f32<0x140>(b3 ? lo : hi);
+    }
+  }
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float lo = r1[8 * h + q], hi = r1[8 * h + q + 4];
+      r2[4 * h + q] = (b2 ? hi : lo) + dpp_f32<0x141>(b2 ? lo : hi);
+    }
+  }
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const float lo = r2[4 * h + q], hi = r2[4 * h + q + 2];
+      r3[2 * h + q] = (b1 ? hi : lo) + dpp_f32<0x4E>(b1 ? lo : hi);
+    }
+  }
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const float lo = r3[2 * h], hi = r3[2 * h + 1];
+    out[h] = (b0 ? hi : lo) + dpp_f32<0xB1>(b0 ? lo : hi);
+  }
+}
+
+// Reduce-scatter over the four 16-lane rows of the wave: v[b] of every lane
+// in; lane 16 r + c gets the sum over r' of v[r] of lane 16 r' + c.  Fixed
+// order: rows paired by r ^ 1 (lane ^ 16), then r ^ 2 (lane ^ 32).
+__device__ __forceinline__ float rows4_reduce_scatter(const float (&v)[4], int r4) {
+  const bool b0 = (r4 & 1) != 0, b1 = (r4 & 2) != 0;
+  float t[2];
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const float lo = v[2 * q], hi = v[2 * q + 1];
+    t[q] = (b0 ? hi : lo) + __shfl_xor(b0 ? lo : hi, 16);
+  }
+  return (b1 ? t[1] : t[0]) + __shfl_xor(b1 ? t[0] : t[1], 32);
 }
 
 // LDS layout of the matrix-core form (floats): operands [32 slabs][64 lanes]
@@ -548,42 +627,52 @@ __device__ __forceinline__ void sym_mfma_pair(const double* const* __restrict__ 
   const int c16 = lane & 15, r4 = lane >> 4;
   const float ctr[3] = {centre.x, centre.y, centre.z};
 
-  // staging: centred coordinates c, norm |c|^2, value x (0 past the end)
-  auto stage = [&](int64_t X, int k, bool source, bool target) {
-    const int64_t j = X * kSymTile + k;
-    const int64_t jc = j < n ? j : n - 1;
-    float c[3] = {0.0f, 0.0f, 0.0f};
+  // staging: centred coordinates c, norm |c|^2, value x (0 past the end).
+  // Thread t stages points t and t + 256 of both tiles; every global load is
+  // issued first (clamped indices, no branches), then the LDS writes.
+  constexpr int kPer = kSymTile / kBlock;
+  double pc[2][kPer][3], pv[2][kPer];
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) c[d] = (float)(points[d][off + jc] * scale) - ctr[d];
-    float nc = c[0] * c[0];
-    if (DIM > 1) nc = fmaf(c[1], c[1], nc);
-    if (DIM > 2) nc = fmaf(c[2], c[2], nc);
-    const float xv = j < n ? (float)x[j] : 0.0f;
-    const int o = (k >> 4) * kWave + (k & 15);
-    if (source) {
-      mf[kMfB + o] = -2.0f * c[0];
-      mf[kMfB + o + 16] = -2.0f * c[1];
-      mf[kMfB + o + 32] = -2.0f * c[2];
-      mf[kMfB + o + 48] = FACTOR ? nc : 1.0f;
-      if (!FACTOR) mf[kMfNorm + k] = nc;
-      mf[kMfXs + k] = xv;
+  for (int side = 0; side < 2; ++side) {  // 0: sources (tile J), 1: targets (tile I)
+#pragma unroll
+    for (int it = 0; it < kPer; ++it) {
+      const int64_t j = (side == 0 ? J : I) * kSymTile + tid + it * kBlock;
+      const int64_t jc = j < n ? j : n - 1;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) pc[side][it][d] = global_load(points[d] + off + jc);
+      pv[side][it] = global_load(x + jc);
     }
-    if (target) {
-      mf[o] = c[0];
-      mf[o + 16] = c[1];
-      mf[o + 32] = c[2];
-      mf[o + 48] = FACTOR ? 1.0f : nc;
-      const float sc = FACTOR ? __builtin_amdgcn_exp2f(-nc) : 1.0f;
-      mf[kMfXt + k] = xv * sc;
-      if (FACTOR) mf[kMfScale + k] = sc;
-    }
-  };
-  for (int k = tid; k < kSymTile; k += kBlock) {
-    if (I == J) {
-      stage(I, k, true, true);
-    } else {
-      stage(J, k, true, false);
-      stage(I, k, false, true);
+  }
+#pragma unroll
+  for (int side = 0; side < 2; ++side) {
+#pragma unroll
+    for (int it = 0; it < kPer; ++it) {
+      const int k = tid + it * kBlock;
+      const int64_t j = (side == 0 ? J : I) * kSymTile + k;
+      float c[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) c[d] = (float)(pc[side][it][d] * scale) - ctr[d];
+      float nc = c[0] * c[0];
+      if (DIM > 1) nc = fmaf(c[1], c[1], nc);
+      if (DIM > 2) nc = fmaf(c[2], c[2], nc);
+      const float xv = j < n ? (float)pv[side][it] : 0.0f;
+      const int o = (k >> 4) * kWave + (k & 15);
+      if (side == 0) {
+        mf[kMfB + o] = -2.0f * c[0];
+        mf[kMfB + o + 16] = -2.0f * c[1];
+        mf[kMfB + o + 32] = -2.0f * c[2];
+        mf[kMfB + o + 48] = FACTOR ? nc : 1.0f;
+        if (!FACTOR) mf[kMfNorm + k] = nc;
+        mf[kMfXs + k] = xv;
+      } else {
+        mf[o] = c[0];
+        mf[o + 16] = c[1];
+        mf[o + 32] = c[2];
+        mf[o + 48] = FACTOR ? 1.0f : nc;
+        const float sc = FACTOR ? __builtin_amdgcn_exp2f(-nc) : 1.0f;
+        mf[kMfXt + k] = xv * sc;
+        if (FACTOR) mf[kMfScale + k] = sc;
+      }
     }
   }
   // the wave's 8 group masks (scalar loads, before the barrier)
@@ -596,51 +685,56 @@ __device__ __forceinline__ void sym_mfma_pair(const double* const* __restrict__ 
   float A[8], accR[8][4];
 #pragma unroll
   for (int a = 0; a < 8; ++a) {
-    A[a] = mf[(w * 8 + a) * kWave + lane];
+    A[a] = mf[(4 * a + w) * kWave + lane];
 #pragma unroll
     for (int v = 0; v < 4; ++v) accR[a][v] = 0.0f;
   }
-  const float* xt = &mf[kMfXt + w * 2 * kWave + 4 * r4];  // xt[16 a + v]: target 16 a + 4 r + v
+  const float* xt = &mf[kMfXt + 16 * w + 4 * r4];  // xt[64 a + v]: target 16 (4 a + w) + 4 r + v
+  // One 64-source group: target slabs a (outer) against the group's four
+  // source slabs b; the kept MFMAs of slab a are issued before their results
+  // are used.  accR[a][v] collects over b, accC[b] over a and v, in the same
+  // order whatever the mask.
   auto mfma_group = [&](int g, uint32_t gm, auto diagonal) -> float {
     constexpr bool DIAG = decltype(diagonal)::value;
-    float accC[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    float bv[4], xn[4];
+    f32x4 cq[4];
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
-      const uint32_t bm = (gm >> (8 * b)) & 0xffu;
-      if (bm == 0) continue;
-      const float bv = mf[kMfB + (g * 4 + b) * kWave + lane];
-      const float xn = mf[kMfXs + g * kWave + 16 * b + c16];
-      f32x4 cq = {0.0f, 0.0f, 0.0f, 0.0f};
+      bv[b] = mf[kMfB + (g * 4 + b) * kWave + lane];
+      xn[b] = mf[kMfXs + g * kWave + 16 * b + c16];
+      cq[b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
       if (!FACTOR) {
         const float nq = mf[kMfNorm + g * kWave + 16 * b + c16];
-        cq = f32x4{nq, nq, nq, nq};
+        cq[b] = f32x4{nq, nq, nq, nq};
       }
+    }
+    float accC[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-      for (int a = 0; a < 8; ++a) {
-        if (((bm >> a) & 1u) == 0) continue;
-        const f32x4 xm = DIAG ? f32x4{0.0f, 0.0f, 0.0f, 0.0f} : *reinterpret_cast<const f32x4*>(&xt[16 * a]);
-        const f32x4 s = __builtin_amdgcn_mfma_f32_16x16x4f32(A[a], bv, cq, 0, 0, 0);
+    for (int a = 0; a < 8; ++a) {
+      // bit b: tile (a, b) kept (bit 8 b + a of gm)
+      const uint32_t cm = ((gm >> a) & 1u) | ((gm >> (7 + a)) & 2u) | ((gm >> (14 + a)) & 4u) | ((gm >> (21 + a)) & 8u);
+      if (cm == 0) continue;
+      const f32x4 xm = DIAG ? f32x4{0.0f, 0.0f, 0.0f, 0.0f} : *reinterpret_cast<const f32x4*>(&xt[64 * a]);
+      f32x4 s[4];
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+        if ((cm >> b) & 1u) s[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[a], bv[b], cq[b], 0, 0, 0);
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        if (((cm >> b) & 1u) == 0) continue;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          float sv = s[v];
+          float sv = s[b][v];
           if (KIND == MPLAK_KERNEL_MATERN32) sv = fmaxf(sv, 0.0f);  // sqrt of a rounding-negative s
           const float e = kernel_entry<float, KIND>(sv, nullptr);
-          accR[a][v] = fmaf(e, xn, accR[a][v]);
+          accR[a][v] = fmaf(e, xn[b], accR[a][v]);
           if (!DIAG) accC[b] = fmaf(e, xm[v], accC[b]);
         }
       }
     }
     if (DIAG) return 0.0f;
     // lane 16 r + c keeps the sum of source 16 r + c of the group
-    float mine = 0.0f;
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      float t = accC[b];
-      t += __shfl_xor(t, 16);
-      t += __shfl_xor(t, 32);
-      if (r4 == b) mine = t;
-    }
-    return mine;
+    return rows4_reduce_scatter(accC, r4);
   };
   float tg[kSymGroups];
 #pragma unroll
@@ -671,20 +765,17 @@ __device__ __forceinline__ void sym_mfma_pair(const double* const* __restrict__ 
     }
   }
   // row sums: lane 16 r + c writes rows q = c and q = c + 16 of its lane
-  // row's 32 (q = 4 a + v, tile row w * 128 + 16 a + 4 r + v)
-  float out[2] = {0.0f, 0.0f};
+  // row's 32 (q = 4 a + v, tile row 16 (4 a + w) + 4 r + v)
+  float rv[32], out[2];
 #pragma unroll
   for (int a = 0; a < 8; ++a) {
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const float s = row16_sum(accR[a][v]);
-      if (4 * a + v == c16) out[0] = s;
-      if (4 * a + v == c16 + 16) out[1] = s;
-    }
+    for (int v = 0; v < 4; ++v) rv[4 * a + v] = accR[a][v];
   }
+  row16_reduce_scatter(rv, c16, out);
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
-    const int k = w * 2 * kWave + 64 * h + 16 * (c16 >> 2) + 4 * r4 + (c16 & 3);
+    const int k = 256 * h + 64 * (c16 >> 2) + 16 * w + 4 * r4 + (c16 & 3);
     const float v = FACTOR ? out[h] * mf[kMfScale + k] : out[h];
     part_i[p * kSymTile + k] = (P)v;
   }
@@ -932,9 +1023,9 @@ int run_sym(const mplak_kernel_desc* d, int precision, int mode, double drop_s, 
 //  - bmask[p]: bit 8 w + g set when the 128 x 64 block of wave w's targets
 //    of tile I and source group g of tile J is kept (the screened kernel's
 //    test, on the same precision-cast coordinates);
-//  - tmask[p][w][g]: bit 8 b + a set when the 16 x 16 tile of target slab a
-//    of wave w and source slab b of group g is kept (the matrix-core form's
-//    granularity; a kept tile lies in a kept block);
+//  - tmask[p][w][g]: bit 8 b + a set when the 16 x 16 tile of target slab
+//    4 a + w of tile I and source slab 4 g + b of tile J is kept (the
+//    matrix-core form's granularity and wave assignment);
 // and for every tile the centre of its box and whether it is compact
 // (half-diagonal^2 <= kCenterRadius2, FP32 plans only).  Pairs are listed by
 // estimated cost, heaviest first (ties by pair index): kept 16 x 16 tiles /
@@ -1083,7 +1174,7 @@ __global__ __launch_bounds__(kBlock) void plan_tmask_kernel(const double* __rest
     load_box(sboxes + ((size_t)J * kSlabs + 4 * g + b) * 6, sb);
     for (int a = 0; a < 8; ++a) {
       T tb[6];
-      load_box(sboxes + ((size_t)I * kSlabs + 8 * w + a) * 6, tb);
+      load_box(sboxes + ((size_t)I * kSlabs + 4 * a + w) * 6, tb);
       if (box_gap2<T, DIM>(tb, sb) <= (T)drop_s) bits |= 1u << (8 * b + a);
     }
   }
