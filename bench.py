@@ -423,14 +423,15 @@ def measure_inner_product(K, table, xv, yv, stream_ptr, ext, reps=20):
 
     for _ in range(2):
         launch()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record(ext)
-    for _ in range(reps):
+    # one event pair per launch (kernel + finish), averaged: the launch's own
+    # duration, without the gaps between launches
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+    for e0, e1 in ev:
+        e0.record(ext)
         launch()
-    e1.record(ext)
-    e1.synchronize()
-    t = e0.elapsed_time(e1) / reps * 1e-3
+        e1.record(ext)
+    ev[-1][1].synchronize()
+    t = sum(e0.elapsed_time(e1) for e0, e1 in ev) / reps * 1e-3
     counts = torch.zeros(5, dtype=torch.int64, device="cuda")
     _native.check(K.mplak_screen_plan_counts(C.byref(d), plan.data_ptr(), pb, counts.data_ptr(), stream_ptr))
     torch.cuda.synchronize()

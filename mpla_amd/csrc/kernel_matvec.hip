@@ -1031,7 +1031,8 @@ int run_sym(const mplak_kernel_desc* d, int precision, int mode, double drop_s, 
 // estimated cost, heaviest first (ties by pair index): kept 16 x 16 tiles /
 // 32 on compact target tiles, 3 x kept blocks on the others (the vector-unit
 // form costs about three times as much per entry).  Pairs with nothing kept
-// are left out, and their partials are zeroed once at build time.
+// are left out: no workgroup writes their partials, and the finish's partner
+// lists never read them.
 // ---------------------------------------------------------------------------
 
 constexpr uint64_t kPlanDropped = 127;  // key class of pairs with nothing kept
@@ -1225,17 +1226,6 @@ __global__ __launch_bounds__(kBlock) void plan_pair_kernel(const double* __restr
   bmask[p] = bits;
 }
 
-__global__ __launch_bounds__(kBlock) void plan_zero_kernel(const uint64_t* __restrict__ keys, int64_t np,
-                                                           double* __restrict__ part_i, double* __restrict__ part_j) {
-  for (int64_t p = blockIdx.x; p < np; p += gridDim.x) {
-    if ((keys[p] >> 32) != kPlanDropped) continue;
-    for (int k = threadIdx.x; k < kSymTile; k += kBlock) {
-      part_i[p * kSymTile + k] = 0.0;
-      part_j[p * kSymTile + k] = 0.0;
-    }
-  }
-}
-
 // list = pair indices in key order; *count = pairs with something kept.
 __global__ __launch_bounds__(kBlock) void plan_list_kernel(const uint64_t* __restrict__ keys, int64_t np,
                                                            uint32_t* __restrict__ list, int64_t* __restrict__ count) {
@@ -1402,8 +1392,6 @@ int mplak_screen_plan_build(const mplak_kernel_desc* d, int precision, double dr
   double* boxes = reinterpret_cast<double*>(base + L.boxes);
   uint64_t* kin = reinterpret_cast<uint64_t*>(base + L.keys_in);
   uint64_t* kout = reinterpret_cast<uint64_t*>(base + L.keys_out);
-  double* pi = reinterpret_cast<double*>(base + L.part);
-  double* pj = pi + (size_t)np * kSymTile;
   const double scale = coord_scale(d->kind, d->length_scale);
   const double drop_s = drop_distance2(d->kind, drop_log2);
   hipStream_t hs = as_stream(s);
@@ -1418,10 +1406,6 @@ int mplak_screen_plan_build(const mplak_kernel_desc* d, int precision, double dr
     launch_plan_geometry<double>(hs, d, scale, drop_s, q);
   else
     launch_plan_geometry<float>(hs, d, scale, drop_s, q);
-  st = launch_status();
-  if (st != MPLAK_OK) return st;
-  hipLaunchKernelGGL(plan_zero_kernel, dim3((unsigned)std::min<int64_t>(np, 4096)), dim3(kBlock), 0, hs, kin, np, pi,
-                     pj);
   st = launch_status();
   if (st != MPLAK_OK) return st;
   hipLaunchKernelGGL(plan_partner_kernel, dim3((unsigned)((nt * kFinishSlices + kBlock - 1) / kBlock)), dim3(kBlock),
