@@ -824,9 +824,11 @@ __global__ __launch_bounds__(kBlock) void kmv_sym_planned_kernel(
 // list (planned product) a wave visits only the listed partners of its slice,
 // in the same order; the partials it leaves out are zero, so the result is
 // the same.  The wave reads its list with one load per lane and takes the
-// partners from it with readlane, four independent partial loads at a time.
+// partners from it with readlane, kFinishBatch independent partial loads at a
+// time.
 constexpr int kFinishRows = kWave;
 constexpr int kFinishSlices = kBlock / kWave;
+constexpr int kFinishBatch = 8;  // independent partial loads in flight per wave
 
 template <typename PT>
 __device__ __forceinline__ double sym_partial(const PT* __restrict__ part_i, const PT* __restrict__ part_j,
@@ -855,15 +857,13 @@ __global__ __launch_bounds__(kBlock) void kmv_sym_finish_kernel(
         const int m = cnt - q0 < kWave ? cnt - q0 : kWave;
         const int mine = r < m ? (int)lst[q0 + r] : 0;
         int q = 0;
-        for (; q + 4 <= m; q += 4) {
-          const double v0 = sym_partial(part_i, part_j, X, (int64_t)__builtin_amdgcn_readlane(mine, q), t);
-          const double v1 = sym_partial(part_i, part_j, X, (int64_t)__builtin_amdgcn_readlane(mine, q + 1), t);
-          const double v2 = sym_partial(part_i, part_j, X, (int64_t)__builtin_amdgcn_readlane(mine, q + 2), t);
-          const double v3 = sym_partial(part_i, part_j, X, (int64_t)__builtin_amdgcn_readlane(mine, q + 3), t);
-          s += v0;
-          s += v1;
-          s += v2;
-          s += v3;
+        for (; q + kFinishBatch <= m; q += kFinishBatch) {
+          double v[kFinishBatch];
+#pragma unroll
+          for (int u = 0; u < kFinishBatch; ++u)
+            v[u] = sym_partial(part_i, part_j, X, (int64_t)__builtin_amdgcn_readlane(mine, q + u), t);
+#pragma unroll
+          for (int u = 0; u < kFinishBatch; ++u) s += v[u];
         }
         for (; q < m; ++q) s += sym_partial(part_i, part_j, X, (int64_t)__builtin_amdgcn_readlane(mine, q), t);
       }

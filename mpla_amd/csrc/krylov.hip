@@ -20,9 +20,13 @@ namespace mplak {
 namespace {
 
 constexpr int kVecMaxBlocks = 1024;
+// Elements per thread of the fused vector kernels: few, so that a vector of
+// 64 Ki elements spreads over 128 workgroups (these kernels are latency-bound
+// at solver sizes; the partial sums stay a few KB).
+constexpr int kVecPerThread = 2;
 
 inline int vec_blocks(int64_t n) {
-  int64_t b = (n + kBlock * 8 - 1) / (kBlock * 8);
+  int64_t b = (n + kBlock * kVecPerThread - 1) / (kBlock * kVecPerThread);
   if (b < 1) b = 1;
   if (b > kVecMaxBlocks) b = kVecMaxBlocks;
   return static_cast<int>(b);
