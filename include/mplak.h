@@ -177,6 +177,28 @@ int mplak_screen_plan_build(const mplak_kernel_desc* desc, int precision, double
 int mplak_kernel_matvec_planned(const mplak_kernel_desc* desc, int precision, double drop_log2, void* plan,
                                 size_t plan_bytes, const double* x, double* y, const int32_t* skip_if,
                                 mplak_stream_t stream);
+/* No reference counterpart.  mplak_kernel_matvec_planned that also leaves
+ * the partial sums of <x,x> and <x,y> for mplak_cg1_iterate: dots holds
+ * 2 * mplak_planned_dots_count(n) doubles (all <x,x> partials, then all
+ * <x,y> partials), each summed in a fixed order.  y is bitwise that of
+ * mplak_kernel_matvec_planned. */
+int64_t mplak_planned_dots_count(int64_t n);
+int mplak_kernel_matvec_planned_dots(const mplak_kernel_desc* desc, int precision, double drop_log2, void* plan,
+                                     size_t plan_bytes, const double* x, double* y, double* dots,
+                                     const int32_t* skip_if, mplak_stream_t stream);
+/* No reference counterpart.  Relaxed planned FP32 product for inexact inner
+ * solves: nlev (1..3) FP32 plans of the same desc (no coincidence shift), at
+ * drop levels drop_log2[0] > drop_log2[1] > ..., each of plan_bytes.  Level
+ * l > 0 is used while *sel <= sel_below[l-1] (sel_below decreasing; sel is
+ * device memory, read by the kernels, e.g. the inner solve's relative
+ * residual in its mplak_cg_state), level 0 otherwise; nlev = 1 is
+ * mplak_kernel_matvec_planned_dots on plans[0].  All levels use the partial
+ * sums of plans[0].  dots as for mplak_kernel_matvec_planned_dots (may be
+ * NULL). */
+int mplak_kernel_matvec_planned_relaxed(const mplak_kernel_desc* desc, int nlev, const double* drop_log2,
+                                        void* const* plans, size_t plan_bytes, const double* sel,
+                                        const double* sel_below, const double* x, double* y, double* dots,
+                                        const int32_t* skip_if, mplak_stream_t stream);
 /* Writes counts[0] = tile pairs the plan lists, counts[1] = 128 x 64 blocks
  * they hold that pass the screen, counts[2] = those of counts[1] on diagonal
  * tile pairs, counts[3] and counts[4] = kernel entries the planned FP32
@@ -271,6 +293,19 @@ int mplak_cg_start(int64_t n, const double* b, const double* z, double* r, doubl
 int mplak_cg_iterate(int64_t n, double* x, double* r, double* d, const double* z,
                      const mplak_cg_state* in, mplak_cg_state* out, void* work,
                      size_t work_bytes, mplak_stream_t stream);
+
+/* No reference counterpart (used by the mixed-precision inner solves).  One
+ * iteration of single-reduction CG (Chronopoulos & Gear 1989) after
+ * w = A r with dots = the ndots-pair partials of <r,r> and <r,w> (as left by
+ * mplak_kernel_matvec_planned_dots): beta = gamma/gamma_old, alpha =
+ * gamma/(delta - beta gamma/alpha_old), p = r + beta p, s = w + beta s,
+ * x += alpha p, r -= alpha s, in one kernel.  The stopping test is on
+ * gamma = <r,r> of the residual entering the iteration (state.rr), so a
+ * converged solve has applied the operator once more than it updated; k
+ * counts updates, and k == 0 starts the recurrence (p and s need no initial
+ * values).  Same state double-buffering as mplak_cg_iterate. */
+int mplak_cg1_iterate(int64_t n, double* x, double* r, double* p, double* s, const double* w, const double* dots,
+                      int64_t ndots, const mplak_cg_state* in, mplak_cg_state* out, mplak_stream_t stream);
 
 /* Recompute rr = <r,r> into *state (used after loading a checkpointed r). */
 int mplak_cg_refresh_rr(int64_t n, const double* r, mplak_cg_state* state, void* work,

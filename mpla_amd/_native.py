@@ -122,6 +122,11 @@ MPLAK_SIGNATURES = {
     "mplak_screen_plan_build": (C.c_int, [C.POINTER(mplak_kernel_desc), C.c_int, _dbl, _vp, _sz, _vp]),
     "mplak_kernel_matvec_planned": (C.c_int, [C.POINTER(mplak_kernel_desc), C.c_int, _dbl, _vp, _sz, _vp, _vp, _vp,
                                               _vp]),
+    "mplak_planned_dots_count": (_i64, [_i64]),
+    "mplak_kernel_matvec_planned_dots": (C.c_int, [C.POINTER(mplak_kernel_desc), C.c_int, _dbl, _vp, _sz, _vp, _vp,
+                                                   _vp, _vp, _vp]),
+    "mplak_kernel_matvec_planned_relaxed": (C.c_int, [C.POINTER(mplak_kernel_desc), C.c_int, _vp, _vp, _sz, _vp, _vp,
+                                                      _vp, _vp, _vp, _vp, _vp]),
     "mplak_screen_plan_counts": (C.c_int, [C.POINTER(mplak_kernel_desc), _vp, _sz, _vp, _vp]),
     "mplak_spatial_order_workspace_bytes": (_sz, [_i64, _i32]),
     "mplak_spatial_order": (C.c_int, [_vp, _i32, _i64, _i64, _vp, _vp, _sz, _vp]),
@@ -135,6 +140,7 @@ MPLAK_SIGNATURES = {
     "mplak_cg_workspace_bytes": (_sz, [_i64]),
     "mplak_cg_start": (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, C.c_int, _dbl, _i32, _dbl, _vp, _sz, _vp]),
     "mplak_cg_iterate": (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
+    "mplak_cg1_iterate": (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp]),
     "mplak_cg_refresh_rr": (C.c_int, [_i64, _vp, _vp, _vp, _sz, _vp]),
     "mplak_cg_copy_state": (C.c_int, [_vp, _vp, _vp]),
     "mplak_bcg_workspace_bytes": (_sz, [_i64, _i64]),

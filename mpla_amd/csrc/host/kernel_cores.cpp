@@ -61,6 +61,10 @@ size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
 // points: an entry below 2^-80 of the kernel's maximum changes no FP64 row
 // sum of realistic size.
 constexpr double kResidualDropLog2 = 80.0;
+// Loosest relaxed screen of the mixed inner products (entries below 2^-6 =
+// 1/64 are never dropped) and tightest screen the safeguard goes to.
+constexpr double kMinRelaxDrop = 6.0;
+constexpr double kMaxDrop = 60.0;
 
 // Plain FP64 CG (relative rule) through the fused core, printing nothing.
 void plain_fp64_cg(mpla_vector* b, mpla_generic_matrix* A, mpla_vector* x, int iter_max, double eps,
@@ -131,12 +135,21 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
   // (default 1e-4).  The solve runs on the points sorted along a Morton curve
   // (b, x permuted in, x permuted back out), so that the FP32 inner products
   // can skip blocks of entries below 2^-drop: MPLA_IR_DROP_LOG2, by default
-  // 6 bits below the inner tolerance (20 for 1e-4); <= 0 keeps every entry.
-  // The FP64 residual products keep every entry above 2^-80 (the rest cannot
-  // move an FP64 row sum), so the final residual is the true one whatever the
-  // inner operator dropped.
+  // 3 bits below the inner tolerance (17 for 1e-4); <= 0 keeps every entry.
+  // A refinement step that gains much less than eta (the screen costs
+  // contraction) tightens the screen by 6 bits for the next inner solve.
+  // Within an inner solve the products are relaxed as its residual falls
+  // (inexact Krylov: Simoncini & Szyld, SIAM J. Sci. Comput. 25, 2003; van
+  // den Eshof & Sleijpen, SIAM J. Matrix Anal. Appl. 26, 2004, where the
+  // product error may grow like ||r_0|| / ||r_k||): the screen drops 5 bits
+  // more once ||r_k|| <= 2^-5 ||r_0|| and 9 bits more once ||r_k|| <=
+  // 2^-9 ||r_0|| (MPLA_IR_RELAX=0 keeps one level).  The FP64 residual
+  // products keep every entry above 2^-80 (the rest cannot move an FP64 row
+  // sum), so the final residual is the true one whatever the inner operator
+  // dropped.
   const double eta_floor = env_double("MPLA_IR_INNER_TOL", 1.0e-4);
-  const double drop = env_double("MPLA_IR_DROP_LOG2", std::ceil(-std::log2(eta_floor)) + 6.0);
+  double drop = env_double("MPLA_IR_DROP_LOG2", std::ceil(-std::log2(eta_floor)) + 3.0);
+  const bool relax = env_double("MPLA_IR_RELAX", 1.0) != 0.0;
   mplak_kernel_desc shape = desc0;
   shape.row_offset = shape.col_offset = 0;
   const size_t pb = mplak_screen_plan_bytes(&shape);
@@ -149,8 +162,21 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
   const size_t dwb = mplak_dot_workspace_bytes(n);
   const size_t ob = ordered ? mplak_spatial_order_workspace_bytes(n, dim) : 0;
   if (ordered && ob == 0) fail("mpla_kernel_conjugate_gradient_mixed", "spatial order workspace query failed");
-  const size_t order_bytes = ordered ? 3 * vb + (size_t)dim * vb + 256 + align256(ob) + align256(pb) : 0;
-  char* base = static_cast<char*>(arena(inst, kMixed, 4 * vb + sb + wb + align256(dwb) + 256 + order_bytes));
+  // ordered: inner CG in single-reduction form, whose inner products come
+  // from the planned product's finish (direction s = A p and the partials)
+  const int64_t nd = ordered ? mplak_planned_dots_count(n) : 0;
+  const size_t cg1_bytes = ordered ? vb + align256(2 * sizeof(double) * (size_t)nd) : 0;
+  // relaxed levels of the inner products: drop, drop - 5, drop - 9 (each at
+  // least kMinRelaxDrop), one screening plan each
+  constexpr double kRelaxBits[3] = {0.0, 5.0, 9.0};
+  const double relax_below[2] = {std::ldexp(1.0, -5), std::ldexp(1.0, -9)};
+  int nlev = 1;
+  if (ordered && relax)
+    while (nlev < 3 && drop - kRelaxBits[nlev] >= kMinRelaxDrop) ++nlev;
+  const size_t order_bytes =
+      ordered ? 3 * vb + (size_t)dim * vb + 256 + align256(ob) + (size_t)nlev * align256(pb) : 0;
+  char* base =
+      static_cast<char*>(arena(inst, kMixed, 4 * vb + sb + wb + align256(dwb) + 256 + order_bytes + cg1_bytes));
   double* r = reinterpret_cast<double*>(base);
   double* d = reinterpret_cast<double*>(base + vb);
   double* z = reinterpret_cast<double*>(base + 2 * vb);
@@ -166,16 +192,27 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
   double* xs = x->data;        // solution in solver order
   const double* bs = b->data;  // right-hand side in solver order
   int64_t* perm = nullptr;
-  void* plan = nullptr;
+  void* plans[3] = {nullptr, nullptr, nullptr};
+  double drops[3] = {drop, drop, drop};
+  auto build_plans = [&]() {
+    for (int l = 0; l < nlev; ++l) {
+      drops[l] = drop - kRelaxBits[l];
+      ck(mplak_screen_plan_build(&desc, MPLAK_PREC_F32, drops[l], plans[l], pb, s), "mixed CG screening plan");
+    }
+  };
+  double* sv = nullptr;
+  double* dots = nullptr;
   if (ordered) {
     char* ob0 = dotw + align256(dwb) + 256;
+    sv = reinterpret_cast<double*>(ob0 + order_bytes);
+    dots = reinterpret_cast<double*>(ob0 + order_bytes + vb);
     perm = reinterpret_cast<int64_t*>(ob0);
     double* bp = reinterpret_cast<double*>(ob0 + vb);
     double* xp = reinterpret_cast<double*>(ob0 + 2 * vb);
     double* pts = reinterpret_cast<double*>(ob0 + 3 * vb);
     double** table = reinterpret_cast<double**>(ob0 + 3 * vb + (size_t)dim * vb);
     void* owork = ob0 + 3 * vb + (size_t)dim * vb + 256;
-    plan = ob0 + 3 * vb + (size_t)dim * vb + 256 + align256(ob);
+    for (int l = 0; l < nlev; ++l) plans[l] = ob0 + 3 * vb + (size_t)dim * vb + 256 + align256(ob) + l * align256(pb);
     ck(mplak_spatial_order(desc0.points, dim, desc0.row_offset, n, perm, owork, ob, s), "mixed CG spatial order");
     const int64_t ld = (int64_t)(vb / sizeof(double));
     ck(mplak_gather_points(desc0.points, dim, desc0.row_offset, perm, n, pts, ld, s), "mixed CG point gather");
@@ -189,7 +226,7 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
     desc.row_offset = desc.col_offset = 0;
     xs = xp;
     bs = bp;
-    ck(mplak_screen_plan_build(&desc, MPLAK_PREC_F32, drop, plan, pb, s), "mixed CG screening plan");
+    build_plans();
   }
   auto unpermute = [&]() {
     if (ordered) ck(mplak_scatter_f64(x->data, xs, perm, n, s), "mixed CG permute back");
@@ -203,19 +240,25 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
   bool x_zero = *static_cast<double*>(read_back(inst, dscalar, sizeof(double))) == 0.0;
 
   const int max_outer = 40;
-  int total_inner = 0, outer = 0, matvecs64 = 0;
-  double prev_rel = INFINITY;
+  int total_inner = 0, inner_products = 0, outer = 0, matvecs64 = 0;
+  double prev_rel = INFINITY, prev_eta = 1.0;
   mplak_cg_state cur{};
 
-  // inner CG iteration on A_f32 e = r (vectors and scalars in FP64)
+  // inner CG iteration on A_f32 e = r (vectors and scalars in FP64).  On
+  // ordered points: single-reduction CG, two launches per iteration (the
+  // planned product on r with its finish leaving <r,r> and <r,Ar>, then one
+  // vector kernel); otherwise classic CG around the screened/exact product.
   auto iteration = [&](int parity) {
     mplak_cg_state* in = st + parity;
     mplak_cg_state* out = st + (1 - parity);
-    if (ordered)
-      ck(mplak_kernel_matvec_planned(&desc, MPLAK_PREC_F32, drop, plan, pb, d, z, &in->done, s),
+    if (ordered) {
+      ck(mplak_kernel_matvec_planned_relaxed(&desc, nlev, drops, plans, pb, &in->rel, relax_below, r, z, dots,
+                                             &in->done, s),
          "mixed CG inner product");
-    else
-      apply_kernel(desc, MPLAK_PREC_F32, d, z, &in->done, inst, "mixed CG inner product");
+      ck(mplak_cg1_iterate(n, e, r, d, sv, z, dots, nd, in, out, s), "mixed CG inner iteration");
+      return;
+    }
+    apply_kernel(desc, MPLAK_PREC_F32, d, z, &in->done, inst, "mixed CG inner product");
     ck(mplak_cg_iterate(n, e, r, d, z, in, out, work, wb, s), "mixed CG inner iteration");
   };
   auto normalize = [&]() { ck(mplak_cg_copy_state(st + 1, st, s), "mixed CG state copy"); };
@@ -251,10 +294,16 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
       break;
     }
     if (total_inner >= iter_max || outer >= max_outer) break;
+    if (ordered && outer > 0 && cur.rel > 16.0 * prev_eta * prev_rel && drop + 6.0 <= kMaxDrop) {
+      // the screen, not eta, limited the last step: keep more entries
+      drop += 6.0;
+      build_plans();
+    }
     prev_rel = cur.rel;
 
     // inner solve to ||r_inner|| <= eta ||r||, e_0 = 0 (so r_inner,0 = r, d_0 = r)
     const double eta = std::min(0.5, std::max(eta_floor, 0.5 * epsilon / cur.rel));
+    prev_eta = eta;
     mplak_cg_state inner = cur;
     inner.bnorm2 = cur.rr;
     inner.rel = 1.0;
@@ -272,6 +321,9 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
       ist = *static_cast<mplak_cg_state*>(read_back(inst, st, sizeof(mplak_cg_state)));
     }
     total_inner += ist.k;
+    // the single-reduction form applies the operator once more than it
+    // updates when it converges (the product whose <r,r> meets the test)
+    inner_products += ist.k + (ordered && ist.done == MPLAK_CONVERGED ? 1 : 0);
     ck(mplak_daxpy_f64(n, 1.0, e, xs, s), "mixed CG update");
     x_zero = false;
     ++outer;
@@ -292,7 +344,7 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
   inst->last_iterations = iterations;
   inst->last_converged = converged;
   inst->last_rel_res = rel;
-  inst->last_matvecs = total_inner + matvecs64;
+  inst->last_matvecs = inner_products + matvecs64;
   if (inst->is_parent && verbosity(inst) >= 1)
     printf("MPLA mixed CG stopped after %d inner iterations (%d refinement steps) with a final true relative "
            "residual norm of %le.\n",

@@ -793,14 +793,54 @@ __device__ __forceinline__ void sym_mfma_pair(const double* const* __restrict__ 
   }
 }
 
+// Screening plans of one operator at up to kMaxLevels drop levels (level 0
+// the tightest), for relaxed products in inexact inner solves: level l > 0 is
+// used while the value at `sel` is <= below[l] (thresholds decreasing), level
+// 0 otherwise (also for a NaN).  The kernel and its finish read the same
+// value, so they use the same level; every level writes the partial sums of
+// level 0's plan, whose partner lists of the chosen level are all the finish
+// reads.  A single plan is the case nlev = 1.
+constexpr int kMaxLevels = 3;
+struct PlanLevels {
+  const uint32_t* list[kMaxLevels];
+  const int64_t* count[kMaxLevels];
+  const float4* tinfo[kMaxLevels];
+  const uint32_t* bmask[kMaxLevels];
+  const uint32_t* tmask[kMaxLevels];
+  const uint32_t* partners[kMaxLevels];
+  const int32_t* pcount[kMaxLevels];
+  double below[kMaxLevels];
+  const double* sel;
+  int nlev;
+};
+
+// Element l of a kernel-argument array by selects (no dynamic indexing, so
+// the array stays in the argument segment).
+template <typename P>
+__device__ __forceinline__ P level_pick(const P (&a)[kMaxLevels], int l) {
+  return l == 0 ? a[0] : (l == 1 ? a[1] : a[2]);
+}
+
+__device__ __forceinline__ int plan_level(const PlanLevels& lv) {
+  if (lv.sel == nullptr || lv.nlev <= 1) return 0;
+  const double v = *lv.sel;
+  int l = 0;
+  if (v <= lv.below[1]) l = 1;
+  if (lv.nlev > 2 && v <= lv.below[2]) l = 2;
+  return l;
+}
+
 template <int KIND, int DIM>
 __global__ __launch_bounds__(kBlock) void kmv_sym_planned_kernel(
     const double* const* __restrict__ points, int64_t off, int64_t n, double scale, const double* __restrict__ x,
-    float* __restrict__ part_i, float* __restrict__ part_j, const uint32_t* __restrict__ list,
-    const int64_t* __restrict__ count, const float4* __restrict__ tinfo, const uint32_t* __restrict__ bmask,
-    const uint32_t* __restrict__ tmask, const int32_t* __restrict__ skip) {
+    float* __restrict__ part_i, float* __restrict__ part_j, const PlanLevels lv, const int32_t* __restrict__ skip) {
   if (skip != nullptr && *skip != 0) return;
-  if ((int64_t)blockIdx.x >= *count) return;
+  const int lev = plan_level(lv);
+  if ((int64_t)blockIdx.x >= *level_pick(lv.count, lev)) return;
+  const uint32_t* __restrict__ list = level_pick(lv.list, lev);
+  const float4* __restrict__ tinfo = level_pick(lv.tinfo, lev);
+  const uint32_t* __restrict__ bmask = level_pick(lv.bmask, lev);
+  const uint32_t* __restrict__ tmask = level_pick(lv.tmask, lev);
   __shared__ __attribute__((aligned(16))) float mf[kMfFloats];
   static_assert((kBlock / kWave) * kSymTile <= kMfFloats, "column sums reuse the staging buffer");
   static_assert(sizeof(SymSrc<float>) * kSymGroups <= sizeof(mf), "vector-unit staging fits");
@@ -825,7 +865,9 @@ __global__ __launch_bounds__(kBlock) void kmv_sym_planned_kernel(
 // in the same order; the partials it leaves out are zero, so the result is
 // the same.  The wave reads its list with one load per lane and takes the
 // partners from it with readlane, kFinishBatch independent partial loads at a
-// time.
+// time.  With `dots` the finish also leaves, per workgroup, the partial sums
+// of <x,x> and <x,y> over its rows (wave butterfly, so a fixed order):
+// dots[b] and dots[gridDim.x + b].
 constexpr int kFinishRows = kWave;
 constexpr int kFinishSlices = kBlock / kWave;
 constexpr int kFinishBatch = 8;  // independent partial loads in flight per wave
@@ -841,11 +883,15 @@ __device__ __forceinline__ double sym_partial(const PT* __restrict__ part_i, con
 template <bool LIST, typename PT>
 __global__ __launch_bounds__(kBlock) void kmv_sym_finish_kernel(
     int64_t n, int64_t ntiles, const PT* __restrict__ part_i, const PT* __restrict__ part_j, double nugget,
-    const double* __restrict__ x, double* __restrict__ y, const uint32_t* __restrict__ partners,
-    const int32_t* __restrict__ pcount, int64_t pcap, const int32_t* __restrict__ skip) {
+    const double* __restrict__ x, double* __restrict__ y, const PlanLevels lv, int64_t pcap,
+    const int32_t* __restrict__ skip, double* __restrict__ dots) {
   if (skip != nullptr && *skip != 0) return;
+  const int lev = LIST ? plan_level(lv) : 0;
+  const uint32_t* __restrict__ partners = level_pick(lv.partners, lev);
+  const int32_t* __restrict__ pcount = level_pick(lv.pcount, lev);
   __shared__ double red[kFinishSlices][kFinishRows];
   const int r = threadIdx.x % kFinishRows, k = threadIdx.x / kFinishRows;
+  double dxx = 0.0, dxy = 0.0;
   for (int64_t row0 = (int64_t)blockIdx.x * kFinishRows; row0 < n; row0 += (int64_t)gridDim.x * kFinishRows) {
     const int64_t i = row0 + r;
     const int64_t X = row0 / kSymTile, t = i % kSymTile;  // rows past n read their tile's padding partials
@@ -877,8 +923,19 @@ __global__ __launch_bounds__(kBlock) void kmv_sym_finish_kernel(
       double v = red[0][r];
 #pragma unroll
       for (int q = 1; q < kFinishSlices; ++q) v += red[q][r];
-      if (nugget != 0.0) v = fma(nugget, x[i], v);
+      const double xi = x[i];
+      if (nugget != 0.0) v = fma(nugget, xi, v);
       y[i] = v;
+      dxx = fma(xi, xi, dxx);
+      dxy = fma(xi, v, dxy);
+    }
+  }
+  if (dots != nullptr && k == 0) {
+    dxx = wave_sum(dxx);
+    dxy = wave_sum(dxy);
+    if (r == 0) {
+      dots[blockIdx.x] = dxx;
+      dots[gridDim.x + blockIdx.x] = dxy;
     }
   }
 }
@@ -894,16 +951,21 @@ inline bool use_symmetric(const mplak_kernel_desc* d) {
 
 inline int64_t sym_tiles(int64_t n) { return (n + kSymTile - 1) / kSymTile; }
 
+// Workgroups of the symmetric finish (one per 64 rows, grid-strided past 2^16).
+inline int64_t sym_finish_blocks(int64_t n) {
+  const int64_t fb = (n + kFinishRows - 1) / kFinishRows;
+  return fb > 65536 ? 65536 : fb;
+}
+
 struct SymArgs {
   double scale, thr, drop_s;
   const double* x;
   double *pi, *pj;
   const uint32_t* list;
   const int64_t* count;
-  // planned FP32 without coincidence shift: the plan's tile data, for the
-  // planned kernel (matrix-core form on compact target tiles)
-  const float4* tinfo;
-  const uint32_t *bmask, *tmask;
+  // planned FP32 without coincidence shift: the plans' tile data, for the
+  // planned kernel (matrix-core form on compact target tiles); null otherwise
+  const PlanLevels* lv;
   const int32_t* skip;
 };
 
@@ -912,11 +974,11 @@ void launch_sym_dim(hipStream_t st, bool coinc, const mplak_kernel_desc* d, cons
   const int64_t nt = sym_tiles(d->row_count);
   const dim3 grid((unsigned)(nt * (nt + 1) / 2));
   if constexpr (MODE == kSymPlanned && sizeof(T) == 4) {
-    if (a.tinfo != nullptr && !coinc) {
+    if (a.lv != nullptr && !coinc) {
       float* fi = reinterpret_cast<float*>(a.pi);
       float* fj = fi + (size_t)(nt * (nt + 1) / 2) * kSymTile;
       hipLaunchKernelGGL((kmv_sym_planned_kernel<KIND, DIM>), grid, dim3(kBlock), 0, st, d->points, d->row_offset,
-                         d->row_count, a.scale, a.x, fi, fj, a.list, a.count, a.tinfo, a.bmask, a.tmask, a.skip);
+                         d->row_count, a.scale, a.x, fi, fj, *a.lv, a.skip);
       return;
     }
   }
@@ -967,23 +1029,23 @@ size_t sym_workspace_bytes(int64_t n) {
 
 // Symmetric product: tile-pair kernel (exact, screened or planned) and the
 // finish.  `work` holds part_i then part_j ([pairs][512] doubles each).
-int run_sym(const mplak_kernel_desc* d, int precision, int mode, double drop_s, const uint32_t* list,
-            const int64_t* count, const float4* tinfo, const uint32_t* bmask, const uint32_t* tmask,
-            const uint32_t* partners, const int32_t* pcount, int64_t pcap, const double* x, double* y, void* work,
-            const int32_t* skip_if, mplak_stream_t s) {
+// Planned: `lv` holds the plan(s); more than one level only with FP32 tiles.
+int run_sym(const mplak_kernel_desc* d, int precision, int mode, double drop_s, const PlanLevels* lv, int64_t pcap,
+            const double* x, double* y, void* work, const int32_t* skip_if, mplak_stream_t s,
+            double* dots = nullptr) {
   SymArgs a;
   const bool coinc = d->coincident_shift != 0.0;
   // planned FP32 without coincidence shift: the planned kernel, FP32 partials
-  const bool tiles = mode == kSymPlanned && precision == MPLAK_PREC_F32 && tinfo != nullptr && !coinc;
-  a.tinfo = tiles ? tinfo : nullptr;
-  a.bmask = bmask;
-  a.tmask = tmask;
+  const bool tiles = mode == kSymPlanned && precision == MPLAK_PREC_F32 && lv != nullptr && !coinc;
+  if (mode == kSymPlanned && lv == nullptr) return MPLAK_ERR_ARG;
+  if (!tiles && lv != nullptr && lv->nlev > 1) return MPLAK_ERR_ARG;
+  a.lv = tiles ? lv : nullptr;
   a.scale = coord_scale(d->kind, d->length_scale);
   a.thr = (1e-14 * a.scale) * (1e-14 * a.scale);
   a.drop_s = drop_s;
   a.x = x;
-  a.list = list;
-  a.count = count;
+  a.list = lv != nullptr ? lv->list[0] : nullptr;
+  a.count = lv != nullptr ? lv->count[0] : nullptr;
   a.skip = skip_if;
   const int64_t nt = sym_tiles(d->row_count);
   a.pi = static_cast<double*>(work);
@@ -999,19 +1061,19 @@ int run_sym(const mplak_kernel_desc* d, int precision, int mode, double drop_s, 
     launch_sym_any<kSymExact>(hs, precision, coinc, d, a);
   int st = launch_status();
   if (st != MPLAK_OK) return st;
-  int64_t fb = (d->row_count + kFinishRows - 1) / kFinishRows;
-  if (fb > 65536) fb = 65536;
+  const int64_t fb = sym_finish_blocks(d->row_count);
   if (tiles) {
     const float* fi = reinterpret_cast<const float*>(pi);
     const float* fj = fi + (size_t)(nt * (nt + 1) / 2) * kSymTile;
     hipLaunchKernelGGL((kmv_sym_finish_kernel<true, float>), dim3((unsigned)fb), dim3(kBlock), 0, hs, d->row_count,
-                       nt, fi, fj, d->nugget, x, y, partners, pcount, pcap, skip_if);
-  } else if (partners != nullptr) {
+                       nt, fi, fj, d->nugget, x, y, *lv, pcap, skip_if, dots);
+  } else if (lv != nullptr) {
     hipLaunchKernelGGL((kmv_sym_finish_kernel<true, double>), dim3((unsigned)fb), dim3(kBlock), 0, hs, d->row_count,
-                       nt, pi, pj, d->nugget, x, y, partners, pcount, pcap, skip_if);
+                       nt, pi, pj, d->nugget, x, y, *lv, pcap, skip_if, dots);
   } else {
+    const PlanLevels none{};
     hipLaunchKernelGGL((kmv_sym_finish_kernel<false, double>), dim3((unsigned)fb), dim3(kBlock), 0, hs,
-                       d->row_count, nt, pi, pj, d->nugget, x, y, nullptr, nullptr, 0, skip_if);
+                       d->row_count, nt, pi, pj, d->nugget, x, y, none, 0, skip_if, dots);
   }
   return launch_status();
 }
@@ -1070,6 +1132,17 @@ PlanLayout plan_layout(int64_t n) {
   L.part = L.partners + align_up(sizeof(uint32_t) * (size_t)nt * kFinishSlices * (size_t)L.pcap, 256);
   L.total = L.part + 2 * np * kSymTile * sizeof(double);
   return L;
+}
+
+// Level l of `lv` from the plan at `base`.
+void plan_level_view(const PlanLayout& L, char* base, int l, PlanLevels& lv) {
+  lv.list[l] = reinterpret_cast<const uint32_t*>(base + L.list);
+  lv.count[l] = reinterpret_cast<const int64_t*>(base + L.count);
+  lv.tinfo[l] = reinterpret_cast<const float4*>(base + L.tinfo);
+  lv.bmask[l] = reinterpret_cast<const uint32_t*>(base + L.bmask);
+  lv.tmask[l] = reinterpret_cast<const uint32_t*>(base + L.tmask);
+  lv.partners[l] = reinterpret_cast<const uint32_t*>(base + L.partners);
+  lv.pcount[l] = reinterpret_cast<const int32_t*>(base + L.pcount);
 }
 
 template <typename T>
@@ -1367,8 +1440,7 @@ int mplak_kernel_matvec_screened(const mplak_kernel_desc* d, int precision, doub
   if (d->row_count == 0) return MPLAK_OK;
   if (x == nullptr || y == nullptr) return MPLAK_ERR_ARG;
   if (work == nullptr || work_bytes < sym_workspace_bytes(d->row_count)) return MPLAK_ERR_WORKSPACE;
-  return run_sym(d, precision, kSymScreened, drop_distance2(d->kind, drop_log2), nullptr, nullptr, nullptr, nullptr,
-                 nullptr, nullptr, nullptr, 0, x, y, work, skip_if, s);
+  return run_sym(d, precision, kSymScreened, drop_distance2(d->kind, drop_log2), nullptr, 0, x, y, work, skip_if, s);
 }
 
 size_t mplak_screen_plan_bytes(const mplak_kernel_desc* d) {
@@ -1450,11 +1522,60 @@ int mplak_kernel_matvec_planned(const mplak_kernel_desc* d, int precision, doubl
   if (L.total == 0) return MPLAK_ERR_HIP;
   if (plan == nullptr || plan_bytes < L.total) return MPLAK_ERR_WORKSPACE;
   char* base = static_cast<char*>(plan);
-  return run_sym(d, precision, kSymPlanned, drop_distance2(d->kind, drop_log2),
-                 reinterpret_cast<const uint32_t*>(base + L.list), reinterpret_cast<const int64_t*>(base + L.count),
-                 reinterpret_cast<const float4*>(base + L.tinfo), reinterpret_cast<const uint32_t*>(base + L.bmask),
-                 reinterpret_cast<const uint32_t*>(base + L.tmask), reinterpret_cast<const uint32_t*>(base + L.partners), reinterpret_cast<const int32_t*>(base + L.pcount),
-                 L.pcap, x, y, base + L.part, skip_if, s);
+  PlanLevels lv{};
+  plan_level_view(L, base, 0, lv);
+  lv.nlev = 1;
+  return run_sym(d, precision, kSymPlanned, drop_distance2(d->kind, drop_log2), &lv, L.pcap, x, y, base + L.part,
+                 skip_if, s);
+}
+
+int64_t mplak_planned_dots_count(int64_t n) { return n > 0 ? mplak::sym_finish_blocks(n) : 0; }
+
+int mplak_kernel_matvec_planned_dots(const mplak_kernel_desc* d, int precision, double drop_log2, void* plan,
+                                     size_t plan_bytes, const double* x, double* y, double* dots,
+                                     const int32_t* skip_if, mplak_stream_t s) {
+  int st = check_desc(d);
+  if (st != MPLAK_OK) return st;
+  if (precision != MPLAK_PREC_F64 && precision != MPLAK_PREC_F32) return MPLAK_ERR_ARG;
+  if (!(drop_log2 > 0.0) || drop_log2 > 2000.0) return MPLAK_ERR_ARG;
+  if (!sym_eligible(d) || d->row_count == 0) return MPLAK_ERR_ARG;
+  if (x == nullptr || y == nullptr || dots == nullptr) return MPLAK_ERR_ARG;
+  const PlanLayout L = plan_layout(d->row_count);
+  if (L.total == 0) return MPLAK_ERR_HIP;
+  if (plan == nullptr || plan_bytes < L.total) return MPLAK_ERR_WORKSPACE;
+  char* base = static_cast<char*>(plan);
+  PlanLevels lv{};
+  plan_level_view(L, base, 0, lv);
+  lv.nlev = 1;
+  return run_sym(d, precision, kSymPlanned, drop_distance2(d->kind, drop_log2), &lv, L.pcap, x, y, base + L.part,
+                 skip_if, s, dots);
+}
+
+int mplak_kernel_matvec_planned_relaxed(const mplak_kernel_desc* d, int nlev, const double* drop_log2,
+                                        void* const* plans, size_t plan_bytes, const double* sel,
+                                        const double* sel_below, const double* x, double* y, double* dots,
+                                        const int32_t* skip_if, mplak_stream_t s) {
+  int st = check_desc(d);
+  if (st != MPLAK_OK) return st;
+  if (nlev < 1 || nlev > kMaxLevels || drop_log2 == nullptr || plans == nullptr) return MPLAK_ERR_ARG;
+  if (nlev > 1 && (sel == nullptr || sel_below == nullptr)) return MPLAK_ERR_ARG;
+  if (!sym_eligible(d) || d->row_count == 0 || d->coincident_shift != 0.0) return MPLAK_ERR_ARG;
+  if (x == nullptr || y == nullptr) return MPLAK_ERR_ARG;
+  const PlanLayout L = plan_layout(d->row_count);
+  if (L.total == 0) return MPLAK_ERR_HIP;
+  if (plan_bytes < L.total) return MPLAK_ERR_WORKSPACE;
+  PlanLevels lv{};
+  for (int l = 0; l < nlev; ++l) {
+    if (!(drop_log2[l] > 0.0) || drop_log2[l] > 2000.0 || plans[l] == nullptr) return MPLAK_ERR_ARG;
+    if (l > 0 && !(drop_log2[l] < drop_log2[l - 1] && sel_below[l - 1] < (l > 1 ? sel_below[l - 2] : INFINITY)))
+      return MPLAK_ERR_ARG;
+    plan_level_view(L, static_cast<char*>(plans[l]), l, lv);
+    lv.below[l] = l > 0 ? sel_below[l - 1] : INFINITY;
+  }
+  lv.sel = nlev > 1 ? sel : nullptr;
+  lv.nlev = nlev;
+  return run_sym(d, MPLAK_PREC_F32, kSymPlanned, drop_distance2(d->kind, drop_log2[0]), &lv, L.pcap, x, y,
+                 static_cast<char*>(plans[0]) + L.part, skip_if, s, dots);
 }
 
 int mplak_kernel_matvec(const mplak_kernel_desc* d, int precision, const double* x, double* y,
@@ -1469,8 +1590,7 @@ int mplak_kernel_matvec(const mplak_kernel_desc* d, int precision, const double*
   const size_t need = mplak_kernel_matvec_workspace_bytes(d);
   if (work == nullptr || work_bytes < need) return MPLAK_ERR_WORKSPACE;
   if (use_symmetric(d))
-    return run_sym(d, precision, kSymExact, 0.0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0, x,
-                   y, work, skip_if, s);
+    return run_sym(d, precision, kSymExact, 0.0, nullptr, 0, x, y, work, skip_if, s);
 
   const double scale = coord_scale(d->kind, d->length_scale);
   const bool coinc = d->coincident_shift != 0.0;

@@ -167,6 +167,64 @@ __global__ __launch_bounds__(kBlock) void cg_direct_kernel(int64_t n, const doub
   }
 }
 
+// Single-reduction CG (Chronopoulos & Gear, J. Comput. Appl. Math. 25, 1989;
+// the form of Ghysels & Vanroose, Parallel Comput. 40, 2014, Alg. 2).  The
+// operator is applied to r, and the two inner products the iteration needs,
+// gamma = <r,r> and delta = <r,Ar>, arrive as partial sums from the
+// operator's own finish, so one kernel per iteration does the rest:
+//   beta = gamma/gamma_old, alpha = gamma/(delta - beta gamma/alpha_old)
+//   (first iteration: beta = 0, alpha = gamma/delta)
+//   p = r + beta p, s = w + beta s, x += alpha p, r -= alpha s
+// The stopping test is on gamma, the residual entering the iteration; on
+// convergence nothing is updated.  State fields: rr = gamma, alpha and beta
+// of the last update, k = updates done.
+__global__ __launch_bounds__(kBlock) void cg1_update_kernel(int64_t n, double* __restrict__ x,
+                                                            double* __restrict__ r, double* __restrict__ p,
+                                                            double* __restrict__ s, const double* __restrict__ w,
+                                                            const double* __restrict__ dots, int nd,
+                                                            const mplak_cg_state* __restrict__ in,
+                                                            mplak_cg_state* __restrict__ out) {
+  const mplak_cg_state s_in = *in;
+  if (s_in.done != MPLAK_RUNNING) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *out = s_in;
+    return;
+  }
+  __shared__ double scratch[kBlock / kWave];
+  const double gam = reduce_partials<kBlock>(dots, nd, scratch);
+  const double del = reduce_partials<kBlock>(dots + nd, nd, scratch);
+  const double rel = stop_value(s_in.rule, gam, s_in.bnorm2, s_in.n_global);
+  const bool conv = rel < s_in.eps;
+  const bool first = s_in.k == 0;
+  const double beta = first ? 0.0 : gam / s_in.rr;
+  const double alpha = first ? gam / del : gam / (del - beta * gam / s_in.alpha);
+  if (!conv) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+      const double ri = r[i], wi = w[i];
+      const double pi = first ? ri : fma(beta, p[i], ri);
+      const double si = first ? wi : fma(beta, s[i], wi);
+      p[i] = pi;
+      s[i] = si;
+      x[i] = fma(alpha, pi, x[i]);
+      r[i] = fma(-alpha, si, ri);
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    mplak_cg_state o = s_in;
+    o.rr = gam;
+    o.rel = rel;
+    if (conv) {
+      o.done = MPLAK_CONVERGED;
+    } else {
+      o.alpha = alpha;
+      o.beta = beta;
+      o.k = s_in.k + 1;
+      o.done = o.k >= s_in.iter_max - 1 ? MPLAK_ITER_LIMIT : MPLAK_RUNNING;
+    }
+    *out = o;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // BiCGSTAB (src/mpla.cpp:1003-1183).  Per iteration, with the caller's
 // operator applications in between:
@@ -636,6 +694,15 @@ int mplak_cg_iterate(int64_t n, double* x, double* r, double* d, const double* z
   st = launch_status();
   if (st != MPLAK_OK) return st;
   hipLaunchKernelGGL(cg_direct_kernel, dim3(g), dim3(kBlock), 0, hs, n, r, d, in, out, part);
+  return launch_status();
+}
+
+int mplak_cg1_iterate(int64_t n, double* x, double* r, double* p, double* sv, const double* w, const double* dots,
+                      int64_t ndots, const mplak_cg_state* in, mplak_cg_state* out, mplak_stream_t s) {
+  if (n <= 0 || !x || !r || !p || !sv || !w || !dots || !in || !out || in == out) return MPLAK_ERR_ARG;
+  if (ndots <= 0 || ndots > (int64_t)1 << 20) return MPLAK_ERR_ARG;
+  hipLaunchKernelGGL(cg1_update_kernel, dim3(vec_blocks(n)), dim3(kBlock), 0, as_stream(s), n, x, r, p, sv, w, dots,
+                     (int)ndots, in, out);
   return launch_status();
 }
 

@@ -404,6 +404,37 @@ def test_mixed_precision_ordered_screened_vs_exact_inner(s, oracle, monkeypatch)
     assert np.linalg.norm(sols["30"] - sols["0"]) <= 1e-7 * np.linalg.norm(sols["0"])
 
 
+@pytest.mark.parametrize("relax,drop", [("1", None), ("0", None), ("0", "7")])
+def test_mixed_precision_relaxed_and_safeguarded_screens(s, oracle, monkeypatch, relax, drop):
+    """Default screen (3 bits below the inner tolerance) with the inner
+    products relaxed as the inner residual falls (MPLA_IR_RELAX=1), without
+    relaxation, and a screen so loose (2^-7) that the refinement safeguard
+    has to tighten it: every run reaches the FP64 bar on the oracle's
+    residual and the FP64 oracle solution to 1e-6, bitwise repeatably."""
+    n = 13000
+    pts, bh, k = gaussian_system(oracle, n, 0.05, 1.0, 0)
+    monkeypatch.setenv("MPLA_IR_RELAX", relax)
+    if drop is not None:
+        monkeypatch.setenv("MPLA_IR_DROP_LOG2", drop)
+    P = Points(pts)
+    kd = mpla.mpla_kernel_matrix_data(P.ptr, 3, 0, 0, mpla.MPLA_PREC_F64, 0.05, 1.0, 0.0)
+    A = s.generic(n, n, C.addressof(kd))
+    b = s.vector(n, bh)
+    x = s.vector(n)
+    runs = []
+    for _ in range(2):
+        s.set(x, np.zeros(n))
+        s.lib.mpla_kernel_conjugate_gradient_mixed(C.byref(b), C.byref(A), C.byref(x), 5000, 1e-8,
+                                                   mpla.MPLA_PREC_F32, s.I)
+        assert s.inst.last_converged == 1
+        runs.append(s.get(x))
+    assert np.array_equal(runs[0], runs[1])
+    xs = runs[0]
+    assert np.linalg.norm(bh - k.matvec(xs)) / np.linalg.norm(bh) < 1e-8 * 1.01
+    xo, _ = oracle.cg_relative(k.op(), bh, np.zeros(n), 5000, 1e-12)
+    assert np.linalg.norm(xs - xo) <= 1e-6 * np.linalg.norm(xo)
+
+
 def test_graph_replay_is_deterministic(s, oracle):
     n = 4096
     pts, bh, k = gaussian_system(oracle, n)

@@ -337,3 +337,196 @@ def test_plan_counts(lib, oracle):
         assert got == plan_counts_numpy(po, l, 20.0), (dim, got)
         nt = (n + 511) // 512
         assert nt <= got[0] < nt * (nt + 1) // 2 and got[3] + got[4] < 8192 * got[1]
+
+
+def planned_dots(lib, desc, prec, drop, plan, wb, X, y, dots, skip=None):
+    ck(lib.mplak_kernel_matvec_planned_dots(C.byref(desc), prec, drop, plan.data_ptr(), wb, X.data_ptr(),
+                                            y.data_ptr(), dots.data_ptr(), skip, stream()))
+
+
+@pytest.mark.parametrize("kind,l,dim,n", [(0, 0.05, 3, 20000), (1, 0.1, 3, 7000), (0, 0.02, 2, 9000),
+                                          (0, 0.05, 3, 511)])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_planned_dots(lib, oracle, kind, l, dim, n, prec):
+    """The planned product that also leaves the partials of <x,x> and <x,y>:
+    y bitwise that of the planned product; the partials (one pair per 64
+    rows) add up to the two inner products of the returned y within 1e-13
+    (FP64 sums of the same terms in another order)."""
+    pts = ordered_points(lib, oracle, n, dim, 90 + dim)
+    P = dev(pts)
+    T = point_table(P)
+    d = kernel_desc(P, T, kind, l, nugget=0.5)
+    plan, wb = build_plan(lib, d, prec, 20.0)
+    nd = lib.mplak_planned_dots_count(n)
+    assert nd == (n + 63) // 64
+    assert lib.mplak_planned_dots_count(0) == 0
+    x = oracle.uniform(n, 91, 1, -1, 1)
+    X = dev(x)
+    y = empty(n)
+    dots = empty(2 * nd)
+    planned_dots(lib, d, prec, 20.0, plan, wb, X, y, dots)
+    yh, dh = host(y), host(dots)
+    assert np.array_equal(yh, planned(lib, d, prec, 20.0, plan, wb, X))
+    xx, xy = math.fsum(x * x), math.fsum(x * yh)
+    assert abs(math.fsum(dh[:nd]) - xx) <= 1e-13 * xx
+    assert abs(math.fsum(dh[nd:]) - xy) <= 1e-13 * math.fsum(np.abs(x * yh))
+    # a set skip flag leaves y and the partials untouched
+    flag = dev(np.array([1], dtype=np.int32), dtype=torch_int32())
+    y2, d2 = dev(np.full(n, 7.0)), dev(np.full(2 * nd, 7.0))
+    planned_dots(lib, d, prec, 20.0, plan, wb, X, y2, d2, flag.data_ptr())
+    assert np.all(host(y2) == 7.0) and np.all(host(d2) == 7.0)
+    assert lib.mplak_kernel_matvec_planned_dots(C.byref(d), prec, 20.0, plan.data_ptr(), wb, X.data_ptr(),
+                                                y.data_ptr(), None, None, stream()) == _native.MPLAK_ERR_ARG
+
+
+def torch_int32():
+    import torch
+    return torch.int32
+
+
+def cg1_solve(lib, d, prec, drop, b, eps, iter_max=2001):
+    """Single-reduction CG through the C ABI: planned product on r with its
+    dot partials, then mplak_cg1_iterate, until the state is done."""
+    import torch
+    n = d.row_count
+    plan, wb = build_plan(lib, d, prec, drop)
+    nd = lib.mplak_planned_dots_count(n)
+    B = dev(b)
+    x, r, p, s, w, z = (torch.zeros(n, dtype=torch.float64, device="cuda") for _ in range(6))
+    dots = empty(2 * nd)
+    ssz = C.sizeof(_native.mplak_cg_state)
+    st = torch.zeros(2 * ssz // 8, dtype=torch.float64, device="cuda")
+    cw = lib.mplak_cg_workspace_bytes(n)
+    cwork = workspace(cw)
+    ck(lib.mplak_cg_start(n, B.data_ptr(), z.data_ptr(), r.data_ptr(), p.data_ptr(), st.data_ptr(),
+                          _native.MPLAK_STOP_RELATIVE, eps, iter_max, float(n), cwork.data_ptr(), cw, stream()))
+    products = 0
+    for it in range(iter_max + 1):
+        sin, sout = st.data_ptr() + (it & 1) * ssz, st.data_ptr() + (1 - (it & 1)) * ssz
+        planned_dots(lib, d, prec, drop, plan, wb, r, w, dots, sin + _native.mplak_cg_state.done.offset)
+        ck(lib.mplak_cg1_iterate(n, x.data_ptr(), r.data_ptr(), p.data_ptr(), s.data_ptr(), w.data_ptr(),
+                                 dots.data_ptr(), nd, sin, sout, stream()))
+        products += 1
+        state = _native.mplak_cg_state.from_buffer_copy(host(st).tobytes()[(1 - (it & 1)) * ssz:][:ssz])
+        if state.done != _native.MPLAK_RUNNING:
+            return host(x), state, products
+    raise AssertionError("single-reduction CG did not stop")
+
+
+@pytest.mark.parametrize("kind,l", [(0, 0.05), (1, 0.1)])
+def test_cg1_matches_classic_cg_fp64(lib, oracle, kind, l):
+    """Single-reduction CG with the FP64 planned product (nothing dropped at
+    2^-2000) is classic CG rearranged: it stops within one iteration of the
+    oracle's CG (src/mpla.cpp:887-1001 stopping rule) on the same system,
+    its true residual (oracle product) meets the tolerance, and its solution
+    agrees with the oracle's."""
+    n, eps = 4000, 1e-10
+    pts = ordered_points(lib, oracle, n, 3, 95 + kind)
+    b = oracle.uniform(n, 96, 1, -1, 1)
+    P = dev(pts)
+    T = point_table(P)
+    d = kernel_desc(P, T, kind, l, nugget=1.0)
+    x, state, products = cg1_solve(lib, d, 0, 2000.0, b, eps)
+    assert state.done == _native.MPLAK_CONVERGED
+    assert products == state.k + 1
+    k = oracle.Kernel(pts, kind, l, nugget=1.0)
+    xr, info = oracle.cg_relative(k.op(), b, np.zeros(n), 2001, eps)
+    assert info.converged and abs(state.k - info.iterations) <= 1
+    res = np.linalg.norm(b - k.matvec(x)) / np.linalg.norm(b)
+    assert res <= 1.5 * eps
+    assert np.linalg.norm(x - xr) <= 1e-8 * np.linalg.norm(xr)
+
+
+def test_cg1_matches_classic_cg_fp32_planned(lib, oracle):
+    """The mixed solver's inner form: planned FP32 operator (2^-20 screen),
+    tolerance 1e-4.  Single-reduction CG and classic CG (mplak_cg_iterate) on
+    the same operator stop within one iteration of each other, and both
+    solutions meet the tolerance on the oracle's FP64 residual (the FP32
+    operator and its 2^-20 screen move it by far less than 1e-4)."""
+    import torch
+    n, eps, drop = 12000, 1e-4, 20.0
+    pts = ordered_points(lib, oracle, n, 3, 97)
+    b = oracle.uniform(n, 98, 1, -1, 1)
+    P = dev(pts)
+    T = point_table(P)
+    d = kernel_desc(P, T, 0, 0.05, nugget=1.0)
+    x1, s1, _ = cg1_solve(lib, d, 1, drop, b, eps)
+    assert s1.done == _native.MPLAK_CONVERGED
+    # classic CG on the same planned operator
+    plan, wb = build_plan(lib, d, 1, drop)
+    B = dev(b)
+    x, r, dd, z = (torch.zeros(n, dtype=torch.float64, device="cuda") for _ in range(4))
+    ssz = C.sizeof(_native.mplak_cg_state)
+    st = torch.zeros(2 * ssz // 8, dtype=torch.float64, device="cuda")
+    cw = lib.mplak_cg_workspace_bytes(n)
+    cwork = workspace(cw)
+    ck(lib.mplak_cg_start(n, B.data_ptr(), z.data_ptr(), r.data_ptr(), dd.data_ptr(), st.data_ptr(),
+                          _native.MPLAK_STOP_RELATIVE, eps, 2001, float(n), cwork.data_ptr(), cw, stream()))
+    for it in range(2001):
+        sin, sout = st.data_ptr() + (it & 1) * ssz, st.data_ptr() + (1 - (it & 1)) * ssz
+        ck(lib.mplak_kernel_matvec_planned(C.byref(d), 1, drop, plan.data_ptr(), wb, dd.data_ptr(), z.data_ptr(),
+                                           sin + _native.mplak_cg_state.done.offset, stream()))
+        ck(lib.mplak_cg_iterate(n, x.data_ptr(), r.data_ptr(), dd.data_ptr(), z.data_ptr(), sin, sout,
+                                cwork.data_ptr(), cw, stream()))
+        s2 = _native.mplak_cg_state.from_buffer_copy(host(st).tobytes()[(1 - (it & 1)) * ssz:][:ssz])
+        if s2.done != _native.MPLAK_RUNNING:
+            break
+    assert s2.done == _native.MPLAK_CONVERGED
+    assert abs(s1.k - s2.k) <= 1
+    k = oracle.Kernel(pts, 0, 0.05, nugget=1.0)
+    for sol in (x1, host(x)):
+        assert np.linalg.norm(b - k.matvec(sol)) <= 1.5 * eps * np.linalg.norm(b)
+
+
+def test_planned_relaxed_levels(lib, oracle):
+    """Three FP32 plans at 2^-17, 2^-12, 2^-8 behind one launch: the level
+    the selector value picks (<= 2^-5 for level 1, <= 2^-9 for level 2)
+    gives bitwise the planned product of that level's plan, with its dot
+    partials; one level is bitwise mplak_kernel_matvec_planned_dots.
+    Invalid level sets are refused."""
+    import torch
+    n = 20000
+    pts = ordered_points(lib, oracle, n, 3, 99)
+    P = dev(pts)
+    T = point_table(P)
+    d = kernel_desc(P, T, 0, 0.05, nugget=1.0)
+    drops = [17.0, 12.0, 8.0]
+    built = [build_plan(lib, d, 1, dr) for dr in drops]
+    wb = built[0][1]
+    plans = (C.c_void_p * 3)(*[p.data_ptr() for p, _ in built])
+    dl = (C.c_double * 3)(*drops)
+    below = (C.c_double * 2)(2.0 ** -5, 2.0 ** -9)
+    x = oracle.uniform(n, 100, 1, -1, 1)
+    X = dev(x)
+    nd = lib.mplak_planned_dots_count(n)
+    sel = dev(np.array([1.0]))
+    for value, level in ((1.0, 0), (0.04, 0), (2.0 ** -5, 1), (2.0 ** -7, 1), (2.0 ** -9, 2), (1e-6, 2),
+                         (float("nan"), 0)):
+        sel.fill_(value)
+        y, dots = empty(n), empty(2 * nd)
+        ck(lib.mplak_kernel_matvec_planned_relaxed(C.byref(d), 3, dl, plans, wb, sel.data_ptr(), below, X.data_ptr(),
+                                                   y.data_ptr(), dots.data_ptr(), None, stream()))
+        yr, dr_ = host(y), host(dots)
+        plan, pwb = built[level]
+        y1, d1 = empty(n), empty(2 * nd)
+        planned_dots(lib, d, 1, drops[level], plan, pwb, X, y1, d1)
+        assert np.array_equal(yr, host(y1)), (value, level)
+        assert np.array_equal(dr_, host(d1)), (value, level)
+    # one level: the single-plan product
+    y, dots = empty(n), empty(2 * nd)
+    ck(lib.mplak_kernel_matvec_planned_relaxed(C.byref(d), 1, dl, plans, wb, None, None, X.data_ptr(), y.data_ptr(),
+                                               dots.data_ptr(), None, stream()))
+    y1, d1 = empty(n), empty(2 * nd)
+    planned_dots(lib, d, 1, drops[0], built[0][0], wb, X, y1, d1)
+    assert np.array_equal(host(y), host(y1)) and np.array_equal(host(dots), host(d1))
+    bad = _native.MPLAK_ERR_ARG
+    for nlev, dd, bb in ((4, drops, (2.0 ** -5, 2.0 ** -9)), (0, drops, (2.0 ** -5, 2.0 ** -9)),
+                         (3, [17.0, 12.0, 12.0], (2.0 ** -5, 2.0 ** -9)), (3, drops, (2.0 ** -9, 2.0 ** -5)),
+                         (2, drops, (float("nan"), 0.0))):
+        dl2 = (C.c_double * 4)(*(list(dd) + [1.0] * (4 - len(dd))))
+        b2 = (C.c_double * 3)(*(list(bb) + [0.0]))
+        pl4 = (C.c_void_p * 4)(*([p.data_ptr() for p, _ in built] + [built[0][0].data_ptr()]))
+        assert lib.mplak_kernel_matvec_planned_relaxed(C.byref(d), nlev, dl2, pl4, wb, sel.data_ptr(), b2,
+                                                       X.data_ptr(), y.data_ptr(), None, None, stream()) == bad
+    assert lib.mplak_kernel_matvec_planned_relaxed(C.byref(d), 2, dl, plans, wb, None, below, X.data_ptr(),
+                                                   y.data_ptr(), None, None, stream()) == bad
