@@ -37,7 +37,8 @@ EPS = 1e-8
 SEED = 1235
 ITER_MAX = 20001
 FLOP_PER_ENTRY = 3 * DIM + 4  # d subs, d FMA, 1 scale, 1 exp, 1 FMA accumulate
-DROP_LOG2 = 20.0  # the mixed solver's default drop level at its default inner tolerance 1e-4
+DROP_LOG2 = 17.0  # the mixed solver's default drop level at its default inner tolerance 1e-4
+RELAX_LEVELS = (17.0, 12.0, 8.0)  # its relaxed inner-product screens (inner residual > 2^-5, > 2^-9, below)
 PEAK_FP32_NONPACKED = 78.6e12  # one FP32 FMA per lane per cycle; the 157.3 TF figure counts packed FP32
 PEAK_FP32 = 157.3e12
 PEAK_FP64 = 78.6e12
@@ -384,9 +385,9 @@ def product_desc(table):
     return desc
 
 
-def ordered_problem(K, table, stream_ptr):
+def ordered_problem(K, table, stream_ptr, levels=(DROP_LOG2,)):
     """The solver's inner operator: the points in Morton order (as the mixed
-    solver arranges them) and a screening plan at the default drop level."""
+    solver arranges them) and a screening plan at each given drop level."""
     import torch
     from mpla_amd import _native
     desc = product_desc(table)
@@ -399,39 +400,57 @@ def ordered_problem(K, table, stream_ptr):
     otable = torch.tensor([pts[d].data_ptr() for d in range(DIM)], dtype=torch.int64, device="cuda")
     odesc = product_desc(otable)
     pb = K.mplak_screen_plan_bytes(C.byref(odesc))
-    plan = torch.empty(pb // 8 + 1, dtype=torch.float64, device="cuda")
-    _native.check(K.mplak_screen_plan_build(C.byref(odesc), _native.MPLAK_PREC_F32, DROP_LOG2, plan.data_ptr(), pb,
-                                            stream_ptr))
+    plans = []
+    for level in levels:
+        plan = torch.empty(pb // 8 + 1, dtype=torch.float64, device="cuda")
+        _native.check(K.mplak_screen_plan_build(C.byref(odesc), _native.MPLAK_PREC_F32, level, plan.data_ptr(), pb,
+                                                stream_ptr))
+        plans.append(plan)
     torch.cuda.synchronize()
-    return {"desc": odesc, "plan": plan, "plan_bytes": pb, "keep": (pts, otable, perm)}
+    return {"desc": odesc, "plan": plans[0], "plans": plans, "levels": levels, "plan_bytes": pb,
+            "keep": (pts, otable, perm)}
 
 
 def measure_inner_product(K, table, xv, yv, stream_ptr, ext, reps=20):
     """Launch time (HIP events on the library stream) of the planned screened
-    FP32 product, the kernel that dominates a mixed solve, and its executed
+    FP32 product, the kernel that dominates a mixed solve, at each of the
+    solver's relaxed screens, launched as the solver does (one plan per
+    launch, with the single-reduction CG's dot partials), and its executed
     work: the plan's count of evaluated entries (256 per kept 16 x 16 tile on
     compact target tiles, 8192 per kept 128 x 64 block on the others), 15 FLOP
     each (13 + the second accumulation) off the diagonal tiles, 13 on them."""
     import torch
+    op = ordered_problem(K, table, stream_ptr, RELAX_LEVELS)
+    out = [measure_level(K, op, lvl, plan, xv, yv, stream_ptr, ext, reps) for lvl, plan in zip(op["levels"],
+                                                                                             op["plans"])]
+    return out[0], out
+
+
+def measure_level(K, op, level, plan, xv, yv, stream_ptr, ext, reps):
+    import torch
     from mpla_amd import _native
-    op = ordered_problem(K, table, stream_ptr)
-    d, plan, pb = op["desc"], op["plan"], op["plan_bytes"]
+    d, pb = op["desc"], op["plan_bytes"]
+    nd = K.mplak_planned_dots_count(N)
+    dots = torch.empty(2 * nd, dtype=torch.float64, device="cuda")
+    dl = (C.c_double * 1)(level)
+    pl = (C.c_void_p * 1)(plan.data_ptr())
 
     def launch():
-        _native.check(K.mplak_kernel_matvec_planned(C.byref(d), _native.MPLAK_PREC_F32, DROP_LOG2, plan.data_ptr(), pb,
-                                                    xv.data_ptr(), yv.data_ptr(), None, stream_ptr))
+        _native.check(K.mplak_kernel_matvec_planned_relaxed(C.byref(d), 1, dl, pl, pb, None, None, xv.data_ptr(),
+                                                            yv.data_ptr(), dots.data_ptr(), None, stream_ptr))
 
     for _ in range(2):
         launch()
-    # one event pair per launch (kernel + finish), averaged: the launch's own
-    # duration, without the gaps between launches
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
-    for e0, e1 in ev:
-        e0.record(ext)
+    # one event pair around `reps` back-to-back launches (kernel + finish), as
+    # the solver issues them: the average launch duration with the short gaps
+    # between the two kernels but no event packets inside the timed span
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(ext)
+    for _ in range(reps):
         launch()
-        e1.record(ext)
-    ev[-1][1].synchronize()
-    t = sum(e0.elapsed_time(e1) for e0, e1 in ev) / reps * 1e-3
+    e1.record(ext)
+    e1.synchronize()
+    t = e0.elapsed_time(e1) / reps * 1e-3
     counts = torch.zeros(5, dtype=torch.int64, device="cuda")
     _native.check(K.mplak_screen_plan_counts(C.byref(d), plan.data_ptr(), pb, counts.data_ptr(), stream_ptr))
     torch.cuda.synchronize()
@@ -439,7 +458,7 @@ def measure_inner_product(K, table, xv, yv, stream_ptr, ext, reps=20):
     nt = (N + 511) // 512
     flop = evals_off * (FLOP_PER_ENTRY + 2.0) + evals_diag * FLOP_PER_ENTRY
     return {"kernel": "kmv_sym_planned_kernel<GAUSSIAN, 3> + kmv_sym_finish_kernel",
-            "launch_ms": t * 1e3, "drop_log2": DROP_LOG2, "kept_pairs": kept_pairs,
+            "launch_ms": t * 1e3, "drop_log2": level, "kept_pairs": kept_pairs,
             "all_pairs": nt * (nt + 1) // 2, "kept_blocks": kept, "all_blocks": 32 * nt * (nt + 1) // 2,
             "evaluated_entries": evals_off + evals_diag,
             "executed_flop": flop, "executed_tflops": flop / t / 1e12,
@@ -449,7 +468,8 @@ def measure_inner_product(K, table, xv, yv, stream_ptr, ext, reps=20):
 def profile_kernels(K, table, stream_ptr):
     """Launch the measured kernels a fixed number of times, for counter runs:
     3 FP32 and 2 FP64 exact Gaussian products (N=65536), 3 planned screened
-    FP32 products on the ordered points, 3 dgemv (65536 x 65536) and
+    FP32 products on the ordered points at each relaxed screen (2^-17, then
+    2^-12, then 2^-8), 3 dgemv (65536 x 65536) and
     3 copies of 2^27 doubles (1 GiB read + 1 GiB written, the traffic
     calibration)."""
     import torch
@@ -465,11 +485,16 @@ def profile_kernels(K, table, stream_ptr):
             _native.check(K.mplak_kernel_matvec(C.byref(desc), prec, xv.data_ptr(), yv.data_ptr(), work.data_ptr(),
                                                 wb, None, stream_ptr))
     torch.cuda.synchronize()
-    op = ordered_problem(K, table, stream_ptr)
-    for _ in range(3):
-        _native.check(K.mplak_kernel_matvec_planned(C.byref(op["desc"]), _native.MPLAK_PREC_F32, DROP_LOG2,
-                                                    op["plan"].data_ptr(), op["plan_bytes"], xv.data_ptr(),
-                                                    yv.data_ptr(), None, stream_ptr))
+    op = ordered_problem(K, table, stream_ptr, RELAX_LEVELS)
+    nd = K.mplak_planned_dots_count(N)
+    dots = torch.empty(2 * nd, dtype=torch.float64, device="cuda")
+    for level, plan in zip(op["levels"], op["plans"]):
+        dl = (C.c_double * 1)(level)
+        pl = (C.c_void_p * 1)(plan.data_ptr())
+        for _ in range(3):
+            _native.check(K.mplak_kernel_matvec_planned_relaxed(C.byref(op["desc"]), 1, dl, pl, op["plan_bytes"], None,
+                                                                None, xv.data_ptr(), yv.data_ptr(), dots.data_ptr(),
+                                                                None, stream_ptr))
     torch.cuda.synchronize()
     del op
     log("products done")
@@ -637,9 +662,10 @@ def main():
     log(f"kernel matvec: FP32 {t32 * 1e3:.3f} ms ({entries / t32:.3e} entries/s), "
         f"FP64 {t64 * 1e3:.3f} ms ({entries / t64:.3e} entries/s)")
     del work
-    inner = measure_inner_product(K, table, xv, yv, stream_ptr, ext)
-    log(f"inner product (ordered, screened at 2^-{DROP_LOG2:g}, planned): {inner['launch_ms']:.3f} ms, "
-        f"{inner['kept_blocks']} of {inner['all_blocks']} blocks, {inner['executed_tflops']:.1f} TFLOP/s executed")
+    inner, inner_levels = measure_inner_product(K, table, xv, yv, stream_ptr, ext)
+    for lv in inner_levels:
+        log(f"inner product (ordered, screened at 2^-{lv['drop_log2']:g}, planned): {lv['launch_ms']:.3f} ms, "
+            f"{lv['evaluated_entries']:.3e} entries, {lv['executed_tflops']:.1f} TFLOP/s executed")
 
     # ---- dense dgemv at N = 65536 (34 GB, far above the 256 MB MALL) ---------
     dgemv = None
@@ -731,9 +757,10 @@ def main():
                      "kernel": inner["kernel"], "launch_ms": inner["launch_ms"],
                      "basis": "FLOP of the evaluated entries (the plan's count: 256 per kept 16x16 tile, 8192 per "
                               "kept 128x64 block of non-compact tiles; 15 FLOP each off the diagonal tiles, 13 on "
-                              "them) per launch / launch time (HIP events, library stream)",
+                              "them) per launch / launch time (HIP events, library stream), at the solver's base "
+                              "screen 2^-17; the relaxed screens 2^-12 and 2^-8 are in inner_product_levels",
                      "frac_of_nonpacked_fp32": inner["executed_tflops"] * 1e12 / PEAK_FP32_NONPACKED,
-                     "inner_product": inner},
+                     "inner_product": inner, "inner_product_levels": inner_levels},
         "roofline_exact_fp32": {"bound": "valu", "achieved": ach32 / 1e12, "peak": PEAK_FP32 / 1e12,
                                 "unit": "TFLOP/s", "frac": ach32 / PEAK_FP32,
                                 "kernel": "kmv_sym_kernel<float, GAUSSIAN, 3> + kmv_sym_finish_kernel",

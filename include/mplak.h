@@ -207,11 +207,12 @@ int mplak_kernel_matvec_planned_relaxed(const mplak_kernel_desc* desc, int nlev,
 int mplak_screen_plan_counts(const mplak_kernel_desc* desc, const void* plan, size_t plan_bytes, int64_t* counts,
                              mplak_stream_t stream);
 
-/* No reference counterpart.  Spatial (Z-order / Morton) permutation of the
+/* No reference counterpart.  Spatial (Hilbert-curve) permutation of the
  * points p[d][offset + i], i < n: perm[k] is the index i of the k-th point
  * when the points' bounding box is cut into 2^(30/dim) cells per axis and the
- * cells are visited in Morton order; points in one cell keep their index
- * order.  Deterministic.  n < 2^31. */
+ * cells are visited along the Hilbert curve (Skilling's transpose form, axis
+ * 0 the most significant within each level); points in one cell keep their
+ * index order.  Deterministic.  1 <= dim <= 16, n < 2^31. */
 size_t mplak_spatial_order_workspace_bytes(int64_t n, int32_t dim);
 int mplak_spatial_order(const double* const* points, int32_t dim, int64_t offset, int64_t n, int64_t* perm,
                         void* work, size_t work_bytes, mplak_stream_t stream);

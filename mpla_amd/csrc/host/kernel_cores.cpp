@@ -132,7 +132,7 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
   const int64_t n = x->cur_proc_row_count;
   const double n_global = (double)x->vec_row_count;
   // Inner solves stop at a relative residual eta >= MPLA_IR_INNER_TOL
-  // (default 1e-4).  The solve runs on the points sorted along a Morton curve
+  // (default 1e-4).  The solve runs on the points sorted along a Hilbert curve
   // (b, x permuted in, x permuted back out), so that the FP32 inner products
   // can skip blocks of entries below 2^-drop: MPLA_IR_DROP_LOG2, by default
   // 3 bits below the inner tolerance (17 for 1e-4); <= 0 keeps every entry.

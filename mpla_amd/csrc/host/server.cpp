@@ -336,7 +336,7 @@ bool admit(mpla_server* s, Request* r) {
                           hipMemcpyHostToDevice, s->stream),
            "server admit");
   } else {
-    // spatial (Morton) order of the request's points, so that the screened
+    // spatial (Hilbert) order of the request's points, so that the screened
     // products can skip far blocks; the right-hand sides follow the points
     const size_t ob = mplak_spatial_order_workspace_bytes(n, dim);
     if (ob == 0) fail("server admit", "spatial order workspace query failed");

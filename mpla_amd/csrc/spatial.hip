@@ -3,11 +3,18 @@
 // The matrix-free products touch every (target, source) pair; when the points
 // are stored in spatial order, consecutive blocks of points are compact
 // clusters, and a screened product can skip whole blocks whose entries are
-// negligible.  The order is the Z-order (Morton) curve over a uniform grid on
-// the points' bounding box, with 30 / dim bits per axis; points in one grid
-// cell keep their index order.  Sorting uses rocPRIM's device radix sort on
-// 64-bit keys (cell code << 32 | index) over the cell-code bits only; the
-// sort is stable, so the permutation is fully determined by the points.
+// negligible.  The order is the Hilbert curve over a uniform grid on the
+// points' bounding box, with 30 / dim bits per axis; points in one grid cell
+// keep their index order.  Unlike the Z-order (Morton) curve, the Hilbert
+// curve never jumps: consecutive cells are neighbours, so every run of
+// consecutive points is one connected, compact cluster (a Z-order run that
+// crosses a quadrant or octant boundary spans two distant regions, and its
+// bounding box covers both).  The cell index along the curve is Skilling's
+// transpose form (J. Skilling, "Programming the Hilbert curve", AIP Conf.
+// Proc. 707, 381, 2004), bit-interleaved.  Sorting uses rocPRIM's device
+// radix sort on 64-bit keys (cell index << 32 | point index) over the
+// cell-index bits only; the sort is stable, so the permutation is fully
+// determined by the points.
 
 #include <algorithm>
 
@@ -62,10 +69,51 @@ __device__ __forceinline__ uint64_t spread_bits(uint64_t v, int bits, int dim, i
   return out;
 }
 
-__global__ __launch_bounds__(kBlock) void morton_key_kernel(const double* const* __restrict__ points, int dim,
-                                                            int64_t offset, int64_t n,
-                                                            const double* __restrict__ part, int nparts,
-                                                            uint64_t* __restrict__ keys) {
+// Hilbert index of the cell with coordinates X[0..D) (b bits each):
+// Skilling's axes-to-transpose map (undo the excess work of the recursive
+// rotations and reflections, then Gray-encode), after which interleaving the
+// bits of X[0], X[1], ... from the top gives the index along the curve.
+template <int D>
+__device__ __forceinline__ uint64_t hilbert_index(uint32_t (&X)[D], int dim, int b) {
+  const uint32_t M = 1u << (b - 1);
+  for (uint32_t Q = M; Q > 1; Q >>= 1) {
+    const uint32_t P = Q - 1;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      if (i >= dim) break;
+      if (X[i] & Q) {
+        X[0] ^= P;
+      } else {
+        const uint32_t t = (X[0] ^ X[i]) & P;
+        X[0] ^= t;
+        X[i] ^= t;
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 1; i < D; ++i)
+    if (i < dim) X[i] ^= X[i - 1];
+  uint32_t t = 0, last = 0;
+#pragma unroll
+  for (int i = 0; i < D; ++i)
+    if (i == dim - 1) last = X[i];
+  for (uint32_t Q = M; Q > 1; Q >>= 1)
+    if (last & Q) t ^= Q - 1;
+#pragma unroll
+  for (int i = 0; i < D; ++i)
+    if (i < dim) X[i] ^= t;
+  uint64_t code = 0;
+#pragma unroll
+  for (int i = 0; i < D; ++i)
+    if (i < dim) code |= spread_bits(X[i], b, dim, dim - 1 - i);
+  return code;
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void hilbert_key_kernel(const double* const* __restrict__ points, int dim,
+                                                             int64_t offset, int64_t n,
+                                                             const double* __restrict__ part, int nparts,
+                                                             uint64_t* __restrict__ keys) {
   __shared__ double box[2 * kMaxDim];
   if (threadIdx.x < dim) {
     double lo = INFINITY, hi = -INFINITY;
@@ -80,13 +128,17 @@ __global__ __launch_bounds__(kBlock) void morton_key_kernel(const double* const*
   const int bits = 30 / dim;
   const double cells = (double)(1u << bits);
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-    uint64_t code = 0;
-    for (int d = 0; d < dim; ++d) {
+    uint32_t X[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      X[d] = 0;
+      if (d >= dim) continue;
       const double lo = box[d], ext = box[kMaxDim + d] - lo;
       double c = ext > 0.0 ? floor((points[d][offset + i] - lo) / ext * cells) : 0.0;
       c = fmin(fmax(c, 0.0), cells - 1.0);
-      code |= spread_bits((uint64_t)c, bits, dim, d);
+      X[d] = (uint32_t)c;
     }
+    const uint64_t code = hilbert_index<D>(X, dim, bits);
     keys[i] = (code << 32) | (uint64_t)i;
   }
 }
@@ -178,8 +230,19 @@ int mplak_spatial_order(const double* const* points, int32_t dim, int64_t offset
   hipLaunchKernelGGL(box_partial_kernel, dim3(nb), dim3(kBlock), 0, hs, points, dim, offset, n, part);
   int st = launch_status();
   if (st != MPLAK_OK) return st;
-  hipLaunchKernelGGL(morton_key_kernel, dim3(grid_for(n)), dim3(kBlock), 0, hs, points, dim, offset, n, part, nb,
-                     kin);
+  // dims 1-3 with their own register arrays; larger dims share one of kMaxDim
+  if (dim == 1)
+    hipLaunchKernelGGL(hilbert_key_kernel<1>, dim3(grid_for(n)), dim3(kBlock), 0, hs, points, dim, offset, n, part,
+                       nb, kin);
+  else if (dim == 2)
+    hipLaunchKernelGGL(hilbert_key_kernel<2>, dim3(grid_for(n)), dim3(kBlock), 0, hs, points, dim, offset, n, part,
+                       nb, kin);
+  else if (dim == 3)
+    hipLaunchKernelGGL(hilbert_key_kernel<3>, dim3(grid_for(n)), dim3(kBlock), 0, hs, points, dim, offset, n, part,
+                       nb, kin);
+  else
+    hipLaunchKernelGGL(hilbert_key_kernel<kMaxDim>, dim3(grid_for(n)), dim3(kBlock), 0, hs, points, dim, offset, n,
+                       part, nb, kin);
   st = launch_status();
   if (st != MPLAK_OK) return st;
   size_t tb = L.tmp_bytes;

@@ -52,6 +52,45 @@ def morton_perm_numpy(pts):
     return np.argsort(code, kind="stable")
 
 
+def hilbert_perm_numpy(pts):
+    """Hilbert-curve order over 2^(30/dim) cells per axis of the bounding box,
+    ties by index: Skilling's axes-to-transpose map (AIP Conf. Proc. 707,
+    2004), then the bits of X[0], X[1], ... interleaved from the top."""
+    dim, n = pts.shape
+    bits = 30 // dim
+    cells = float(1 << bits)
+    X = []
+    for d in range(dim):
+        lo, hi = pts[d].min(), pts[d].max()
+        ext = hi - lo
+        c = np.floor((pts[d] - lo) / ext * cells) if ext > 0 else np.zeros(n)
+        X.append(np.clip(c, 0.0, cells - 1.0).astype(np.int64))
+    M = 1 << (bits - 1)
+    Q = M
+    while Q > 1:
+        P = Q - 1
+        for i in range(dim):
+            inv = (X[i] & Q) != 0
+            t = np.where(inv, 0, (X[0] ^ X[i]) & P)
+            X[0] = np.where(inv, X[0] ^ P, X[0] ^ t)
+            if i > 0:
+                X[i] = X[i] ^ t
+        Q >>= 1
+    for i in range(1, dim):
+        X[i] = X[i] ^ X[i - 1]
+    t = np.zeros(n, dtype=np.int64)
+    Q = M
+    while Q > 1:
+        t = np.where((X[dim - 1] & Q) != 0, t ^ (Q - 1), t)
+        Q >>= 1
+    X = [x ^ t for x in X]
+    code = np.zeros(n, dtype=np.uint64)
+    for b in range(bits - 1, -1, -1):
+        for i in range(dim):
+            code = (code << np.uint64(1)) | ((X[i] >> b) & 1).astype(np.uint64)
+    return np.argsort(code, kind="stable")
+
+
 def spatial_order(lib, T, dim, n, offset=0):
     import torch
     perm = torch.empty(n, dtype=torch.int64, device="cuda")
@@ -62,15 +101,46 @@ def spatial_order(lib, T, dim, n, offset=0):
     return perm
 
 
-@pytest.mark.parametrize("dim,n", [(1, 1), (1, 1000), (2, 4097), (3, 20000), (4, 777)])
-def test_spatial_order_matches_numpy_morton(lib, oracle, dim, n):
+@pytest.mark.parametrize("dim,n", [(1, 1), (1, 1000), (2, 4097), (3, 20000), (4, 777), (7, 300)])
+def test_spatial_order_matches_numpy_hilbert(lib, oracle, dim, n):
     pts = oracle.uniform(dim * n, 300 + dim, 0, -2.0, 3.0).reshape(dim, n)
     pts[:, n // 2] = pts[:, 0]  # duplicate point: tie broken by index
     P = dev(pts)
     T = point_table(P)
     perm = host(spatial_order(lib, T, dim, n))
     assert np.array_equal(np.sort(perm), np.arange(n))
-    assert np.array_equal(perm, morton_perm_numpy(pts))
+    assert np.array_equal(perm, hilbert_perm_numpy(pts))
+    if dim == 1:  # the one-dimensional Hilbert curve is the line itself
+        assert np.array_equal(perm, np.argsort(pts[0], kind="stable"))
+
+
+@pytest.mark.parametrize("dim,m", [(2, 16), (3, 8)])
+def test_spatial_order_is_a_continuous_curve(lib, dim, m):
+    """Points at the centres of an m^dim lattice of dyadic blocks of the grid
+    (plus one point at the far corner that fixes the bounding box): the order
+    visits every lattice point once, each step to a lattice neighbour (one
+    axis, distance one), as a Hilbert curve must; the Z-order jumps."""
+    bits = 30 // dim
+    step = (1 << bits) // m
+    axes = np.meshgrid(*[np.arange(m)] * dim, indexing="ij")
+    lat = np.stack([a.ravel() for a in axes]).astype(np.float64)
+    pts = np.concatenate([lat * step + step // 2, np.full((dim, 1), float(1 << bits))], axis=1)
+    order = rng_permutation(pts.shape[1], 77)
+    pts = pts[:, order].copy()  # the curve, not the input order, decides
+    P = dev(pts)
+    T = point_table(P)
+    perm = host(spatial_order(lib, T, dim, pts.shape[1]))
+    corner = pts.shape[1] - 1
+    seq = [int(i) for i in perm if order[i] != corner]
+    cells = np.rint((pts[:, seq] - step // 2) / step).astype(np.int64)
+    steps = np.abs(np.diff(cells, axis=1)).sum(0)
+    assert len(seq) == m ** dim and np.all(steps == 1)
+    mz = (pts[:, morton_perm_numpy(pts)] - step // 2) / step
+    assert np.abs(np.diff(np.rint(mz[:, :-1]), axis=1)).sum(0).max() > 1
+
+
+def rng_permutation(n, seed):
+    return np.random.default_rng(seed).permutation(n)
 
 
 def test_spatial_order_offset_and_degenerate_axis(lib, oracle):
@@ -80,7 +150,7 @@ def test_spatial_order_offset_and_degenerate_axis(lib, oracle):
     P = dev(pts)
     T = point_table(P)
     perm = host(spatial_order(lib, T, 3, n, offset=off))
-    assert np.array_equal(perm, morton_perm_numpy(pts[:, off:]))
+    assert np.array_equal(perm, hilbert_perm_numpy(pts[:, off:]))
 
 
 def test_gather_scatter_points_round_trip(lib, oracle):
@@ -116,6 +186,9 @@ def screened(lib, desc, X, prec, drop):
 
 
 def ordered_points(lib, oracle, n, dim, seed):
+    """Points in Z-order: spatially ordered, with a few tiles that straddle
+    octant boundaries (non-compact), so the products' vector-unit form for
+    such tiles runs beside the matrix-core form."""
     pts = oracle.uniform(dim * n, seed, 0).reshape(dim, n)
     return pts[:, morton_perm_numpy(pts)].copy()
 
