@@ -58,9 +58,14 @@ void apply_kernel(const mplak_kernel_desc& d, int precision, const double* x, do
 size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
 
 // Drop level of the FP64 residual products of the mixed solver on ordered
-// points: an entry below 2^-80 of the kernel's maximum changes no FP64 row
-// sum of realistic size.
-constexpr double kResidualDropLog2 = 80.0;
+// points, for n points: the at most n entries below 2^-(53 + log2 n) of a row
+// change it by at most 2^-53 max|x|, a unit in the last place of the largest
+// term (entries are at most 1), so the product is the FP64 product.  Between
+// 2^-60 and 2^-80.
+double residual_drop_log2(int64_t n) {
+  const double d = 53.0 + std::ceil(std::log2((double)std::max<int64_t>(n, 1)));
+  return std::min(80.0, std::max(60.0, d));
+}
 // Loosest relaxed screen of the mixed inner products (entries below 2^-6 =
 // 1/64 are never dropped) and tightest screen the safeguard goes to.
 constexpr double kMinRelaxDrop = 6.0;
@@ -135,7 +140,7 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
   // (default 1e-4).  The solve runs on the points sorted along a Hilbert curve
   // (b, x permuted in, x permuted back out), so that the FP32 inner products
   // can skip blocks of entries below 2^-drop: MPLA_IR_DROP_LOG2, by default
-  // 3 bits below the inner tolerance (17 for 1e-4); <= 0 keeps every entry.
+  // one bit below the inner tolerance (15 for 1e-4); <= 0 keeps every entry.
   // A refinement step that gains much less than eta (the screen costs
   // contraction) tightens the screen by 6 bits for the next inner solve.
   // Within an inner solve the products are relaxed as its residual falls
@@ -144,11 +149,11 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
   // product error may grow like ||r_0|| / ||r_k||): the screen drops 5 bits
   // more once ||r_k|| <= 2^-5 ||r_0|| and 9 bits more once ||r_k|| <=
   // 2^-9 ||r_0|| (MPLA_IR_RELAX=0 keeps one level).  The FP64 residual
-  // products keep every entry above 2^-80 (the rest cannot move an FP64 row
-  // sum), so the final residual is the true one whatever the inner operator
-  // dropped.
+  // products keep every entry above 2^-(53 + log2 n) (the rest change a row
+  // by less than an FP64 ulp of its largest term), so the final residual is
+  // the true one whatever the inner operator dropped.
   const double eta_floor = env_double("MPLA_IR_INNER_TOL", 1.0e-4);
-  double drop = env_double("MPLA_IR_DROP_LOG2", std::ceil(-std::log2(eta_floor)) + 3.0);
+  double drop = env_double("MPLA_IR_DROP_LOG2", std::ceil(-std::log2(eta_floor)) + 1.0);
   const bool relax = env_double("MPLA_IR_RELAX", 1.0) != 0.0;
   mplak_kernel_desc shape = desc0;
   shape.row_offset = shape.col_offset = 0;
@@ -271,11 +276,9 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
       ck(mplak_fill_f64(z, n, 0.0, s), "mixed CG");
     } else {
       if (ordered) {
-        // entries below 2^-80 change each row by at most n 2^-80 max|x|,
-        // far below the rounding error of the FP64 sum itself
         const size_t rwb = mplak_kernel_matvec_screened_workspace_bytes(&desc);
-        ck(mplak_kernel_matvec_screened(&desc, MPLAK_PREC_F64, kResidualDropLog2, xs, z, arena(inst, kCore, rwb), rwb,
-                                        nullptr, s),
+        ck(mplak_kernel_matvec_screened(&desc, MPLAK_PREC_F64, residual_drop_log2(n), xs, z, arena(inst, kCore, rwb),
+                                        rwb, nullptr, s),
            "mixed CG residual product");
       } else {
         apply_kernel(desc, MPLAK_PREC_F64, xs, z, nullptr, inst, "mixed CG residual product");

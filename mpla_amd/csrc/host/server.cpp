@@ -412,9 +412,11 @@ mpla_server* mpla_server_create(const mpla_server_config* cfg) {
   for (int k = s->cfg.max_active - 1; k >= 0; --k) s->free_slots.push_back(k);
   ensure_stage(s, size_t(4) << 20);
   if (s->cfg.mode == MPLA_SERVE_MIXED_BATCHED) {
-    // as the single-system mixed solver: FP32 inner products drop entries
-    // 6 bits below the inner tolerance floor (MPLA_IR_DROP_LOG2 overrides;
-    // <= 0 keeps every entry), FP64 products entries below 2^-80
+    // FP32 inner products drop entries 6 bits below the inner tolerance floor
+    // (MPLA_IR_DROP_LOG2 overrides; <= 0 keeps every entry): a wider margin
+    // than the single-system solver's 3 bits, since the served requests get
+    // no screen-tightening safeguard; FP64 products drop entries below 2^-80
+    // (at most 2^-53 max|x| per row for any n <= 2^27)
     const double floor = s->cfg.inner_tol_floor > 0.0 ? s->cfg.inner_tol_floor : 1e-4;
     s->inner_drop = env_or("MPLA_IR_DROP_LOG2", std::ceil(-std::log2(floor)) + 6.0);
     if (s->inner_drop > 0.0) s->residual_drop = 80.0;

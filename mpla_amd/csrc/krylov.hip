@@ -190,16 +190,42 @@ __global__ __launch_bounds__(kBlock) void cg1_update_kernel(int64_t n, double* _
     return;
   }
   __shared__ double scratch[kBlock / kWave];
+  // The vector loads of the usual launch shape (every element of this thread
+  // in one pass) are issued before the partial sums are reduced, so the two
+  // latency chains overlap; larger vectors finish in the strided loop.
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool first = s_in.k == 0;
+  double rv[kVecPerThread], wv[kVecPerThread], pv[kVecPerThread], sv[kVecPerThread], xv[kVecPerThread];
+#pragma unroll
+  for (int u = 0; u < kVecPerThread; ++u) {
+    const int64_t i = i0 + u * stride;
+    const bool in_range = i < n;
+    rv[u] = in_range ? r[i] : 0.0;
+    wv[u] = in_range ? w[i] : 0.0;
+    pv[u] = in_range && !first ? p[i] : 0.0;
+    sv[u] = in_range && !first ? s[i] : 0.0;
+    xv[u] = in_range ? x[i] : 0.0;
+  }
   const double gam = reduce_partials<kBlock>(dots, nd, scratch);
   const double del = reduce_partials<kBlock>(dots + nd, nd, scratch);
   const double rel = stop_value(s_in.rule, gam, s_in.bnorm2, s_in.n_global);
   const bool conv = rel < s_in.eps;
-  const bool first = s_in.k == 0;
   const double beta = first ? 0.0 : gam / s_in.rr;
   const double alpha = first ? gam / del : gam / (del - beta * gam / s_in.alpha);
   if (!conv) {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+#pragma unroll
+    for (int u = 0; u < kVecPerThread; ++u) {
+      const int64_t i = i0 + u * stride;
+      if (i >= n) break;
+      const double pi = first ? rv[u] : fma(beta, pv[u], rv[u]);
+      const double si = first ? wv[u] : fma(beta, sv[u], wv[u]);
+      p[i] = pi;
+      s[i] = si;
+      x[i] = fma(alpha, pi, xv[u]);
+      r[i] = fma(-alpha, si, rv[u]);
+    }
+    for (int64_t i = i0 + kVecPerThread * stride; i < n; i += stride) {
       const double ri = r[i], wi = w[i];
       const double pi = first ? ri : fma(beta, p[i], ri);
       const double si = first ? wi : fma(beta, s[i], wi);
