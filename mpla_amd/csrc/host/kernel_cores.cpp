@@ -150,8 +150,9 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
   // more once ||r_k|| <= 2^-5 ||r_0|| and 9 bits more once ||r_k|| <=
   // 2^-9 ||r_0|| (MPLA_IR_RELAX=0 keeps one level).  The FP64 residual
   // products keep every entry above 2^-(53 + log2 n) (the rest change a row
-  // by less than an FP64 ulp of its largest term), so the final residual is
-  // the true one whatever the inner operator dropped.
+  // by less than an FP64 ulp of its largest term; screened per 16 x 16 tile
+  // with a plan of their own), so the final residual is the true one
+  // whatever the inner operator dropped.
   const double eta_floor = env_double("MPLA_IR_INNER_TOL", 1.0e-4);
   double drop = env_double("MPLA_IR_DROP_LOG2", std::ceil(-std::log2(eta_floor)) + 1.0);
   const bool relax = env_double("MPLA_IR_RELAX", 1.0) != 0.0;
@@ -179,7 +180,7 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
   if (ordered && relax)
     while (nlev < 3 && drop - kRelaxBits[nlev] >= kMinRelaxDrop) ++nlev;
   const size_t order_bytes =
-      ordered ? 3 * vb + (size_t)dim * vb + 256 + align256(ob) + (size_t)nlev * align256(pb) : 0;
+      ordered ? 3 * vb + (size_t)dim * vb + 256 + align256(ob) + (size_t)(nlev + 1) * align256(pb) : 0;
   char* base =
       static_cast<char*>(arena(inst, kMixed, 4 * vb + sb + wb + align256(dwb) + 256 + order_bytes + cg1_bytes));
   double* r = reinterpret_cast<double*>(base);
@@ -198,6 +199,7 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
   const double* bs = b->data;  // right-hand side in solver order
   int64_t* perm = nullptr;
   void* plans[3] = {nullptr, nullptr, nullptr};
+  void* plan64 = nullptr;  // FP64 tile-screened plan of the residual products
   double drops[3] = {drop, drop, drop};
   auto build_plans = [&]() {
     for (int l = 0; l < nlev; ++l) {
@@ -218,6 +220,7 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
     double** table = reinterpret_cast<double**>(ob0 + 3 * vb + (size_t)dim * vb);
     void* owork = ob0 + 3 * vb + (size_t)dim * vb + 256;
     for (int l = 0; l < nlev; ++l) plans[l] = ob0 + 3 * vb + (size_t)dim * vb + 256 + align256(ob) + l * align256(pb);
+    plan64 = ob0 + 3 * vb + (size_t)dim * vb + 256 + align256(ob) + nlev * align256(pb);
     ck(mplak_spatial_order(desc0.points, dim, desc0.row_offset, n, perm, owork, ob, s), "mixed CG spatial order");
     const int64_t ld = (int64_t)(vb / sizeof(double));
     ck(mplak_gather_points(desc0.points, dim, desc0.row_offset, perm, n, pts, ld, s), "mixed CG point gather");
@@ -232,6 +235,8 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
     xs = xp;
     bs = bp;
     build_plans();
+    ck(mplak_screen_plan_build(&desc, MPLAK_PREC_F64, residual_drop_log2(n), plan64, pb, s),
+       "mixed CG residual plan");
   }
   auto unpermute = [&]() {
     if (ordered) ck(mplak_scatter_f64(x->data, xs, perm, n, s), "mixed CG permute back");
@@ -276,9 +281,7 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
       ck(mplak_fill_f64(z, n, 0.0, s), "mixed CG");
     } else {
       if (ordered) {
-        const size_t rwb = mplak_kernel_matvec_screened_workspace_bytes(&desc);
-        ck(mplak_kernel_matvec_screened(&desc, MPLAK_PREC_F64, residual_drop_log2(n), xs, z, arena(inst, kCore, rwb),
-                                        rwb, nullptr, s),
+        ck(mplak_kernel_matvec_planned(&desc, MPLAK_PREC_F64, residual_drop_log2(n), plan64, pb, xs, z, nullptr, s),
            "mixed CG residual product");
       } else {
         apply_kernel(desc, MPLAK_PREC_F64, xs, z, nullptr, inst, "mixed CG residual product");

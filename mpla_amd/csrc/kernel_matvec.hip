@@ -857,6 +857,217 @@ __global__ __launch_bounds__(kBlock) void kmv_sym_planned_kernel(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Planned FP64 product (plan built for FP64, no coincidence shift): the FP32
+// kernel's 16 x 16 tile screen and wave layout, with the entries from FP64
+// coordinate differences and the table-driven FP64 exponential on the vector
+// unit.  Lane 16 r + c of wave w evaluates, for each kept tile (a, b) of
+// source group g, targets 16 (4 a + w) + 4 r + v (v < 4) against source
+// 64 g + 16 b + c: the row sums stay per lane (accR[a][v]) and are reduced
+// over the 16 lanes of the lane row at the end, the column sums of a group
+// over the wave's 4 lane rows after the group, then over the 4 waves in wave
+// order.  Compared with the 128 x 64 block screen of the vector-unit form,
+// the tile screen evaluates about a quarter fewer pairs at FP64 drop levels
+// on spatially ordered points.  The difference form needs no centring, so
+// every tile takes it, compact or not.
+
+// LDS layout (doubles): target coordinates [3][512], source coordinates
+// [3][512], target values [512], source values [512], the exp2 table; the
+// column sums of the four waves reuse the coordinates.
+constexpr int kT64Tc = 0;
+constexpr int kT64Sc = 3 * kSymTile;
+constexpr int kT64Xt = 6 * kSymTile;
+constexpr int kT64Xs = 7 * kSymTile;
+constexpr int kT64Tab = 8 * kSymTile;
+constexpr int kT64Doubles = 8 * kSymTile + kExp2Table;
+
+// Reduce-scatter over the 16 lanes of a lane row: v[q] (q < 32) of every lane
+// in, out[h] = sum over the row's lanes of v[2 c + h] for lane c.  Round k
+// pairs lane c with c ^ 8, c ^ 4, c ^ 2, c ^ 1 and keeps the half of the
+// values selected by the lane's bit 3, 2, 1, 0, adding the partner's copy.
+// Fixed order.
+__device__ __forceinline__ void row16_reduce_scatter_f64(const double (&v)[32], int c16, double (&out)[2]) {
+  const bool b3 = (c16 & 8) != 0, b2 = (c16 & 4) != 0, b1 = (c16 & 2) != 0, b0 = (c16 & 1) != 0;
+  double r1[16], r2[8], r3[4];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) r1[q] = (b3 ? v[q + 16] : v[q]) + __shfl_xor(b3 ? v[q] : v[q + 16], 8);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) r2[q] = (b2 ? r1[q + 8] : r1[q]) + __shfl_xor(b2 ? r1[q] : r1[q + 8], 4);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) r3[q] = (b1 ? r2[q + 4] : r2[q]) + __shfl_xor(b1 ? r2[q] : r2[q + 4], 2);
+#pragma unroll
+  for (int q = 0; q < 2; ++q) out[q] = (b0 ? r3[q + 2] : r3[q]) + __shfl_xor(b0 ? r3[q] : r3[q + 2], 1);
+}
+
+// Reduce-scatter over the four 16-lane rows of the wave (FP64): lane 16 r + c
+// gets the sum over r' of v[r] of lane 16 r' + c; rows paired by r ^ 1, then
+// r ^ 2.
+__device__ __forceinline__ double rows4_reduce_scatter_f64(const double (&v)[4], int r4) {
+  const bool b0 = (r4 & 1) != 0, b1 = (r4 & 2) != 0;
+  double t[2];
+#pragma unroll
+  for (int q = 0; q < 2; ++q) t[q] = (b0 ? v[2 * q + 1] : v[2 * q]) + __shfl_xor(b0 ? v[2 * q] : v[2 * q + 1], 16);
+  return (b1 ? t[1] : t[0]) + __shfl_xor(b1 ? t[0] : t[1], 32);
+}
+
+template <int KIND, int DIM>
+__device__ __forceinline__ void sym_tile64_pair(const double* const* __restrict__ points, int64_t off, int64_t n,
+                                                double scale, const double* __restrict__ x,
+                                                double* __restrict__ part_i, double* __restrict__ part_j, int64_t p,
+                                                int64_t I, int64_t J, const uint32_t* __restrict__ tmask,
+                                                double* lds) {
+  const int tid = threadIdx.x, lane = tid & (kWave - 1);
+  const int w = __builtin_amdgcn_readfirstlane(tid / kWave);
+  const int c16 = lane & 15, r4 = lane >> 4;
+  double* tc = lds + kT64Tc;
+  double* sc = lds + kT64Sc;
+  double* xt = lds + kT64Xt;
+  double* xs = lds + kT64Xs;
+  double* tab = lds + kT64Tab;
+  // staging: thread t takes points t and t + 256 of both tiles; every global
+  // load is issued before the LDS writes (clamped indices, no branches)
+  constexpr int kPer = kSymTile / kBlock;
+  double pc[2][kPer][3], pv[2][kPer];
+#pragma unroll
+  for (int side = 0; side < 2; ++side) {  // 0: sources (tile J), 1: targets (tile I)
+#pragma unroll
+    for (int it = 0; it < kPer; ++it) {
+      const int64_t j = (side == 0 ? J : I) * kSymTile + tid + it * kBlock;
+      const int64_t jc = j < n ? j : n - 1;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) pc[side][it][d] = global_load(points[d] + off + jc);
+      pv[side][it] = global_load(x + jc);
+    }
+  }
+#pragma unroll
+  for (int side = 0; side < 2; ++side) {
+#pragma unroll
+    for (int it = 0; it < kPer; ++it) {
+      const int k = tid + it * kBlock;
+      const int64_t j = (side == 0 ? J : I) * kSymTile + k;
+      double* cd = side == 0 ? sc : tc;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) cd[d * kSymTile + k] = d < DIM ? pc[side][it][d] * scale : 0.0;
+      (side == 0 ? xs : xt)[k] = j < n ? pv[side][it] : 0.0;
+    }
+  }
+  fill_exp2_table(tab);
+  uint32_t gmask[kSymGroups];
+  const uint32_t* tm = tmask + p * kTmaskWords + w * kSymGroups;
+#pragma unroll
+  for (int g = 0; g < kSymGroups; ++g) gmask[g] = (uint32_t)__builtin_amdgcn_readfirstlane((int)tm[g]);
+  __syncthreads();
+
+  double accR[8][4];
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) accR[a][v] = 0.0;
+  double tg[kSymGroups];
+#pragma unroll
+  for (int g = 0; g < kSymGroups; ++g) tg[g] = 0.0;
+  const bool diag = I == J;
+  auto group_mask = [&](int g) -> uint32_t {
+    uint32_t m = 0;
+#pragma unroll
+    for (int q = 0; q < kSymGroups; ++q)
+      if (q == g) m = gmask[q];
+    return m;
+  };
+#pragma unroll 1
+  for (int g = 0; g < kSymGroups; ++g) {
+    const uint32_t gm = group_mask(g);
+    if (gm == 0) continue;
+    double q[4][3], xq[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int k = g * kWave + 16 * b + c16;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) q[b][d] = sc[d * kSymTile + k];
+      xq[b] = xs[k];
+    }
+    double accC[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      // bit b: tile (a, b) kept (bit 8 b + a of gm)
+      const uint32_t cm = ((gm >> a) & 1u) | ((gm >> (7 + a)) & 2u) | ((gm >> (14 + a)) & 4u) | ((gm >> (21 + a)) & 8u);
+      if (cm == 0) continue;
+      const int t0 = 16 * (4 * a + w) + 4 * r4;
+      double pt[4][3], xp[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) pt[v][d] = tc[d * kSymTile + t0 + v];
+        xp[v] = xt[t0 + v];
+      }
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        if (((cm >> b) & 1u) == 0) continue;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const double d0 = pt[v][0] - q[b][0];
+          double sq = d0 * d0;
+          if (DIM > 1) {
+            const double d1 = pt[v][1] - q[b][1];
+            sq = fma(d1, d1, sq);
+          }
+          if (DIM > 2) {
+            const double d2 = pt[v][2] - q[b][2];
+            sq = fma(d2, d2, sq);
+          }
+          const double e = kernel_entry<double, KIND>(sq, tab);
+          accR[a][v] = fma(e, xq[b], accR[a][v]);
+          accC[b] = fma(e, xp[v], accC[b]);
+        }
+      }
+    }
+    const double cs = rows4_reduce_scatter_f64(accC, r4);  // source 16 r + c of the group
+#pragma unroll
+    for (int q2 = 0; q2 < kSymGroups; ++q2)
+      if (q2 == g) tg[q2] = cs;
+  }
+  // row sums: lane c of lane row r ends with v[2 c + h] = accR[a][v] for
+  // 4 a + v = 2 c + h, target 16 (4 a + w) + 4 r + v
+  double rv[32], out[2];
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) rv[4 * a + v] = accR[a][v];
+  row16_reduce_scatter_f64(rv, c16, out);
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int qq = 2 * c16 + h;
+    const int k = 16 * (4 * (qq >> 2) + w) + 4 * r4 + (qq & 3);
+    part_i[p * kSymTile + k] = out[h];
+  }
+  if (!diag) {
+    __syncthreads();  // every wave is done reading the staged tiles
+#pragma unroll
+    for (int g = 0; g < kSymGroups; ++g) lds[w * kSymTile + g * kWave + lane] = tg[g];
+    __syncthreads();
+    for (int k = tid; k < kSymTile; k += kBlock) {
+      double sum = lds[k];
+#pragma unroll
+      for (int q2 = 1; q2 < kBlock / kWave; ++q2) sum += lds[q2 * kSymTile + k];
+      part_j[p * kSymTile + k] = sum;
+    }
+  }
+}
+
+template <int KIND, int DIM>
+__global__ __launch_bounds__(kBlock) void kmv_sym_planned64_kernel(
+    const double* const* __restrict__ points, int64_t off, int64_t n, double scale, const double* __restrict__ x,
+    double* __restrict__ part_i, double* __restrict__ part_j, const uint32_t* __restrict__ list,
+    const int64_t* __restrict__ count, const uint32_t* __restrict__ tmask, const int32_t* __restrict__ skip) {
+  if (skip != nullptr && *skip != 0) return;
+  if ((int64_t)blockIdx.x >= *count) return;
+  __shared__ __attribute__((aligned(16))) double buf[kT64Doubles];
+  const int64_t p = (int64_t)list[blockIdx.x];
+  int64_t I, J;
+  decode_pair(p, I, J);
+  sym_tile64_pair<KIND, DIM>(points, off, n, scale, x, part_i, part_j, p, I, J, tmask, buf);
+}
+
 // y_i = sum over partner tiles P of row i's tile X of that pair's partial,
 // plus the nugget term.  A workgroup takes 64 consecutive rows (one tile) and
 // four partner slices: wave k sums the partners P = k (mod 4) in ascending
@@ -973,6 +1184,14 @@ template <typename T, int KIND, int DIM, int MODE>
 void launch_sym_dim(hipStream_t st, bool coinc, const mplak_kernel_desc* d, const SymArgs& a) {
   const int64_t nt = sym_tiles(d->row_count);
   const dim3 grid((unsigned)(nt * (nt + 1) / 2));
+  if constexpr (MODE == kSymPlanned && sizeof(T) == 8) {
+    if (a.lv != nullptr && !coinc) {
+      hipLaunchKernelGGL((kmv_sym_planned64_kernel<KIND, DIM>), grid, dim3(kBlock), 0, st, d->points, d->row_offset,
+                         d->row_count, a.scale, a.x, a.pi, a.pj, a.lv->list[0], a.lv->count[0], a.lv->tmask[0],
+                         a.skip);
+      return;
+    }
+  }
   if constexpr (MODE == kSymPlanned && sizeof(T) == 4) {
     if (a.lv != nullptr && !coinc) {
       float* fi = reinterpret_cast<float*>(a.pi);
@@ -1037,9 +1256,11 @@ int run_sym(const mplak_kernel_desc* d, int precision, int mode, double drop_s, 
   const bool coinc = d->coincident_shift != 0.0;
   // planned FP32 without coincidence shift: the planned kernel, FP32 partials
   const bool tiles = mode == kSymPlanned && precision == MPLAK_PREC_F32 && lv != nullptr && !coinc;
+  // planned FP64 without coincidence shift: the FP64 tile form
+  const bool tiles64 = mode == kSymPlanned && precision == MPLAK_PREC_F64 && lv != nullptr && !coinc;
   if (mode == kSymPlanned && lv == nullptr) return MPLAK_ERR_ARG;
   if (!tiles && lv != nullptr && lv->nlev > 1) return MPLAK_ERR_ARG;
-  a.lv = tiles ? lv : nullptr;
+  a.lv = tiles || tiles64 ? lv : nullptr;
   a.scale = coord_scale(d->kind, d->length_scale);
   a.thr = (1e-14 * a.scale) * (1e-14 * a.scale);
   a.drop_s = drop_s;
@@ -1227,7 +1448,9 @@ __global__ __launch_bounds__(kBlock) void plan_tile_kernel(const double* __restr
     const T half = T(0.5) * (hi[d] - lo[d]);
     r2 = fma(half, half, r2);
   }
-  const bool compact = sizeof(T) == 4 && r2 <= (T)kCenterRadius2;
+  // FP32: the matrix-core form's centring bound; FP64: the tile form has no
+  // such bound and takes every tile
+  const bool compact = sizeof(T) == 8 || r2 <= (T)kCenterRadius2;
   tinfo[t] = make_float4((float)(T(0.5) * (lo[0] + hi[0])), (float)(T(0.5) * (lo[1] + hi[1])),
                          (float)(T(0.5) * (lo[2] + hi[2])), compact ? 1.0f : 0.0f);
 }

@@ -11,7 +11,10 @@ FP64, bitwise the screened product, also when the plan is reused for new
 vectors.  In FP32 the planned product takes the squared distances of compact
 tiles from the matrix cores (tile-centred form), so it agrees with the
 screened product row by row within 2e-5 of sum_j |K_ij x_j| and with the FP64
-oracle within the same bar, and it is bitwise reproducible.
+oracle within the same bar, and it is bitwise reproducible.  In FP64 the
+planned product screens per 16 x 16 tile as well, with the entries from FP64
+coordinate differences: with nothing dropped it is the exact product to 1e-12
+of sum_j |K_ij x_j| row by row, and it is bitwise reproducible.
 """
 import ctypes as C
 import math
@@ -290,7 +293,11 @@ def test_planned_product_matches_screened(lib, oracle, kind, l, dim, n, drop):
             yp = planned(lib, d, prec, drop, plan, wb, X)
             ys = screened(lib, d, X, prec, drop)
             if prec == 0:
-                assert np.array_equal(yp, ys)
+                # FP64: 16 x 16 tile screen against 128 x 64 blocks; the
+                # entries only one of them drops are below 2^-drop
+                mass = screened(lib, d, dev(np.abs(x)), 0, drop)
+                assert np.all(np.abs(yp - ys) <= n * 2.0 ** -drop * np.abs(x).max() + 1e-12 * mass)
+                assert np.array_equal(planned(lib, d, prec, drop, plan, wb, X), yp)
             else:
                 mass = screened(lib, d, dev(np.abs(x)), 0, drop)
                 assert np.all(np.abs(yp - ys) <= 2e-5 * mass)
@@ -603,3 +610,32 @@ def test_planned_relaxed_levels(lib, oracle):
                                                        X.data_ptr(), y.data_ptr(), None, None, stream()) == bad
     assert lib.mplak_kernel_matvec_planned_relaxed(C.byref(d), 2, dl, plans, wb, None, below, X.data_ptr(),
                                                    y.data_ptr(), None, None, stream()) == bad
+
+
+@pytest.mark.parametrize("kind,l,dim,n,order", [(0, 0.05, 3, 20000, "hilbert"), (1, 0.1, 3, 7000, "morton"),
+                                                (0, 0.02, 2, 9000, "hilbert"), (0, 0.01, 1, 5000, "none"),
+                                                (1, 0.1, 3, 511, "hilbert"), (0, 0.05, 3, 13000, "none")])
+def test_planned_fp64_tile_form_is_the_exact_product(lib, oracle, kind, l, dim, n, order):
+    """FP64 planned product with nothing dropped (2^-2000): the tile form's
+    result equals the exact symmetric product to 1e-12 of sum_j |K_ij x_j|
+    in every row, and the FP64 oracle on sampled rows to the same bar; on
+    ordered and unordered points, dims 1-3, both kernels, ragged sizes."""
+    pts = oracle.uniform(dim * n, 110 + dim, 0).reshape(dim, n)
+    if order == "hilbert":
+        pts = pts[:, hilbert_perm_numpy(pts)].copy()
+    elif order == "morton":
+        pts = pts[:, morton_perm_numpy(pts)].copy()
+    P = dev(pts)
+    T = point_table(P)
+    d = kernel_desc(P, T, kind, l, nugget=0.5)
+    x = oracle.uniform(n, 111, 1, -1, 1)
+    X = dev(x)
+    plan, wb = build_plan(lib, d, 0, 2000.0)
+    yp = planned(lib, d, 0, 2000.0, plan, wb, X)
+    ye = host(kmv_gpu(lib, d, X))
+    mass = host(kmv_gpu(lib, d, dev(np.abs(x))))
+    assert np.all(np.abs(yp - ye) <= 1e-12 * mass)
+    k = oracle.Kernel(pts, kind, l, nugget=0.5)
+    rows = np.arange(0, n, max(1, n // 97))
+    ref = np.array([k.matvec(x, mode=1, row_off=int(i), rows=1)[0] for i in rows])
+    assert np.all(np.abs(yp[rows] - ref) <= 1e-12 * mass[rows])
