@@ -881,22 +881,20 @@ constexpr int kT64Xs = 7 * kSymTile;
 constexpr int kT64Tab = 8 * kSymTile;
 constexpr int kT64Doubles = 8 * kSymTile + kExp2Table;
 
-// Reduce-scatter over the 16 lanes of a lane row: v[q] (q < 32) of every lane
-// in, out[h] = sum over the row's lanes of v[2 c + h] for lane c.  Round k
-// pairs lane c with c ^ 8, c ^ 4, c ^ 2, c ^ 1 and keeps the half of the
-// values selected by the lane's bit 3, 2, 1, 0, adding the partner's copy.
-// Fixed order.
-__device__ __forceinline__ void row16_reduce_scatter_f64(const double (&v)[32], int c16, double (&out)[2]) {
+// Reduce-scatter over the 16 lanes of a lane row (FP64): v[q] (q < 16) of
+// every lane in, the row's sum of v[c] out for lane c.  Round k pairs lane c
+// with c ^ 8, c ^ 4, c ^ 2, c ^ 1 and keeps the half of the values selected
+// by the lane's bit 3, 2, 1, 0, adding the partner's copy.  Fixed order.
+__device__ __forceinline__ double row16_reduce_scatter_f64(const double (&v)[16], int c16) {
   const bool b3 = (c16 & 8) != 0, b2 = (c16 & 4) != 0, b1 = (c16 & 2) != 0, b0 = (c16 & 1) != 0;
-  double r1[16], r2[8], r3[4];
+  double r1[8], r2[4], r3[2];
 #pragma unroll
-  for (int q = 0; q < 16; ++q) r1[q] = (b3 ? v[q + 16] : v[q]) + __shfl_xor(b3 ? v[q] : v[q + 16], 8);
+  for (int q = 0; q < 8; ++q) r1[q] = (b3 ? v[q + 8] : v[q]) + __shfl_xor(b3 ? v[q] : v[q + 8], 8);
 #pragma unroll
-  for (int q = 0; q < 8; ++q) r2[q] = (b2 ? r1[q + 8] : r1[q]) + __shfl_xor(b2 ? r1[q] : r1[q + 8], 4);
+  for (int q = 0; q < 4; ++q) r2[q] = (b2 ? r1[q + 4] : r1[q]) + __shfl_xor(b2 ? r1[q] : r1[q + 4], 4);
 #pragma unroll
-  for (int q = 0; q < 4; ++q) r3[q] = (b1 ? r2[q + 4] : r2[q]) + __shfl_xor(b1 ? r2[q] : r2[q + 4], 2);
-#pragma unroll
-  for (int q = 0; q < 2; ++q) out[q] = (b0 ? r3[q + 2] : r3[q]) + __shfl_xor(b0 ? r3[q] : r3[q + 2], 1);
+  for (int q = 0; q < 2; ++q) r3[q] = (b1 ? r2[q + 2] : r2[q]) + __shfl_xor(b1 ? r2[q] : r2[q + 2], 2);
+  return (b0 ? r3[1] : r3[0]) + __shfl_xor(b0 ? r3[0] : r3[1], 1);
 }
 
 // Reduce-scatter over the four 16-lane rows of the wave (FP64): lane 16 r + c
@@ -958,15 +956,12 @@ __device__ __forceinline__ void sym_tile64_pair(const double* const* __restrict_
   for (int g = 0; g < kSymGroups; ++g) gmask[g] = (uint32_t)__builtin_amdgcn_readfirstlane((int)tm[g]);
   __syncthreads();
 
-  double accR[8][4];
-#pragma unroll
-  for (int a = 0; a < 8; ++a)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) accR[a][v] = 0.0;
+  // Two passes over the source groups, each for half of the wave's target
+  // slabs (a = 4 h + a'), so that only 16 row sums stay in registers; the
+  // column sums of a group add the two passes in order.
   double tg[kSymGroups];
 #pragma unroll
   for (int g = 0; g < kSymGroups; ++g) tg[g] = 0.0;
-  const bool diag = I == J;
   auto group_mask = [&](int g) -> uint32_t {
     uint32_t m = 0;
 #pragma unroll
@@ -975,72 +970,77 @@ __device__ __forceinline__ void sym_tile64_pair(const double* const* __restrict_
     return m;
   };
 #pragma unroll 1
-  for (int g = 0; g < kSymGroups; ++g) {
-    const uint32_t gm = group_mask(g);
-    if (gm == 0) continue;
-    double q[4][3], xq[4];
+  for (int h = 0; h < 2; ++h) {
+    double accR[4][4];
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int k = g * kWave + 16 * b + c16;
+    for (int a = 0; a < 4; ++a)
 #pragma unroll
-      for (int d = 0; d < 3; ++d) q[b][d] = sc[d * kSymTile + k];
-      xq[b] = xs[k];
-    }
-    double accC[4] = {0.0, 0.0, 0.0, 0.0};
+      for (int v = 0; v < 4; ++v) accR[a][v] = 0.0;
+    // bits 8 b + a of a group mask for this pass's slabs a = 4 h .. 4 h + 3
+    const uint32_t half_bits = 0x0f0f0f0fu << (4 * h);
+#pragma unroll 1
+    for (int g = 0; g < kSymGroups; ++g) {
+      const uint32_t gm = group_mask(g);
+      if ((gm & half_bits) == 0) continue;
+      double accC[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int a = 0; a < 8; ++a) {
-      // bit b: tile (a, b) kept (bit 8 b + a of gm)
-      const uint32_t cm = ((gm >> a) & 1u) | ((gm >> (7 + a)) & 2u) | ((gm >> (14 + a)) & 4u) | ((gm >> (21 + a)) & 8u);
-      if (cm == 0) continue;
-      const int t0 = 16 * (4 * a + w) + 4 * r4;
-      double pt[4][3], xp[4];
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-#pragma unroll
-        for (int d = 0; d < 3; ++d) pt[v][d] = tc[d * kSymTile + t0 + v];
-        xp[v] = xt[t0 + v];
-      }
-#pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        if (((cm >> b) & 1u) == 0) continue;
+      for (int a2 = 0; a2 < 4; ++a2) {
+        const int a = 4 * h + a2;
+        // bit b: tile (a, b) kept (bit 8 b + a of gm)
+        const uint32_t cm =
+            ((gm >> a) & 1u) | ((gm >> (7 + a)) & 2u) | ((gm >> (14 + a)) & 4u) | ((gm >> (21 + a)) & 8u);
+        if (cm == 0) continue;
+        const int t0 = 16 * (4 * a + w) + 4 * r4;
+        double pt[4][3], xp[4];
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          const double d0 = pt[v][0] - q[b][0];
-          double sq = d0 * d0;
-          if (DIM > 1) {
-            const double d1 = pt[v][1] - q[b][1];
-            sq = fma(d1, d1, sq);
+#pragma unroll
+          for (int d = 0; d < 3; ++d) pt[v][d] = tc[d * kSymTile + t0 + v];
+          xp[v] = xt[t0 + v];
+        }
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          if (((cm >> b) & 1u) == 0) continue;
+          const int k = g * kWave + 16 * b + c16;
+          double q[3];
+#pragma unroll
+          for (int d = 0; d < 3; ++d) q[d] = sc[d * kSymTile + k];
+          const double xq = xs[k];
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const double d0 = pt[v][0] - q[0];
+            double sq = d0 * d0;
+            if (DIM > 1) {
+              const double d1 = pt[v][1] - q[1];
+              sq = fma(d1, d1, sq);
+            }
+            if (DIM > 2) {
+              const double d2 = pt[v][2] - q[2];
+              sq = fma(d2, d2, sq);
+            }
+            const double e = kernel_entry<double, KIND>(sq, tab);
+            accR[a2][v] = fma(e, xq, accR[a2][v]);
+            accC[b] = fma(e, xp[v], accC[b]);
           }
-          if (DIM > 2) {
-            const double d2 = pt[v][2] - q[b][2];
-            sq = fma(d2, d2, sq);
-          }
-          const double e = kernel_entry<double, KIND>(sq, tab);
-          accR[a][v] = fma(e, xq[b], accR[a][v]);
-          accC[b] = fma(e, xp[v], accC[b]);
         }
       }
+      const double cs = rows4_reduce_scatter_f64(accC, r4);  // source 16 r + c of the group
+#pragma unroll
+      for (int q2 = 0; q2 < kSymGroups; ++q2)
+        if (q2 == g) tg[q2] += cs;
     }
-    const double cs = rows4_reduce_scatter_f64(accC, r4);  // source 16 r + c of the group
+    // row sums of the pass: lane c of lane row r ends with v[c] = accR[a'][v]
+    // for 4 a' + v = c, target 16 (4 (4 h + a') + w) + 4 r + v
+    double rv[16];
 #pragma unroll
-    for (int q2 = 0; q2 < kSymGroups; ++q2)
-      if (q2 == g) tg[q2] = cs;
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) rv[4 * a + v] = accR[a][v];
+    const double out = row16_reduce_scatter_f64(rv, c16);
+    const int k = 16 * (4 * (4 * h + (c16 >> 2)) + w) + 4 * r4 + (c16 & 3);
+    part_i[p * kSymTile + k] = out;
   }
-  // row sums: lane c of lane row r ends with v[2 c + h] = accR[a][v] for
-  // 4 a + v = 2 c + h, target 16 (4 a + w) + 4 r + v
-  double rv[32], out[2];
-#pragma unroll
-  for (int a = 0; a < 8; ++a)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) rv[4 * a + v] = accR[a][v];
-  row16_reduce_scatter_f64(rv, c16, out);
-#pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const int qq = 2 * c16 + h;
-    const int k = 16 * (4 * (qq >> 2) + w) + 4 * r4 + (qq & 3);
-    part_i[p * kSymTile + k] = out[h];
-  }
-  if (!diag) {
+  if (I != J) {
     __syncthreads();  // every wave is done reading the staged tiles
 #pragma unroll
     for (int g = 0; g < kSymGroups; ++g) lds[w * kSymTile + g * kWave + lane] = tg[g];
