@@ -6,9 +6,10 @@ the screened product with nothing to drop is bitwise the exact symmetric
 product; with a drop level of 2^-30 on ordered points every row differs from
 the oracle's exact FP64 product by at most the dropped mass bound
 n * 2^-30 * max|x| plus the FP32 product tolerance; the planned product
-(screening plan built once, all-dropped tile pairs never launched) is, in
-FP64, bitwise the screened product, also when the plan is reused for new
-vectors.  In FP32 the planned product takes the squared distances of compact
+(screening plan built once, all-dropped tile pairs never launched) agrees,
+in FP64, with the screened product within n * 2^-drop * max|x| (the entries
+only one of the two screens drops) plus 1e-12 of sum_j |K_ij x_j|, also
+when the plan is reused for new vectors, and repeats bitwise.  In FP32 the planned product takes the squared distances of compact
 tiles from the matrix cores (tile-centred form), so it agrees with the
 screened product row by row within 2e-5 of sum_j |K_ij x_j| and with the FP64
 oracle within the same bar, and it is bitwise reproducible.  In FP64 the
