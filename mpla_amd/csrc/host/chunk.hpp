@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <functional>
+#include <vector>
 
 #include "runtime.hpp"
 
@@ -37,10 +38,23 @@ class ChunkRunner {
       : inst_(inst), capture_(capture), L_(L), iteration_(std::move(iteration)),
         normalize_(std::move(normalize)) {}
 
+  // With a slot: a graph captured by an earlier call with the same key is
+  // replayed without capturing again, and a new capture is left in the slot
+  // (which owns it) for later calls.  The key must hold every value the
+  // iteration passes to its kernels.
+  ChunkRunner(mpla_instance* inst, bool capture, int L, std::function<void(int)> iteration,
+              std::function<void()> normalize, GraphSlot* slot, std::vector<unsigned char> key)
+      : ChunkRunner(inst, capture, L, std::move(iteration), std::move(normalize)) {
+    slot_ = slot;
+    key_ = std::move(key);
+    if (capture_ && slot_ != nullptr && slot_->exec != nullptr && slot_->key == key_) exec_ = slot_->exec;
+  }
+
   ChunkRunner(const ChunkRunner&) = delete;
   ChunkRunner& operator=(const ChunkRunner&) = delete;
 
   ~ChunkRunner() {
+    if (slot_ != nullptr) return;  // the slot owns the graph
     if (exec_) (void)hipGraphExecDestroy(exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
   }
@@ -81,6 +95,12 @@ class ChunkRunner {
     }
     graph_ = g;
     hip_ck(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0), "hipGraphInstantiate");
+    if (slot_ != nullptr) {
+      slot_->reset();
+      slot_->graph = graph_;
+      slot_->exec = exec_;
+      slot_->key = key_;
+    }
   }
 
   mpla_instance* inst_;
@@ -91,6 +111,8 @@ class ChunkRunner {
   hipGraph_t graph_ = nullptr;
   hipGraphExec_t exec_ = nullptr;
   bool capture_failed_ = false;
+  GraphSlot* slot_ = nullptr;
+  std::vector<unsigned char> key_;
 };
 
 }  // namespace mpla_rt

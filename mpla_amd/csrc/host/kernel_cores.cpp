@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <vector>
 
 #include "../../../include/kernel.h"
 #include "../../../include/mpla_kernel.h"
@@ -272,7 +273,26 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
     ck(mplak_cg_iterate(n, e, r, d, z, in, out, work, wb, s), "mixed CG inner iteration");
   };
   auto normalize = [&]() { ck(mplak_cg_copy_state(st + 1, st, s), "mixed CG state copy"); };
-  ChunkRunner runner(inst, true, choose_chunk(0, verbosity(inst)), iteration, normalize);
+  // On ordered points the captured chunk is kept in the instance and
+  // replayed by later solves whose kernels would get the same arguments (the
+  // same operator shape and workspace), which saves its capture.
+  const int chunk = choose_chunk(0, verbosity(inst));
+  std::vector<unsigned char> key;
+  auto put = [&key](const void* v, size_t bytes) {
+    const unsigned char* c = static_cast<const unsigned char*>(v);
+    key.insert(key.end(), c, c + bytes);
+  };
+  if (ordered) {
+    const void* ptrs[] = {desc.points, plans[0], plans[1], plans[2], st, r, z, d, e, sv, dots, s};
+    put(ptrs, sizeof(ptrs));
+    const int64_t ints[] = {n, nd, (int64_t)pb, (int64_t)nlev, (int64_t)chunk, (int64_t)desc.dim,
+                            (int64_t)desc.kind, desc.row_offset, desc.row_count, desc.col_offset, desc.col_count};
+    put(ints, sizeof(ints));
+    const double reals[] = {desc.length_scale, desc.nugget, desc.coincident_shift, drops[0], drops[1], drops[2],
+                            relax_below[0], relax_below[1]};
+    put(reals, sizeof(reals));
+  }
+  ChunkRunner runner(inst, true, chunk, iteration, normalize, ordered ? &rt->mixed_chunk : nullptr, key);
 
   bool stagnated = false;
   for (;;) {

@@ -196,6 +196,7 @@ void mpla_destroy_instance(struct mpla_instance* instance) {
   if (!instance || !instance->runtime) return;
   Runtime* rt = static_cast<Runtime*>(instance->runtime);
   (void)hipStreamSynchronize(rt->stream);
+  rt->mixed_chunk.reset();
   for (int a = 0; a < kArenaCount; ++a)
     if (rt->arena[a]) (void)hipFree(rt->arena[a]);
   if (rt->pinned) (void)hipHostFree(rt->pinned);

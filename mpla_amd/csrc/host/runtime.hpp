@@ -11,6 +11,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <vector>
 
 #include "../../../include/mpla.h"
 #include "../../../include/mplak.h"
@@ -21,12 +22,28 @@ namespace mpla_rt {
 // its vectors in kSolver while the plugin core it calls uses kCore.
 enum Arena { kCore = 0, kBlas = 1, kSolver = 2, kMixed = 3, kArenaCount = 4 };
 
+// A captured chunk of solver iterations kept between calls: replayed again
+// when every argument its kernels were captured with (the key) is the same.
+struct GraphSlot {
+  std::vector<unsigned char> key;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  void reset() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    key.clear();
+  }
+};
+
 struct Runtime {
   hipStream_t stream = nullptr;
   int device = 0;
   void* arena[kArenaCount] = {};
   size_t arena_bytes[kArenaCount] = {};
   void* pinned = nullptr;  // kPinnedBytes of page-locked host memory
+  GraphSlot mixed_chunk;   // the mixed solver's inner-iteration chunk
 };
 
 constexpr size_t kPinnedBytes = 65536;

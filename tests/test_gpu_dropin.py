@@ -435,6 +435,33 @@ def test_mixed_precision_relaxed_and_safeguarded_screens(s, oracle, monkeypatch,
     assert np.linalg.norm(xs - xo) <= 1e-6 * np.linalg.norm(xo)
 
 
+def test_mixed_precision_reuses_its_graph_for_new_points(s, oracle):
+    """Two mixed solves of the same size and kernel in one instance, on
+    different points and right-hand sides: the second replays the chunk the
+    first captured (same kernel arguments, new contents behind them).  Both
+    reach the FP64 bar on their own system, and a third solve of the first
+    system repeats the first bitwise."""
+    n = 6000
+    sols = []
+    systems = []
+    for seed in (1301, 1302, 1301):
+        pts, bh, k = gaussian_system(oracle, n, 0.05, 1.0, 0, seed=seed)
+        P = Points(pts)
+        kd = mpla.mpla_kernel_matrix_data(P.ptr, 3, 0, 0, mpla.MPLA_PREC_F64, 0.05, 1.0, 0.0)
+        A = s.generic(n, n, C.addressof(kd))
+        b = s.vector(n, bh)
+        x = s.vector(n, np.zeros(n))
+        s.lib.mpla_kernel_conjugate_gradient_mixed(C.byref(b), C.byref(A), C.byref(x), 5000, 1e-8,
+                                                   mpla.MPLA_PREC_F32, s.I)
+        assert s.inst.last_converged == 1
+        xs = s.get(x)
+        assert np.linalg.norm(bh - k.matvec(xs)) / np.linalg.norm(bh) < 1e-8 * 1.01
+        sols.append(xs)
+        systems.append((P, kd, A, b, x))
+    assert np.array_equal(sols[0], sols[2])
+    assert not np.array_equal(sols[0], sols[1])
+
+
 def test_graph_replay_is_deterministic(s, oracle):
     n = 4096
     pts, bh, k = gaussian_system(oracle, n)
