@@ -388,7 +388,7 @@ def product_desc(table):
 
 
 def ordered_problem(K, table, stream_ptr, levels=(DROP_LOG2,)):
-    """The solver's inner operator: the points in Morton order (as the mixed
+    """The solver's inner operator: the points in Hilbert order (as the mixed
     solver arranges them) and a screening plan at each given drop level."""
     import torch
     from mpla_amd import _native
