@@ -43,9 +43,14 @@ void mpla_dgemv_core_kernel_fused(struct mpla_vector* b, struct mpla_generic_mat
  * operator in `inner_precision` (MPLA_PREC_F32; MPLA_PREC_F64 runs plain CG).
  * Iterative refinement: r = b - A64 x (FP64 operator), inner CG on A_low e = r
  * to a relative tolerance, x += e, until the true FP64 residual meets
- * epsilon.  iter_max bounds the total inner iterations.  On exit
- * instance->last_rel_res is the true FP64 relative residual of x and
- * instance->last_iterations the total inner iterations. */
+ * epsilon.  iter_max bounds the total inner iterations.  On a square block
+ * the solve runs on the points in Hilbert-curve order: the inner operator
+ * drops entries below 2^-(bit below the inner tolerance), looser as the
+ * inner residual falls (MPLA_IR_DROP_LOG2, MPLA_IR_RELAX), in single-reduction
+ * CG; the FP64 residual products drop only entries whose sum cannot reach an
+ * ulp of a row's largest term.  On exit instance->last_rel_res is the true
+ * FP64 relative residual of x, instance->last_iterations the total inner
+ * iterations and instance->last_matvecs the operator applications. */
 void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_generic_matrix* A, struct mpla_vector* x, int iter_max, double epsilon, int inner_precision, struct mpla_instance* instance);
 
 /* Solve (K + nugget I) X = B for nrhs right-hand sides: B and X are device
