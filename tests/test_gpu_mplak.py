@@ -53,8 +53,11 @@ def dgemv_gpu(lib, A_dev, lda, m, n, x_dev, skip=None):
     return y
 
 
+# (6000, 6000) and (6001, 6000) hold more than 256 MiB: the kernel then reads A
+# with nontemporal loads, 16 bytes per lane for the even leading dimension and
+# 8 bytes for the odd one.  Every partial sum is an integer below 2^53.
 @pytest.mark.parametrize("m,n", [(1, 1), (3, 2), (127, 129), (128, 64), (1000, 1000), (4096, 300),
-                                 (333, 5000)])
+                                 (333, 5000), (6000, 6000), (6001, 6000)])
 def test_dgemv_test_library_pattern_exact(lib, m, n):
     A = np.arange(m * n, dtype=np.float64)  # column-major A[idx] = idx
     x = np.arange(n, dtype=np.float64)
@@ -65,7 +68,8 @@ def test_dgemv_test_library_pattern_exact(lib, m, n):
 
 
 @pytest.mark.parametrize("m,n,lda", [(513, 700, 513), (512, 700, 515), (4096, 4096, 4096),
-                                     (1, 9000, 1), (9000, 1, 9000)])
+                                     (1, 9000, 1), (9000, 1, 9000), (6000, 6000, 6000),
+                                     (6000, 6000, 6001)])
 def test_dgemv_random_vs_oracle(lib, oracle, m, n, lda):
     A = oracle.uniform(lda * n, 1234, 1, -1.0, 1.0)
     x = oracle.uniform(n, 1234, 2, -1.0, 1.0)
