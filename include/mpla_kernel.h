@@ -47,9 +47,12 @@ void mpla_dgemv_core_kernel_fused(struct mpla_vector* b, struct mpla_generic_mat
  * the solve runs on the points in Hilbert-curve order: the inner operator
  * drops entries below 2^-(bit below the inner tolerance), looser as the
  * inner residual falls (MPLA_IR_DROP_LOG2, MPLA_IR_RELAX), in single-reduction
- * CG; the FP64 residual products drop only entries whose sum cannot reach an
- * ulp of a row's largest term.  On exit instance->last_rel_res is the true
- * FP64 relative residual of x, instance->last_iterations the total inner
+ * CG; the FP64 residual products drop only entries below
+ * 2^-(log2 n + log2(1/nugget) + log2(1/epsilon) + 12), whose sum moves the
+ * residual by at most 2^-12 epsilon ||b|| (K is positive semi-definite; with
+ * nugget <= 0 or a negative coincidence shift they drop only entries below an
+ * FP64 ulp of a row's largest term).  On exit instance->last_rel_res is the
+ * FP64 relative residual of x to that accuracy, instance->last_iterations the total inner
  * iterations and instance->last_matvecs the operator applications. */
 void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_generic_matrix* A, struct mpla_vector* x, int iter_max, double epsilon, int inner_precision, struct mpla_instance* instance);
 

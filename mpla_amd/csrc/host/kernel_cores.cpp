@@ -67,6 +67,23 @@ double residual_drop_log2(int64_t n) {
   const double d = 53.0 + std::ceil(std::log2((double)std::max<int64_t>(n, 1)));
   return std::min(80.0, std::max(60.0, d));
 }
+// Drop level of the FP64 residual products for a solve of (K + nugget I) x =
+// b to relative residual epsilon.  Every dropped entry is below 2^-drop, so a
+// row loses at most 2^-drop ||x||_1 <= 2^-drop sqrt(n) ||x||_2, and the
+// residual's error is at most n 2^-drop ||x||_2 <= n 2^-drop ||b||_2 / nugget
+// (K is positive semi-definite, so ||x|| <= ||b|| / nugget for the solution,
+// which the refined iterates approach).  With drop = log2 n + log2(1/nugget)
+// + log2(1/epsilon) + 12 that error is at most 2^-12 epsilon ||b||_2: the
+// stopping test and the reported residual are the true ones to 2^-12 of the
+// tolerance.  Never tighter than residual_drop_log2(n), where the product is
+// the FP64 product; that level is also taken for nugget <= 0 or epsilon <= 0.
+double residual_screen_log2(int64_t n, double nugget, double epsilon) {
+  const double exact = residual_drop_log2(n);
+  if (!(nugget > 0.0) || !(epsilon > 0.0)) return exact;
+  const double d = std::ceil(std::log2((double)std::max<int64_t>(n, 1)) - std::log2(nugget) -
+                             std::log2(epsilon) + 12.0);
+  return std::min(exact, std::max(24.0, d));
+}
 // Loosest relaxed screen of the mixed inner products (entries below 2^-6 =
 // 1/64 are never dropped) and tightest screen the safeguard goes to.
 constexpr double kMinRelaxDrop = 6.0;
@@ -150,13 +167,16 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
   // product error may grow like ||r_0|| / ||r_k||): the screen drops 5 bits
   // more once ||r_k|| <= 2^-5 ||r_0|| and 9 bits more once ||r_k|| <=
   // 2^-9 ||r_0|| (MPLA_IR_RELAX=0 keeps one level).  The FP64 residual
-  // products keep every entry above 2^-(53 + log2 n) (the rest change a row
-  // by less than an FP64 ulp of its largest term; screened per 16 x 16 tile
-  // with a plan of their own), so the final residual is the true one
-  // whatever the inner operator dropped.
+  // products keep every entry above 2^-residual_screen_log2 (what they leave
+  // out moves the residual by at most 2^-12 epsilon ||b||; screened per
+  // 16 x 16 tile with a plan of their own), so the stopping test and the
+  // final residual are the true ones whatever the inner operator dropped.
   const double eta_floor = env_double("MPLA_IR_INNER_TOL", 1.0e-4);
   double drop = env_double("MPLA_IR_DROP_LOG2", std::ceil(-std::log2(eta_floor)) + 1.0);
   const bool relax = env_double("MPLA_IR_RELAX", 1.0) != 0.0;
+  // a non-negative coincidence shift adds a positive semi-definite block
+  const double res_drop =
+      residual_screen_log2(n, desc0.coincident_shift >= 0.0 ? desc0.nugget : 0.0, epsilon);
   mplak_kernel_desc shape = desc0;
   shape.row_offset = shape.col_offset = 0;
   const size_t pb = mplak_screen_plan_bytes(&shape);
@@ -236,7 +256,7 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
     xs = xp;
     bs = bp;
     build_plans();
-    ck(mplak_screen_plan_build(&desc, MPLAK_PREC_F64, residual_drop_log2(n), plan64, pb, s),
+    ck(mplak_screen_plan_build(&desc, MPLAK_PREC_F64, res_drop, plan64, pb, s),
        "mixed CG residual plan");
   }
   auto unpermute = [&]() {
@@ -301,7 +321,7 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
       ck(mplak_fill_f64(z, n, 0.0, s), "mixed CG");
     } else {
       if (ordered) {
-        ck(mplak_kernel_matvec_planned(&desc, MPLAK_PREC_F64, residual_drop_log2(n), plan64, pb, xs, z, nullptr, s),
+        ck(mplak_kernel_matvec_planned(&desc, MPLAK_PREC_F64, res_drop, plan64, pb, xs, z, nullptr, s),
            "mixed CG residual product");
       } else {
         apply_kernel(desc, MPLAK_PREC_F64, xs, z, nullptr, inst, "mixed CG residual product");
