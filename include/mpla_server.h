@@ -16,7 +16,10 @@
  *                             residuals / dot products / updates (iterative
  *                             refinement), all active requests batched;
  *   MPLA_SERVE_FP64_SERIAL    plain FP64 CG, one request on the GPU at a time.
- * Every result meets its tolerance on the true FP64 relative residual.
+ * Every result meets its tolerance on the FP64 relative residual.  In mixed
+ * mode the FP64 products drop only entries whose sum moves a residual by at
+ * most 2^-12 of the request's tolerance (at most the entries below
+ * 2^-(53 + log2 n), which change no product beyond FP64 rounding).
  */
 #ifndef mpla_server_h__
 #define mpla_server_h__

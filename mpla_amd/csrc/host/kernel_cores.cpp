@@ -58,32 +58,6 @@ void apply_kernel(const mplak_kernel_desc& d, int precision, const double* x, do
 
 size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
 
-// Drop level of the FP64 residual products of the mixed solver on ordered
-// points, for n points: the at most n entries below 2^-(53 + log2 n) of a row
-// change it by at most 2^-53 max|x|, a unit in the last place of the largest
-// term (entries are at most 1), so the product is the FP64 product.  Between
-// 2^-60 and 2^-80.
-double residual_drop_log2(int64_t n) {
-  const double d = 53.0 + std::ceil(std::log2((double)std::max<int64_t>(n, 1)));
-  return std::min(80.0, std::max(60.0, d));
-}
-// Drop level of the FP64 residual products for a solve of (K + nugget I) x =
-// b to relative residual epsilon.  Every dropped entry is below 2^-drop, so a
-// row loses at most 2^-drop ||x||_1 <= 2^-drop sqrt(n) ||x||_2, and the
-// residual's error is at most n 2^-drop ||x||_2 <= n 2^-drop ||b||_2 / nugget
-// (K is positive semi-definite, so ||x|| <= ||b|| / nugget for the solution,
-// which the refined iterates approach).  With drop = log2 n + log2(1/nugget)
-// + log2(1/epsilon) + 12 that error is at most 2^-12 epsilon ||b||_2: the
-// stopping test and the reported residual are the true ones to 2^-12 of the
-// tolerance.  Never tighter than residual_drop_log2(n), where the product is
-// the FP64 product; that level is also taken for nugget <= 0 or epsilon <= 0.
-double residual_screen_log2(int64_t n, double nugget, double epsilon) {
-  const double exact = residual_drop_log2(n);
-  if (!(nugget > 0.0) || !(epsilon > 0.0)) return exact;
-  const double d = std::ceil(std::log2((double)std::max<int64_t>(n, 1)) - std::log2(nugget) -
-                             std::log2(epsilon) + 12.0);
-  return std::min(exact, std::max(24.0, d));
-}
 // Loosest relaxed screen of the mixed inner products (entries below 2^-6 =
 // 1/64 are never dropped) and tightest screen the safeguard goes to.
 constexpr double kMinRelaxDrop = 6.0;

@@ -9,6 +9,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
 #include <cstddef>
 #include <cstdint>
 #include <vector>
@@ -67,6 +69,34 @@ inline void hip_ck(hipError_t e, const char* what) {
 }
 
 inline mplak_stream_t stream_of(mpla_instance* inst) { return runtime(inst)->stream; }
+
+// Drop level at which a screened FP64 kernel product of n points is the FP64
+// product: the at most n entries below 2^-(53 + log2 n) of a row change it by
+// at most 2^-53 max|x|, a unit in the last place of the largest term
+// (entries are at most 1).  Between 2^-60 and 2^-80.
+inline double residual_drop_log2(int64_t n) {
+  const double d = 53.0 + std::ceil(std::log2((double)std::max<int64_t>(n, 1)));
+  return std::min(80.0, std::max(60.0, d));
+}
+
+// Drop level of the FP64 residual products for a solve of (K + nugget I) x =
+// b to relative residual epsilon (per right-hand side).  Every dropped entry
+// is below 2^-drop, so a row loses at most 2^-drop ||x||_1 <= 2^-drop sqrt(n)
+// ||x||_2, and the residual's error is at most n 2^-drop ||x||_2 <=
+// n 2^-drop ||b||_2 / nugget (K is positive semi-definite, so ||x|| <=
+// ||b|| / nugget for the solution, which the refined iterates approach).
+// With drop = log2 n + log2(1/nugget) + log2(1/epsilon) + 12 that error is at
+// most 2^-12 epsilon ||b||_2: the stopping test and the reported residual are
+// the true ones to 2^-12 of the tolerance.  Never tighter than
+// residual_drop_log2(n); that level is also taken for nugget <= 0 or
+// epsilon <= 0.
+inline double residual_screen_log2(int64_t n, double nugget, double epsilon) {
+  const double exact = residual_drop_log2(n);
+  if (!(nugget > 0.0) || !(epsilon > 0.0)) return exact;
+  const double d = std::ceil(std::log2((double)std::max<int64_t>(n, 1)) - std::log2(nugget) -
+                             std::log2(epsilon) + 12.0);
+  return std::min(exact, std::max(24.0, d));
+}
 
 // Copy `bytes` from device to the pinned buffer and wait; returns the buffer.
 void* read_back(mpla_instance* inst, const void* dev, size_t bytes);

@@ -106,6 +106,7 @@ struct Request {
   double *part = nullptr, *cgpart = nullptr;
   int64_t* perm = nullptr;  // spatial order (mixed mode): stored row k is request row perm[k]
   bool ordered = false;
+  double residual_drop = 0.0;  // screen of the FP64 products (mixed mode; 0 = exact)
   double prev_max = INFINITY;
 };
 
@@ -155,7 +156,8 @@ struct mpla_server {
   char* dev_stage = nullptr;
   size_t dev_stage_bytes = 0;
   std::chrono::steady_clock::time_point t0;
-  // drop levels of the screened products (mixed mode; 0 = exact)
+  // drop level of the FP32 inner products (mixed mode; 0 = exact); FP64
+  // products are screened per request when residual_drop > 0
   double inner_drop = 0.0, residual_drop = 0.0;
 
   double now() const {
@@ -328,6 +330,8 @@ bool admit(mpla_server* s, Request* r) {
   const int64_t n = r->req.n;
   const int dim = r->req.dim;
   r->ordered = s->cfg.mode == MPLA_SERVE_MIXED_BATCHED && (s->inner_drop > 0.0 || s->residual_drop > 0.0);
+  r->residual_drop =
+      s->residual_drop > 0.0 ? mpla_rt::residual_screen_log2(n, r->req.nugget, r->req.tolerance) : 0.0;
   if (!r->ordered) {
     hip_ck(hipMemcpyAsync(r->points, r->req.points, sizeof(double) * (size_t)dim * (size_t)n, hipMemcpyHostToDevice,
                           s->stream),
@@ -415,8 +419,9 @@ mpla_server* mpla_server_create(const mpla_server_config* cfg) {
     // FP32 inner products drop entries 6 bits below the inner tolerance floor
     // (MPLA_IR_DROP_LOG2 overrides; <= 0 keeps every entry): a wider margin
     // than the single-system solver's 3 bits, since the served requests get
-    // no screen-tightening safeguard; FP64 products drop entries below 2^-80
-    // (at most 2^-53 max|x| per row for any n <= 2^27)
+    // no screen-tightening safeguard.  FP64 products drop, per request, the
+    // entries whose sum moves a residual by at most 2^-12 of the request's
+    // tolerance (mpla_rt::residual_screen_log2; the FP64-exact level at most)
     const double floor = s->cfg.inner_tol_floor > 0.0 ? s->cfg.inner_tol_floor : 1e-4;
     s->inner_drop = env_or("MPLA_IR_DROP_LOG2", std::ceil(-std::log2(floor)) + 6.0);
     if (s->inner_drop > 0.0) s->residual_drop = 80.0;
@@ -500,7 +505,7 @@ int mpla_server_step(mpla_server* s) {
     Upload u;
     std::vector<mplak_ragged_system> prod;
     for (Request* r : resid)
-      if (!r->x_zero) prod.push_back(product_system(*r, MPLAK_PREC_F64, r->X, r->Z, nullptr, s->residual_drop));
+      if (!r->x_zero) prod.push_back(product_system(*r, MPLAK_PREC_F64, r->X, r->Z, nullptr, r->residual_drop));
     add_products(u, prod, MPLAK_PREC_F64);
     // Z = 0 for requests whose X is still zero (the fill op writes the X field)
     const size_t zb = u.csys.size();
@@ -583,7 +588,7 @@ int mpla_server_step(mpla_server* s) {
       const bool f64 = serial || r->escalated;
       (f64 ? p64 : p32).push_back(product_system(*r, f64 ? MPLAK_PREC_F64 : MPLAK_PREC_F32, r->D, r->Z,
                                                  &s->groups[r->slot].all_done,
-                                                 f64 ? s->residual_drop : s->inner_drop));
+                                                 f64 ? r->residual_drop : s->inner_drop));
     }
     const size_t base = u.csys.size();
     for (Request* r : inner) u.csys.push_back(cg_system(s, *r, r->E, r->X));
