@@ -67,9 +67,12 @@ def test_dgemv_test_library_pattern_exact(lib, m, n):
     assert np.array_equal(y, expect)
 
 
+# Above 256 MiB (nontemporal loads): square, odd leading dimension, a wide
+# block (one row panel, 256 column slabs) and a tall one (one slab, 8-byte
+# loads).
 @pytest.mark.parametrize("m,n,lda", [(513, 700, 513), (512, 700, 515), (4096, 4096, 4096),
                                      (1, 9000, 1), (9000, 1, 9000), (6000, 6000, 6000),
-                                     (6000, 6000, 6001)])
+                                     (6000, 6000, 6001), (100, 700000, 100), (300000, 150, 300001)])
 def test_dgemv_random_vs_oracle(lib, oracle, m, n, lda):
     A = oracle.uniform(lda * n, 1234, 1, -1.0, 1.0)
     x = oracle.uniform(n, 1234, 2, -1.0, 1.0)
