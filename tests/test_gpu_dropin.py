@@ -330,7 +330,10 @@ def test_fused_core_plain_fp64_cg(s, oracle):
     assert np.linalg.norm(xs - xo) <= 1e-7 * np.linalg.norm(xo)
 
 
-@pytest.mark.parametrize("kind,ell,nugget", [(0, 0.05, 1.0), (1, 0.1, 0.5)])
+# The FP64 residual products drop entries below 2^-(log2 n + log2(1/nugget) +
+# log2(1/eps) + 12): the smaller nugget moves that screen by 3.3 bits, and the
+# reported residual must still be the oracle's to 1e-3 of itself.
+@pytest.mark.parametrize("kind,ell,nugget", [(0, 0.05, 1.0), (1, 0.1, 0.5), (0, 0.05, 0.1)])
 def test_mixed_precision_ir_reaches_fp64_residual(s, oracle, kind, ell, nugget):
     n = 8192
     pts, bh, k = gaussian_system(oracle, n, ell, nugget, kind)
