@@ -39,6 +39,8 @@ ITER_MAX = 20001
 FLOP_PER_ENTRY = 3 * DIM + 4  # d subs, d FMA, 1 scale, 1 exp, 1 FMA accumulate
 DROP_LOG2 = 15.0  # the mixed solver's default drop level at its default inner tolerance 1e-4
 RELAX_LEVELS = (15.0, 10.0, 6.0)  # its relaxed inner-product screens (inner residual > 2^-5, > 2^-9, below)
+# its FP64 residual screen for this solve: log2 N + log2(1/nugget) + log2(1/eps) + 12, rounded up (55)
+RESIDUAL_DROP_LOG2 = float(math.ceil(math.log2(N) - math.log2(NUGGET) - math.log2(EPS) + 12.0))
 PEAK_FP32_NONPACKED = 78.6e12  # one FP32 FMA per lane per cycle; the 157.3 TF figure counts packed FP32
 PEAK_FP32 = 157.3e12
 PEAK_FP64 = 78.6e12
@@ -471,9 +473,9 @@ def profile_kernels(K, table, stream_ptr):
     """Launch the measured kernels a fixed number of times, for counter runs:
     3 FP32 and 2 FP64 exact Gaussian products (N=65536), 3 planned screened
     FP32 products on the ordered points at each relaxed screen (2^-15, then
-    2^-10, then 2^-6), 3 dgemv (65536 x 65536) and
-    3 copies of 2^27 doubles (1 GiB read + 1 GiB written, the traffic
-    calibration)."""
+    2^-10, then 2^-6), 2 planned FP64 products at the solver's residual
+    screen (2^-55), 3 dgemv (65536 x 65536) and 3 copies of 2^27 doubles
+    (1 GiB read + 1 GiB written, the traffic calibration)."""
     import torch
     from mpla_amd import _native
     desc = product_desc(table)
@@ -497,8 +499,15 @@ def profile_kernels(K, table, stream_ptr):
             _native.check(K.mplak_kernel_matvec_planned_relaxed(C.byref(op["desc"]), 1, dl, pl, op["plan_bytes"], None,
                                                                 None, xv.data_ptr(), yv.data_ptr(), dots.data_ptr(),
                                                                 None, stream_ptr))
+    plan64 = torch.empty(op["plan_bytes"] // 8 + 1, dtype=torch.float64, device="cuda")
+    _native.check(K.mplak_screen_plan_build(C.byref(op["desc"]), _native.MPLAK_PREC_F64, RESIDUAL_DROP_LOG2,
+                                            plan64.data_ptr(), op["plan_bytes"], stream_ptr))
+    for _ in range(2):
+        _native.check(K.mplak_kernel_matvec_planned(C.byref(op["desc"]), _native.MPLAK_PREC_F64, RESIDUAL_DROP_LOG2,
+                                                    plan64.data_ptr(), op["plan_bytes"], xv.data_ptr(), yv.data_ptr(),
+                                                    None, stream_ptr))
     torch.cuda.synchronize()
-    del op
+    del op, plan64
     log("products done")
     del work
     n = 65536
