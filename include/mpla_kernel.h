@@ -40,7 +40,13 @@ void mpla_dgemv_core_kernel_fused(struct mpla_vector* b, struct mpla_generic_mat
 
 /* Solve (K + nugget I) x = b to ||b - A x||/||b|| < epsilon, with the
  * residual, norms, dot products and updates in FP64 and the inner CG's
- * operator in `inner_precision` (MPLA_PREC_F32; MPLA_PREC_F64 runs plain CG).
+ * operator in `inner_precision` (MPLA_PREC_F32).  MPLA_PREC_F64 runs plain
+ * FP64 CG (the recurrences and relative rule of
+ * mpla_generic_conjugate_gradient_with_checkpoint_restart): on a square block
+ * of up to 3 dimensions without a coincidence shift, on the points in
+ * Hilbert-curve order with the FP64 product screened at 2^-(53 + log2 n),
+ * whose dropped entries change no row by more than an ulp of its largest
+ * term; otherwise with the fused core on every entry.
  * Iterative refinement: r = b - A64 x (FP64 operator), inner CG on A_low e = r
  * to a relative tolerance, x += e, until the true FP64 residual meets
  * epsilon.  iter_max bounds the total inner iterations.  On a square block

@@ -36,7 +36,7 @@ extern "C" {
 
 #define MPLA_REQUEST_PENDING 0
 #define MPLA_REQUEST_CONVERGED 1
-#define MPLA_REQUEST_ITER_LIMIT 2
+#define MPLA_REQUEST_ITER_LIMIT 2 /* stopped unconverged: iteration limit, 40 refinement steps or a non-finite residual */
 #define MPLA_REQUEST_REJECTED (-1) /* cannot fit the budget or exceeds the limits */
 #define MPLA_REQUEST_UNKNOWN (-2)
 

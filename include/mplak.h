@@ -49,6 +49,7 @@ typedef void* mplak_stream_t; /* a hipStream_t; NULL is the default stream */
 #define MPLAK_RUNNING 0
 #define MPLAK_CONVERGED 1
 #define MPLAK_ITER_LIMIT 2
+#define MPLAK_BREAKDOWN 3 /* mplak_cg1_iterate: step length not finite and positive; no update made */
 
 const char* mplak_status_string(int status);
 
@@ -308,7 +309,10 @@ int mplak_cg_iterate(int64_t n, double* x, double* r, double* d, const double* z
  * gamma = <r,r> of the residual entering the iteration (state.rr), so a
  * converged solve has applied the operator once more than it updated; k
  * counts updates, and k == 0 starts the recurrence (p and s need no initial
- * values).  Same state double-buffering as mplak_cg_iterate. */
+ * values).  A step length alpha that is not finite and positive (the
+ * operator is not positive definite on the Krylov space, or a partial sum is
+ * not finite) stops the solve with done = MPLAK_BREAKDOWN before any vector
+ * is updated.  Same state double-buffering as mplak_cg_iterate. */
 int mplak_cg1_iterate(int64_t n, double* x, double* r, double* p, double* s, const double* w, const double* dots,
                       int64_t ndots, const mplak_cg_state* in, mplak_cg_state* out, mplak_stream_t stream);
 

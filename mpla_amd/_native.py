@@ -29,6 +29,7 @@ MPLAK_STOP_RELATIVE = 1
 MPLAK_RUNNING = 0
 MPLAK_CONVERGED = 1
 MPLAK_ITER_LIMIT = 2
+MPLAK_BREAKDOWN = 3
 MPLAK_RAGGED_START = 0
 MPLAK_RAGGED_START_FINAL = 1
 MPLAK_RAGGED_GROUP = 2

@@ -130,6 +130,8 @@ void mpla_kernel_block_conjugate_gradient_mixed(double* B, double* X, int nrhs, 
   double prev_max = INFINITY, max_rel = INFINITY;
   bool all_conv = false;
   const int max_outer = 60;
+  const int max_bf16_steps = (int)env_double("MPLA_IR_BF16_STEPS", 1.0);
+  int bf16_steps = 0;
 
   for (;;) {
     // true FP64 residuals R = B - K X
@@ -153,6 +155,12 @@ void mpla_kernel_block_conjugate_gradient_mixed(double* B, double* X, int nrhs, 
              outer, max_rel, prec, total_inner);
     if (all_conv || total_inner >= iter_max || outer >= max_outer) break;
     if (outer > 0 && max_rel > 0.5 * prev_max) prec = escalate(prec);
+    // BF16 entries bound each refinement step to a gain of about its inner
+    // tolerance floor (1e-2), and every step restarts the Krylov space: after
+    // its first step a BF16 solve costs more per decade than BF16x3 at 1.4x the
+    // product time (measured on the 64-RHS Matern system), so it continues
+    // in BF16x3 (MPLA_IR_BF16_STEPS sets how many steps stay in BF16).
+    if (prec == MPLA_PREC_BF16 && bf16_steps >= max_bf16_steps) prec = MPLA_PREC_BF16X3;
     prev_max = max_rel;
 
     // inner batched CG on K_prec E = R
@@ -198,6 +206,7 @@ void mpla_kernel_block_conjugate_gradient_mixed(double* B, double* X, int nrhs, 
     total_inner += kmax;
     ck(mplak_daxpy_f64(nn, 1.0, E, X, s), "block CG update");
     x_zero = false;
+    if (prec == MPLA_PREC_BF16) ++bf16_steps;
     ++outer;
   }
   if (rel_res) {

@@ -77,6 +77,84 @@ void plain_fp64_cg(mpla_vector* b, mpla_generic_matrix* A, mpla_vector* x, int i
   inst->verbosity = v;
 }
 
+// Operator of the ordered plain FP64 solve: the planned FP64 product on
+// points in Hilbert order with the FP64-exact screen.
+struct PlannedPayload {
+  mplak_kernel_desc desc;
+  void* plan;
+  size_t plan_bytes;
+  double drop;
+};
+
+void planned_fp64_core(mpla_vector* b, mpla_generic_matrix* A, mpla_vector* x, mpla_instance* inst) {
+  const PlannedPayload* p = static_cast<const PlannedPayload*>(A->data);
+  ck(mplak_kernel_matvec_planned(&p->desc, MPLAK_PREC_F64, p->drop, p->plan, p->plan_bytes, x->data, b->data,
+                                 inst->skip_flag, stream_of(inst)),
+     "ordered FP64 CG product");
+}
+
+// Plain FP64 CG (the reference recurrences and relative rule) on the points
+// in Hilbert order, with the FP64 product screened at 2^-(53 + log2 n): the at
+// most n entries it leaves out of a row change that row by at most an ulp of
+// its largest term.  Square blocks of up to 3 dimensions without a
+// coincidence shift; anything else runs plain_fp64_cg.
+void ordered_fp64_cg(mpla_vector* b, mpla_generic_matrix* A, mpla_vector* x, int iter_max, double eps,
+                     mpla_instance* inst) {
+  const mplak_kernel_desc desc0 = fused_desc(A);
+  const int64_t n = x->cur_proc_row_count;
+  mplak_kernel_desc shape = desc0;
+  shape.row_offset = shape.col_offset = 0;
+  const size_t pb = mplak_screen_plan_bytes(&shape);
+  const bool ok = desc0.row_offset == desc0.col_offset && desc0.row_count == desc0.col_count && pb > 0 && n > 0 &&
+                  desc0.coincident_shift == 0.0;
+  if (!ok) {
+    plain_fp64_cg(b, A, x, iter_max, eps, inst);
+    return;
+  }
+  const int dim = desc0.dim;
+  const size_t vb = align256(sizeof(double) * (size_t)n);
+  const size_t ob = mplak_spatial_order_workspace_bytes(n, dim);
+  if (ob == 0) fail("mpla_kernel_conjugate_gradient_mixed", "spatial order workspace query failed");
+  char* base = static_cast<char*>(arena(inst, kMixed, 3 * vb + (size_t)dim * vb + 256 + align256(ob) + align256(pb)));
+  int64_t* perm = reinterpret_cast<int64_t*>(base);
+  double* bp = reinterpret_cast<double*>(base + vb);
+  double* xp = reinterpret_cast<double*>(base + 2 * vb);
+  double* pts = reinterpret_cast<double*>(base + 3 * vb);
+  double** table = reinterpret_cast<double**>(base + 3 * vb + (size_t)dim * vb);
+  void* owork = base + 3 * vb + (size_t)dim * vb + 256;
+  void* plan = base + 3 * vb + (size_t)dim * vb + 256 + align256(ob);
+  mplak_stream_t s = stream_of(inst);
+  Runtime* rt = runtime(inst);
+  ck(mplak_spatial_order(desc0.points, dim, desc0.row_offset, n, perm, owork, ob, s), "ordered FP64 CG order");
+  const int64_t ld = (int64_t)(vb / sizeof(double));
+  ck(mplak_gather_points(desc0.points, dim, desc0.row_offset, perm, n, pts, ld, s), "ordered FP64 CG points");
+  double** host_table = static_cast<double**>(rt->pinned);
+  for (int k = 0; k < dim; ++k) host_table[k] = pts + (size_t)k * ld;
+  hip_ck(hipMemcpyAsync(table, host_table, sizeof(double*) * dim, hipMemcpyHostToDevice, rt->stream),
+         "hipMemcpyAsync");
+  ck(mplak_gather_f64(bp, b->data, perm, n, s), "ordered FP64 CG permute");
+  ck(mplak_gather_f64(xp, x->data, perm, n, s), "ordered FP64 CG permute");
+  PlannedPayload pay;
+  pay.desc = desc0;
+  pay.desc.points = table;
+  pay.desc.row_offset = pay.desc.col_offset = 0;
+  pay.plan = plan;
+  pay.plan_bytes = pb;
+  pay.drop = residual_drop_log2(n);
+  ck(mplak_screen_plan_build(&pay.desc, MPLAK_PREC_F64, pay.drop, plan, pb, s), "ordered FP64 CG plan");
+  mpla_generic_matrix Ap = *A;
+  Ap.data = &pay;
+  mpla_vector bv = vector_view(b, bp);
+  mpla_vector xv = vector_view(x, xp);
+  const int v = inst->verbosity;
+  inst->verbosity = v >= 2 ? v : 0;
+  mpla_generic_conjugate_gradient_with_checkpoint_restart(&bv, &Ap, &xv, iter_max, eps, planned_fp64_core, 0, false,
+                                                          inst);
+  inst->verbosity = v;
+  ck(mplak_scatter_f64(x->data, xp, perm, n, s), "ordered FP64 CG permute back");
+  hip_ck(hipStreamSynchronize(rt->stream), "ordered FP64 CG");
+}
+
 double env_double(const char* name, double dflt) {
   const char* v = getenv(name);
   return v ? atof(v) : dflt;
@@ -118,7 +196,7 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
                                           struct mpla_instance* inst) {
   const mpla_kernel_matrix_data* kd = static_cast<const mpla_kernel_matrix_data*>(A->data);
   if (inner_precision == MPLA_PREC_F64) {
-    plain_fp64_cg(b, A, x, iter_max, epsilon, inst);
+    ordered_fp64_cg(b, A, x, iter_max, epsilon, inst);
     return;
   }
   if (inner_precision != MPLA_PREC_F32)
@@ -154,8 +232,10 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
   mplak_kernel_desc shape = desc0;
   shape.row_offset = shape.col_offset = 0;
   const size_t pb = mplak_screen_plan_bytes(&shape);
+  // the planned relaxed inner product has no coincidence-shift form: shifted
+  // operators take the unordered path (exact FP32 inner products)
   const bool ordered = drop > 0.0 && desc0.row_offset == desc0.col_offset && desc0.row_count == desc0.col_count &&
-                       pb > 0 && n > 0;
+                       pb > 0 && n > 0 && desc0.coincident_shift == 0.0;
   const int dim = desc0.dim;
   const size_t vb = align256(sizeof(double) * (size_t)n);
   const size_t sb = align256(2 * sizeof(mplak_cg_state));
@@ -308,8 +388,10 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
     if (verbosity(inst) >= 2 && inst->is_parent)
       printf("Refinement step %d: true relative residual %le after %d inner iterations\n", outer, cur.rel,
              total_inner);
-    if (!(cur.rel >= epsilon)) break;
-    if (outer > 0 && cur.rel > 0.5 * prev_rel) {
+    if (cur.rel < epsilon) break;
+    // a non-finite residual (non-finite input) or a step that gained less
+    // than 2x: refinement cannot go further with this inner precision
+    if (!std::isfinite(cur.rel) || (outer > 0 && cur.rel > 0.5 * prev_rel)) {
       stagnated = true;
       break;
     }
@@ -343,7 +425,16 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
     total_inner += ist.k;
     // the single-reduction form applies the operator once more than it
     // updates when it converges (the product whose <r,r> meets the test)
-    inner_products += ist.k + (ordered && ist.done == MPLAK_CONVERGED ? 1 : 0);
+    inner_products += ist.k + (ordered && ist.done != MPLAK_ITER_LIMIT ? 1 : 0);
+    // An inner solve that broke down (the screened FP32 operator lost
+    // definiteness: a step length that is not finite and positive) stopped
+    // before that update, so e is its last finite iterate.  A non-finite
+    // inner residual from the classic form leaves e unusable: x keeps its
+    // last finite value and the solve finishes in FP64.
+    if (!std::isfinite(ist.rel) || (ist.done == MPLAK_BREAKDOWN && ist.k == 0)) {
+      stagnated = true;
+      break;
+    }
     ck(mplak_daxpy_f64(n, 1.0, e, xs, s), "mixed CG update");
     x_zero = false;
     ++outer;
@@ -352,14 +443,21 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
   unpermute();
   int iterations = total_inner;
   double rel = cur.rel;
-  bool converged = !(cur.rel >= epsilon);
+  bool converged = cur.rel < epsilon;
   if (stagnated && total_inner < iter_max) {
-    // refinement cannot gain more with this inner precision: finish in FP64
-    plain_fp64_cg(b, A, x, iter_max - total_inner, epsilon, inst);
+    // refinement cannot gain more with this inner precision: finish in plain
+    // FP64 CG, to half the tolerance so that the drift between its recursive
+    // residual and the true one cannot decide the outcome, then report the
+    // true residual b - A x of the exact FP64 product
+    plain_fp64_cg(b, A, x, iter_max - total_inner, 0.5 * epsilon, inst);
     iterations += inst->last_matvecs - 1;
-    matvecs64 += inst->last_matvecs;
-    rel = inst->last_rel_res;
-    converged = inst->last_converged != 0;
+    matvecs64 += inst->last_matvecs + 1;
+    apply_kernel(desc0, MPLAK_PREC_F64, x->data, z, nullptr, inst, "mixed CG final residual");
+    ck(mplak_cg_start(n, b->data, z, r, d, st, MPLAK_STOP_RELATIVE, epsilon, 2, n_global, work, wb, s),
+       "mixed CG final residual");
+    cur = *static_cast<mplak_cg_state*>(read_back(inst, st, sizeof(mplak_cg_state)));
+    rel = cur.rel;
+    converged = rel < epsilon;
   }
   inst->last_iterations = iterations;
   inst->last_converged = converged;

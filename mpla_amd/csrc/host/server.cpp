@@ -42,6 +42,11 @@ inline void ck(int status, const char* what) {
 
 size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
 
+// FP64 residual evaluations after which a request retires as
+// MPLA_REQUEST_ITER_LIMIT whatever its iteration count (as the mixed solver's
+// outer-step limit)
+constexpr int kMaxRefinementSteps = 40;
+
 // Caching device allocator: blocks are rounded to 1 MiB and kept on free
 // lists after use.
 class DevicePool {
@@ -530,14 +535,19 @@ int mpla_server_step(mpla_server* s) {
     for (Request* r : resid) {
       mplak_cg_state* st = s->host_slot_states(r->slot);
       double max_rel = 0.0;
-      bool all = true;
+      bool all = true, finite = true;
       for (int c = 0; c < r->req.nrhs; ++c) {
         max_rel = std::max(max_rel, st[c].rel);
         if (!(st[c].rel < r->req.tolerance)) all = false;
+        if (!std::isfinite(st[c].rel)) finite = false;
       }
+      if (!finite) max_rel = NAN;
       r->res.refinement_steps += 1;
       r->res.max_rel_residual = max_rel;
-      if (all || r->res.iterations >= s->cfg.max_iterations) {
+      // a non-finite residual cannot improve, and refinement steps are capped
+      // so that a request whose inner solves gain nothing still retires
+      if (all || !finite || r->res.iterations >= s->cfg.max_iterations ||
+          r->res.refinement_steps > kMaxRefinementSteps) {
         r->res.status = all ? MPLA_REQUEST_CONVERGED : MPLA_REQUEST_ITER_LIMIT;
         finished.push_back(r);
         continue;
@@ -548,7 +558,7 @@ int mpla_server_step(mpla_server* s) {
       const double floor_eta = f64 ? 0.0 : s->cfg.inner_tol_floor;
       for (int c = 0; c < r->req.nrhs; ++c) {
         mplak_cg_state& h = st[c];
-        if (!(h.rel >= r->req.tolerance)) {
+        if (h.rel < r->req.tolerance) {
           h.done = MPLAK_CONVERGED;
           continue;
         }

@@ -12,6 +12,7 @@
 // a fixed function of n and partial sums are always combined in the same
 // order, so every block recomputes identical alpha/beta values.
 
+#include <float.h>
 #include <math.h>
 
 #include "common.hpp"
@@ -213,7 +214,9 @@ __global__ __launch_bounds__(kBlock) void cg1_update_kernel(int64_t n, double* _
   const bool conv = rel < s_in.eps;
   const double beta = first ? 0.0 : gam / s_in.rr;
   const double alpha = first ? gam / del : gam / (del - beta * gam / s_in.alpha);
-  if (!conv) {
+  // breakdown: <p, A p> = gamma/alpha is not positive (or not finite)
+  const bool broke = !conv && !(alpha > 0.0 && alpha <= DBL_MAX);
+  if (!conv && !broke) {
 #pragma unroll
     for (int u = 0; u < kVecPerThread; ++u) {
       const int64_t i = i0 + u * stride;
@@ -241,6 +244,8 @@ __global__ __launch_bounds__(kBlock) void cg1_update_kernel(int64_t n, double* _
     o.rel = rel;
     if (conv) {
       o.done = MPLAK_CONVERGED;
+    } else if (broke) {
+      o.done = MPLAK_BREAKDOWN;
     } else {
       o.alpha = alpha;
       o.beta = beta;

@@ -199,12 +199,14 @@ __global__ __launch_bounds__(kBlock) void ragged_start_final_kernel(const mplak_
     s.bnorm2 = bb;
     s.eps = S.eps;
     s.n_global = (double)S.n;
-    s.rel = sqrt(rr) / sqrt(bb);
+    // a zero right-hand side with a zero residual is solved exactly (x = 0);
+    // 0/0 would otherwise leave a NaN that never meets the tolerance
+    s.rel = (bb == 0.0 && rr == 0.0) ? 0.0 : sqrt(rr) / sqrt(bb);
     s.alpha = s.beta = 0.0;
     s.k = 0;
     s.iter_max = 2;
     s.rule = MPLAK_STOP_RELATIVE;
-    s.done = !(s.rel >= S.eps) ? MPLAK_CONVERGED : MPLAK_RUNNING;
+    s.done = s.rel < S.eps ? MPLAK_CONVERGED : MPLAK_RUNNING;
     S.states[c] = s;
   }
 }

@@ -330,6 +330,31 @@ def test_fused_core_plain_fp64_cg(s, oracle):
     assert np.linalg.norm(xs - xo) <= 1e-7 * np.linalg.norm(xo)
 
 
+@pytest.mark.parametrize("n,dim,kind,ell,nugget", [(8192, 3, 0, 0.05, 1.0), (5000, 2, 1, 0.1, 0.5),
+                                                   (300, 1, 0, 0.05, 1.0)])
+def test_ordered_plain_fp64_cg(s, oracle, n, dim, kind, ell, nugget):
+    """mpla_kernel_conjugate_gradient_mixed with MPLA_PREC_F64: plain FP64 CG
+    (src/mpla.cpp:887-1001) on Hilbert-ordered points with the FP64 product
+    screened at 2^-(53 + log2 n).  Same iterations as the oracle's CG on the
+    exact operator (+-2 %), true residual below the tolerance, solution within
+    1e-7 of the oracle's, in the caller's order."""
+    pts = oracle.uniform(dim * n, 1300 + n, 0).reshape(dim, n)
+    bh = oracle.uniform(n, 1300 + n, 1, -1, 1)
+    k = oracle.Kernel(pts, kind, ell, nugget=nugget)
+    P = Points(pts)
+    kd = mpla.mpla_kernel_matrix_data(P.ptr, dim, 0, kind, mpla.MPLA_PREC_F64, ell, nugget, 0.0)
+    A = s.generic(n, n, C.addressof(kd))
+    b = s.vector(n, bh)
+    x = s.vector(n, np.zeros(n))
+    s.lib.mpla_kernel_conjugate_gradient_mixed(C.byref(b), C.byref(A), C.byref(x), 2001, 1e-8, mpla.MPLA_PREC_F64,
+                                               s.I)
+    xs = s.get(x)
+    xo, info = oracle.cg_relative(k.op(), bh, np.zeros(n), 2001, 1e-8)
+    assert s.inst.last_converged == 1 and iters_close(s.inst.last_iterations, info.iterations)
+    assert np.linalg.norm(bh - k.matvec(xs)) / np.linalg.norm(bh) < 1e-8 * 1.001
+    assert np.linalg.norm(xs - xo) <= 1e-7 * np.linalg.norm(xo)
+
+
 # The FP64 residual products drop entries below 2^-(log2 n + log2(1/nugget) +
 # log2(1/eps) + 12): the smaller nugget moves that screen by 3.3 bits, and the
 # reported residual must still be the oracle's to 1e-3 of itself.
@@ -351,6 +376,56 @@ def test_mixed_precision_ir_reaches_fp64_residual(s, oracle, kind, ell, nugget):
     assert abs(true_rel - s.inst.last_rel_res) <= 1e-3 * true_rel + 1e-15
     xo, _ = oracle.cg_relative(k.op(), bh, np.zeros(n), 5000, 1e-12)
     assert np.linalg.norm(xs - xo) <= 1e-6 * np.linalg.norm(xo)
+
+
+@pytest.mark.parametrize("shift", [1e-8, -1e-8])
+def test_mixed_precision_with_coincidence_shift(s, oracle, shift):
+    """A non-zero coincidence shift (the reference core's +1e-8 where points
+    coincide, src/kernel.cu:45-48, here also negative) has no planned inner
+    product: the mixed solve takes the unordered path and still meets the true
+    FP64 residual of the shifted operator, checked by the oracle."""
+    n = 4096
+    pts, bh, _ = gaussian_system(oracle, n)
+    pts[:, 1::97] = pts[:, 0:1]  # coincident points, so the shift applies off the diagonal too
+    k = oracle.Kernel(pts, 0, 0.05, nugget=1.0, coincident_shift=shift)
+    P = Points(pts)
+    kd = mpla.mpla_kernel_matrix_data(P.ptr, 3, 0, mpla.MPLA_KERNEL_GAUSSIAN, mpla.MPLA_PREC_F64, 0.05, 1.0, shift)
+    A = s.generic(n, n, C.addressof(kd))
+    b = s.vector(n, bh)
+    x = s.vector(n, np.zeros(n))
+    s.lib.mpla_kernel_conjugate_gradient_mixed(C.byref(b), C.byref(A), C.byref(x), 5000, 1e-8, mpla.MPLA_PREC_F32,
+                                               s.I)
+    xs = s.get(x)
+    true_rel = np.linalg.norm(bh - k.matvec(xs)) / np.linalg.norm(bh)
+    assert s.inst.last_converged == 1
+    assert true_rel < 1e-8 * 1.01
+    assert abs(true_rel - s.inst.last_rel_res) <= 1e-3 * true_rel + 1e-15
+    xo, _ = oracle.cg_relative(k.op(), bh, np.zeros(n), 5000, 1e-12)
+    assert np.linalg.norm(xs - xo) <= 1e-6 * np.linalg.norm(xo)
+
+
+# Typical interpolation nuggets: the residual screen's ||x|| <= ||b||/nugget
+# bound is loosest here and the FP32 inner operator may lose definiteness
+# (the inner solve then stops at its breakdown, or refinement stagnates and
+# the solve finishes in FP64).  Either way the reported residual is the true
+# one and meets the tolerance.
+@pytest.mark.parametrize("nugget", [1e-3, 1e-5])
+def test_mixed_precision_small_nugget(s, oracle, nugget):
+    n = 8192
+    pts, bh, k = gaussian_system(oracle, n, 0.05, nugget)
+    P = Points(pts)
+    kd = mpla.mpla_kernel_matrix_data(P.ptr, 3, 0, mpla.MPLA_KERNEL_GAUSSIAN, mpla.MPLA_PREC_F64, 0.05, nugget, 0.0)
+    A = s.generic(n, n, C.addressof(kd))
+    b = s.vector(n, bh)
+    x = s.vector(n, np.zeros(n))
+    s.lib.mpla_kernel_conjugate_gradient_mixed(C.byref(b), C.byref(A), C.byref(x), 20000, 1e-8,
+                                               mpla.MPLA_PREC_F32, s.I)
+    xs = s.get(x)
+    assert np.all(np.isfinite(xs))
+    true_rel = np.linalg.norm(bh - k.matvec(xs)) / np.linalg.norm(bh)
+    assert s.inst.last_converged == 1
+    assert true_rel < 1e-8 * 1.01
+    assert abs(true_rel - s.inst.last_rel_res) <= 1e-2 * true_rel + 1e-15
 
 
 @pytest.mark.parametrize("n,dim,kind,ell,nugget", [(1, 3, 0, 0.05, 1.0), (2, 1, 1, 0.3, 0.5), (300, 2, 0, 0.05, 1.0),

@@ -120,3 +120,39 @@ def test_server_screened_ordered_vs_exact(oracle, monkeypatch):
         _, _, worst = host_check(oracle, r, a)
         assert worst < r["tolerance"] * 1.01
         assert np.linalg.norm(a - b) <= 1e-6 * np.linalg.norm(b)
+
+
+def test_server_zero_rhs_column_retires(oracle):
+    """A right-hand side that is all zero is solved exactly by x = 0 at the
+    first residual; the request retires with its other columns converged."""
+    require_gpu()
+    r = serving.request_stream(1, 100.0, seed=29, n_min=3000, n_max=3000)[0]
+    r = dict(r, nrhs=2)
+    B = np.zeros((r["n"], 2), order="F")
+    B[:, 0] = serving.request_stream(1, 100.0, seed=31, n_min=3000, n_max=3000)[0]["rhs"][: r["n"]]
+    r["rhs"] = B.reshape(-1, order="F").copy()
+    for mode in (mpla.MPLA_SERVE_MIXED_BATCHED, mpla.MPLA_SERVE_FP64_SERIAL):
+        with serving.Server(mode=mode) as srv:
+            res, sols = srv.run_trace([r])
+        assert res[0].status == mpla.MPLA_REQUEST_CONVERGED
+        assert np.isfinite(res[0].max_rel_residual) and res[0].max_rel_residual < r["tolerance"]
+        assert np.all(sols[0][:, 1] == 0.0)
+        pts = r["points"].reshape(r["dim"], r["n"])
+        k = oracle.Kernel(pts, 0 if r["kind"] == mpla.MPLA_KERNEL_GAUSSIAN else 1, r["length_scale"],
+                          nugget=r["nugget"])
+        assert np.linalg.norm(B[:, 0] - k.matvec(sols[0][:, 0])) / np.linalg.norm(B[:, 0]) < r["tolerance"] * 1.01
+
+
+@pytest.mark.parametrize("nugget", [1e-3, 1e-5])
+def test_server_small_nugget_mixed(oracle, nugget):
+    """Mixed-mode serving at typical interpolation nuggets: every request
+    converges to its tolerance on the oracle's residual."""
+    require_gpu()
+    reqs = [dict(r, nugget=nugget) for r in serving.request_stream(3, 3000.0, seed=37, n_min=4000, n_max=8192)]
+    with serving.Server() as srv:
+        res, sols = srv.run_trace(reqs)
+    for r, x, X in zip(reqs, res, sols):
+        assert x.status == mpla.MPLA_REQUEST_CONVERGED
+        _, _, worst = host_check(oracle, r, X)
+        assert worst < r["tolerance"] * 1.01
+        assert abs(worst - x.max_rel_residual) <= 1e-2 * worst + 1e-15
