@@ -44,6 +44,7 @@ RESIDUAL_DROP_LOG2 = float(math.ceil(math.log2(N) - math.log2(NUGGET) - math.log
 PEAK_FP32_NONPACKED = 78.6e12  # one FP32 FMA per lane per cycle; the 157.3 TF figure counts packed FP32
 PEAK_FP32 = 157.3e12
 PEAK_FP64 = 78.6e12
+PEAK_BF16_MFMA = 2.5e15  # dense
 PEAK_HBM = 8.0e12
 
 
@@ -250,7 +251,37 @@ def bench_block(K, L, inst, stream_ptr, args):
     out = {}
     names = {"bf16": mpla.MPLA_PREC_BF16, "bf16x3": mpla.MPLA_PREC_BF16X3, "f32": mpla.MPLA_PREC_F32,
              "f64": mpla.MPLA_PREC_F64}
+    cols = (0, 21, 42, 63)
     sols = {}
+    # the shared K*V product alone, per precision (HIP events on the library stream)
+    desc = _native.mplak_kernel_desc()
+    desc.points, desc.dim, desc.kind = table.data_ptr(), 3, _native.MPLAK_KERNEL_MATERN32
+    desc.length_scale, desc.nugget, desc.coincident_shift = 0.1, 0.5, 0.0
+    desc.row_offset, desc.row_count, desc.col_offset, desc.col_count = 0, n, 0, n
+    Yd = torch.empty(n * R, dtype=torch.float64, device="cuda")
+    ext = torch.cuda.ExternalStream(stream_ptr)
+    kv = {}
+    for name in [p for p in args.block_precisions.split(",") if p and p != "f64"]:
+        prec = names[name]
+        wb = K.mplak_kernel_mrhs_workspace_bytes(C.byref(desc), prec, R)
+        w = torch.empty(wb // 8 + 1, dtype=torch.float64, device="cuda")
+        f = lambda: _native.check(K.mplak_kernel_mrhs(C.byref(desc), prec, R, Bd.data_ptr(), n, Yd.data_ptr(), n,
+                                                      w.data_ptr(), wb, None, stream_ptr))
+        f()
+        ms = _event_ms(ext, f, 5)
+        gen = 16.0 * n * n
+        con = 2.0 * n * n * R * (3 if name == "bf16x3" else 1)
+        mfma_peak = PEAK_BF16_MFMA if name.startswith("bf16") else PEAK_FP32
+        kv[name] = {"kernel": "mrhs_mfma_kernel", "ms": ms, "entry_tflops": gen / (ms * 1e-3) / 1e12,
+                    "entry_frac_valu": gen / (ms * 1e-3) / PEAK_FP32,
+                    "contraction_tflops": con / (ms * 1e-3) / 1e12,
+                    "contraction_frac_mfma": con / (ms * 1e-3) / mfma_peak,
+                    "mfma_peak_tflops": mfma_peak / 1e12}
+        log(f"K*V {name}: {ms:.2f} ms, entries {gen / (ms * 1e-3) / 1e12:.1f} TFLOP/s, "
+            f"contraction {con / (ms * 1e-3) / 1e12:.0f} TFLOP/s")
+        del w
+    del Yd
+    out["kv_product"] = kv
     for name in [p for p in args.block_precisions.split(",") if p]:
         Xd.zero_()
         torch.cuda.synchronize()
@@ -263,13 +294,14 @@ def bench_block(K, L, inst, stream_ptr, args):
         out[name] = {"seconds": t, "rhs_solves_per_s": R / t, "shared_products": inst.last_iterations,
                      "max_true_rel_residual": max(rel), "converged": bool(inst.last_converged)}
         log(f"Matern block solve {name}: {t:.2f} s for {R} RHS, {inst.last_iterations} shared products, max rel {max(rel):.2e}")
-    if args.block_oracle_columns and sols:
+    if not args.no_block_oracle_columns and sols:
         x = mpla.mpla_vector()
         bv = mpla.mpla_vector()
         L.mpla_init_vector(C.byref(x), C.byref(inst), n)
         L.mpla_init_vector(C.byref(bv), C.byref(inst), n)
         errs = {k: [] for k in sols}
-        for c in (0, 21, 42, 63):
+        t0 = time.perf_counter()
+        for c in cols:
             _native.check(K.mplak_copy_f64(bv.data, Bd.data_ptr() + 8 * c * n, n, stream_ptr))
             _native.check(K.mplak_fill_f64(x.data, n, 0.0, stream_ptr))
             L.mpla_kernel_conjugate_gradient_mixed(C.byref(bv), C.byref(A), C.byref(x), 20000, 1e-12,
@@ -278,8 +310,13 @@ def bench_block(K, L, inst, stream_ptr, args):
             L.mpla_copy_distributed_vector_to_cpu(xo.ctypes.data, C.byref(x), C.byref(inst))
             for k, X in sols.items():
                 errs[k].append(float(np.linalg.norm(X[:, c] - xo) / np.linalg.norm(xo)))
+        t_or = time.perf_counter() - t0
         for k in sols:
             out[k]["rel_error_vs_fp64_columns_0_21_42_63"] = errs[k]
+        out["fp64_columns"] = {"columns": list(cols), "tolerance": 1e-12, "seconds": t_or,
+                               "solver": "plain FP64 CG (mpla_kernel_conjugate_gradient_mixed, MPLA_PREC_F64)"}
+        log(f"Matern block: FP64 columns {cols} in {t_or:.1f} s; errors " +
+            ", ".join(f"{k} {max(v):.2e}" for k, v in errs.items()))
         L.mpla_free_vector(C.byref(x), C.byref(inst))
         L.mpla_free_vector(C.byref(bv), C.byref(inst))
     L.mpla_free_generic_matrix(C.byref(A), C.byref(inst))
@@ -364,7 +401,8 @@ def bench_served(args):
     out = {"requests": count, "arrival_rate_per_s": rate, "plain_capacity_solves_per_s": cap,
            "capacity_measured_on": f"the first {len(head)} requests offered at t=0",
            "mean_n": float(sum(r["n"] for r in reqs) / count)}
-    for name, mode in (("mixed_batched", mpla.MPLA_SERVE_MIXED_BATCHED), ("fp64_serial", mpla.MPLA_SERVE_FP64_SERIAL)):
+    modes = (("mixed_batched", mpla.MPLA_SERVE_MIXED_BATCHED), ("fp64_serial", mpla.MPLA_SERVE_FP64_SERIAL))
+    for name, mode in modes:
         with serving.Server(mode=mode) as srv:
             t0 = time.perf_counter()
             res, _ = srv.run_trace(reqs)
@@ -374,9 +412,244 @@ def bench_served(args):
         out[name] = {"solves_per_s": ok / span, "makespan_s": span, "converged": ok,
                      "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st.get("mean_ms"),
                      "max_true_rel_residual": max(x.max_rel_residual for x in res),
-                     "iterations": sum(x.iterations for x in res)}
+                     "iterations": sum(x.iterations for x in res),
+                     "slowest": serving.slowest(reqs, res, 5)}
+        if mode == mpla.MPLA_SERVE_FP64_SERIAL and not args.skip_cpu:
+            out["cpu"] = cpu_served(reqs, res)
         log(f"served stream {name}: {ok}/{count} converged, {ok / span:.2f} solves/s, p50 {st['p50_ms']:.1f} ms, "
             f"p99 {st['p99_ms']:.1f} ms")
+    # capacity: the first `saturated` requests all offered at t = 0 (closed system, no pacing)
+    sat = reqs[:min(count, args.served_saturated)]
+    out["saturated"] = {"requests": len(sat), "offered": "all at t=0"}
+    for name, mode in modes:
+        with serving.Server(mode=mode) as srv:
+            t0 = time.perf_counter()
+            res, _ = srv.run_trace(sat, arrivals=False)
+            span = time.perf_counter() - t0
+        ok = sum(1 for x in res if x.status == mpla.MPLA_REQUEST_CONVERGED)
+        st = serving.latency_stats(res)
+        out["saturated"][name] = {"capacity_solves_per_s": ok / span, "makespan_s": span, "converged": ok,
+                                  "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
+                                  "max_true_rel_residual": max(x.max_rel_residual for x in res)}
+        log(f"served saturated {name}: {ok}/{len(sat)} in {span:.2f} s = {ok / span:.2f} solves/s, "
+            f"p50 {st['p50_ms']:.0f} ms, p99 {st['p99_ms']:.0f} ms")
+    a, b = out["saturated"]["mixed_batched"], out["saturated"]["fp64_serial"]
+    out["saturated"]["capacity_ratio_mixed_over_fp64"] = a["capacity_solves_per_s"] / b["capacity_solves_per_s"]
+    return out
+
+
+def cpu_dense(args):
+    """CPU legs of the dense configurations with the oracle (OpenMP, this
+    host's cores): the N = 4096 CG solve and 100 dgemv, dgemv at N = 65536
+    (the headline dgemv size, 34 GB on the host) and BiCGSTAB at N = 32768 in
+    full, on the same generated inputs as the GPU legs."""
+    import numpy as np
+    from oracle import binding as oracle
+    out = {}
+    n = 4096
+    B = oracle.uniform(n * n, 1235, 10, -1.0, 1.0).reshape((n, n), order="F")
+    M = B @ B.T / n
+    A = np.asfortranarray(0.5 * (M + M.T) + np.eye(n))
+    del M, B
+    b = oracle.uniform(n, 1235, 11, -1.0, 1.0)
+    op = oracle.dense_op(A, mode=0)
+    t0 = time.perf_counter()
+    xs, info = oracle.cg_relative(op, b, np.zeros(n), 2001, 1e-10)
+    tcg = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for _ in range(100):
+        oracle.dgemv(A, xs, mode=0)
+    tg = (time.perf_counter() - t0) / 100
+    out["dense_cg_n4096"] = {"solves_per_s": 1.0 / tcg, "ms_per_solve": tcg * 1e3, "iterations": info.iterations,
+                             "dgemv_ms": tg * 1e3, "dgemv_gbps": (8.0 * n * n + 16 * n) / tg / 1e9,
+                             "cores": oracle.threads(), "kind": "port",
+                             "sample": "full CG solve (relative 1e-10) and 100 dgemv, A = B B^T/N + I built with "
+                                       "numpy (same B as the GPU leg)"}
+    log(f"cpu dense CG n=4096: {tcg * 1e3:.1f} ms ({info.iterations} iterations), dgemv {tg * 1e3:.2f} ms")
+    del A, op
+    n = 65536
+    A = oracle.uniform(n * n, 1234, 1, -1.0, 1.0).reshape((n, n), order="F")
+    xg = oracle.uniform(n, 1234, 2, -1.0, 1.0)
+    oracle.dgemv(A, xg, mode=0)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        oracle.dgemv(A, xg, mode=0)
+    tg = (time.perf_counter() - t0) / 3
+    out["dgemv_65536"] = {"ms": tg * 1e3, "gbps": (8.0 * n * n + 16 * n) / tg / 1e9, "cores": oracle.threads(),
+                          "kind": "port", "sample": "3 dgemv on the 34 GB matrix of the GPU leg (host RAM)"}
+    log(f"cpu dgemv n={n}: {tg * 1e3:.0f} ms, {(8.0 * n * n) / tg / 1e9:.1f} GB/s")
+    del A
+    n = 32768
+    A = oracle.uniform(n * n, 1235, 20, -1.0, 1.0).reshape((n, n), order="F")
+    A *= 1.0 / math.sqrt(n)
+    A[np.arange(n), np.arange(n)] += oracle.uniform(n, 1235, 21, 2.0, 3.0)
+    b = oracle.uniform(n, 1235, 22, -1.0, 1.0)
+    t0 = time.perf_counter()
+    _, info = oracle.bicgstab(oracle.dense_op(A, mode=0), b, np.zeros(n), 2001, 1e-10)
+    tb = time.perf_counter() - t0
+    out["bicgstab_32768"] = {"ms_per_solve": tb * 1e3, "solves_per_s": 1.0 / tb, "iterations": info.iterations,
+                             "products": info.matvecs, "cores": oracle.threads(), "kind": "port",
+                             "sample": "full BiCGSTAB solve (3 products per iteration, relative 1e-10)"}
+    log(f"cpu BiCGSTAB n={n}: {tb:.2f} s ({info.iterations} iterations)")
+    del A
+    return out
+
+
+def cpu_block(shared_products, seconds_budget=None):
+    """CPU K*V of the Matern-3/2 N = 131072 system: one right-hand side's
+    product with the oracle, extrapolated (labelled) to the 64 columns of one
+    shared product and to the products of one block solve."""
+    import numpy as np
+    from oracle import binding as oracle
+    n = 131072
+    pts = oracle.uniform(3 * n, 1235, 30).reshape(3, n)
+    k = oracle.Kernel(pts, oracle.MATERN32, 0.1, nugget=0.5)
+    v = oracle.normal(n, 1235, 31)
+    t0 = time.perf_counter()
+    k.matvec(v, mode=0)
+    t1 = time.perf_counter() - t0
+    out = {"one_column_product_s": t1, "kv_product_s_extrapolated": 64 * t1, "cores": oracle.threads(),
+           "kind": "port", "sample": "1 FP64 matrix-free product of one column (OpenMP), extrapolated x64 columns"}
+    if shared_products:
+        out["block_solve_s_extrapolated"] = 64 * t1 * shared_products
+        out["extrapolated_to_shared_products"] = shared_products
+    log(f"cpu Matern K*V: {t1:.2f} s per column, {64 * t1:.0f} s per 64-column product (extrapolated)")
+    return out
+
+
+def cpu_served(reqs, results, budget_s=25.0):
+    """The served stream's plain FP64 path on the CPU: the requests of a
+    prefix of the trace solved one at a time (oracle CG per right-hand side,
+    relative 1e-8), as many as fit a time budget predicted from the GPU
+    iteration counts; service times only (arrivals ignored)."""
+    import numpy as np
+    from oracle import binding as oracle
+    from mpla_amd import mpla
+    rate = None
+    served, times = 0, []
+    predicted = 0.0
+    for r, res in zip(reqs, results):
+        if rate is not None:
+            cost = (res.iterations + 2) * float(r["n"]) ** 2 / rate
+            if predicted + cost > budget_s:
+                break
+            predicted += cost
+        pts = r["points"].reshape(r["dim"], r["n"])
+        kind = oracle.GAUSSIAN if r["kind"] == mpla.MPLA_KERNEL_GAUSSIAN else oracle.MATERN32
+        k = oracle.Kernel(pts, kind, r["length_scale"], nugget=r["nugget"])
+        Bm = r["rhs"].reshape((r["n"], r["nrhs"]), order="F")
+        t0 = time.perf_counter()
+        products = 0
+        for c in range(r["nrhs"]):
+            _, info = oracle.cg_relative(k.op(mode=0), np.ascontiguousarray(Bm[:, c]), np.zeros(r["n"]), 20000,
+                                         r["tolerance"])
+            products += info.matvecs
+        t = time.perf_counter() - t0
+        times.append(t)
+        served += 1
+        rate = products * float(r["n"]) ** 2 / t
+    tot = sum(times)
+    out = {"requests": served, "solves_per_s": served / tot if tot else None, "seconds": tot,
+           "service_ms_p50": percentile(times, 50) * 1e3, "service_ms_p99": percentile(times, 99) * 1e3,
+           "cores": oracle.threads(), "kind": "port",
+           "sample": f"the first {served} requests of the trace, one at a time (oracle FP64 CG per right-hand side)"}
+    log(f"cpu served prefix: {served} requests in {tot:.1f} s")
+    return out
+
+
+def bench_hmatrix(K, L, inst, stream_ptr, ext):
+    """The H-matrix plugin at N = 131072 beside the exact product on the same
+    points: the reference H-matrix driver's setting (src/h_matrix_test.cu:
+    Halton points in [0,1]^2, exp(-r^2), c_leaf 512, k 20, eta 1.5) through
+    mpla_init_hmglib with the Gaussian system assembler (virtual entries), and
+    through the closed-form kernel path in 2-D and 3-D."""
+    import numpy as np
+    import torch
+    from mpla_amd import _native, mpla
+    n = 131072
+    out = {"n": n, "c_leaf": 512, "k": 20, "eta": 1.5, "kernel": "exp(-r^2)", "points": "Halton"}
+    xv = torch.empty(n, dtype=torch.float64, device="cuda")
+    _native.check(K.mplak_fill_uniform_f64(xv.data_ptr(), n, SEED, 40, 0, -1.0, 1.0, stream_ptr))
+    yv = torch.empty(n, dtype=torch.float64, device="cuda")
+    for dim, virtual in ((2, True), (2, False), (3, False)):
+        name = f"d{dim}_" + ("assembler" if virtual else "closed_form")
+        ph = np.empty((dim, n))
+        assert L.mpla_halton_points(ph.ctypes.data, dim, n, 0, n) == 0  # host memory
+        pts = torch.as_tensor(ph, device="cuda")
+        torch.cuda.synchronize()
+        table = torch.tensor([pts[d].data_ptr() for d in range(dim)], dtype=torch.int64, device="cuda")
+        ptrs = (C.c_void_p * dim)(*[pts[d].data_ptr() for d in range(dim)])
+        coords = (C.c_void_p * 2)(C.cast(ptrs, C.c_void_p), C.cast(ptrs, C.c_void_p))
+        A = mpla.mpla_generic_matrix()
+        L.mpla_init_generic_matrix(C.byref(A), C.byref(inst), n, n)
+        data = mpla.h_matrix_data()
+        A.data = C.addressof(data)
+        assem_pp = None
+        t0 = time.perf_counter()
+        if virtual:
+            assem_pp = torch.zeros(1, dtype=torch.int64, device="cuda")
+            L.create_gaussian_kernel_system_assembler_object(assem_pp.data_ptr(), table.data_ptr(), n, dim)
+            torch.cuda.synchronize()
+            ids = torch.arange(n, dtype=torch.int32, device="cuda")
+            idp = (C.c_void_p * 2)(ids.data_ptr(), ids.data_ptr())
+            counts = (C.c_int * 2)(n, n)
+            t0 = time.perf_counter()
+            L.mpla_init_hmglib(C.byref(A), counts, coords, idp, int(assem_pp.cpu()[0]), 1.5, dim, 32, 512, 20,
+                               b"\0", b"\0", 1 << 24, 1 << 24, C.byref(inst))
+        else:
+            L.mpla_init_hmatrix_kernel(C.byref(A), coords, dim, mpla.MPLA_KERNEL_GAUSSIAN, 1.0 / math.sqrt(2.0), 0.0,
+                                       1.5, 512, 20, 0.0, C.byref(inst))
+        torch.cuda.synchronize()
+        setup = time.perf_counter() - t0
+        info = mpla.mpla_hmatrix_info()
+        L.mpla_hmatrix_get_info(C.byref(A), C.byref(info))
+        xm, ym = mpla.mpla_vector(), mpla.mpla_vector()
+        L.mpla_init_vector(C.byref(xm), C.byref(inst), n)
+        L.mpla_init_vector(C.byref(ym), C.byref(inst), n)
+        _native.check(K.mplak_copy_f64(xm.data, xv.data_ptr(), n, stream_ptr))
+        hcore, fcore = mpla.core("mpla_dgemv_core_hmglib_h_matrix"), mpla.core("mpla_dgemv_core_hmglib_full_matrix")
+        fh = lambda: L.mpla_generic_dgemv(C.byref(ym), C.byref(A), C.byref(xm), hcore, C.byref(inst))
+        fh()
+        t_h = _event_ms(ext, fh, 10)
+        yh = torch.empty(n, dtype=torch.float64, device="cuda")
+        _native.check(K.mplak_copy_f64(yh.data_ptr(), ym.data, n, stream_ptr))
+        ff = lambda: L.mpla_generic_dgemv(C.byref(ym), C.byref(A), C.byref(xm), fcore, C.byref(inst))
+        t_f = _event_ms(ext, ff, 1)
+        # the exact symmetric FP64 product on the same points (the fastest exact product of this build)
+        d = _native.mplak_kernel_desc()
+        d.points, d.dim, d.kind = table.data_ptr(), dim, _native.MPLAK_KERNEL_GAUSSIAN
+        d.length_scale, d.nugget, d.coincident_shift = 1.0 / math.sqrt(2.0), 0.0, 0.0
+        d.row_offset, d.row_count, d.col_offset, d.col_count = 0, n, 0, n
+        wb = K.mplak_kernel_matvec_workspace_bytes(C.byref(d))
+        w = torch.empty(wb // 8 + 1, dtype=torch.float64, device="cuda")
+        fe = lambda: _native.check(K.mplak_kernel_matvec(C.byref(d), _native.MPLAK_PREC_F64, xv.data_ptr(),
+                                                         yv.data_ptr(), w.data_ptr(), wb, None, stream_ptr))
+        fe()
+        t_e = _event_ms(ext, fe, 3)
+        torch.cuda.synchronize()
+        err = float(torch.linalg.norm(yh - yv) / torch.linalg.norm(yv))
+        near_flop = float(info.near_entries) * (3 * dim + 4)
+        far_bytes = 8.0 * float(info.stored_terms)
+        out[name] = {"setup_s": setup, "h_product_ms": t_h, "full_core_ms": t_f, "exact_product_ms": t_e,
+                     "speedup_vs_exact": t_e / t_h, "rel_error_vs_exact": err,
+                     "near_entries": info.near_entries, "far_entries": info.far_entries,
+                     "near_fraction": info.near_entries / float(n) / n, "far_blocks": info.far_blocks,
+                     "near_blocks": info.near_blocks, "mean_rank": info.mean_rank, "max_rank": info.max_rank,
+                     "stored_terms": info.stored_terms, "device_bytes": info.device_bytes,
+                     "near_tflops": near_flop / (t_h * 1e-3) / 1e12,
+                     "near_frac_fp64": near_flop / (t_h * 1e-3) / PEAK_FP64,
+                     "far_factor_gbps": far_bytes / (t_h * 1e-3) / 1e9}
+        log(f"H-matrix {name}: setup {setup:.2f} s, product {t_h:.3f} ms (full {t_f:.2f} ms, exact {t_e:.3f} ms), "
+            f"rel error {err:.2e}, near {info.near_entries / float(n) / n:.3f}, mean rank {info.mean_rank:.1f}")
+        L.mpla_destroy_hmglib(C.byref(A), C.byref(inst))
+        L.mpla_free_vector(C.byref(xm), C.byref(inst))
+        L.mpla_free_vector(C.byref(ym), C.byref(inst))
+        L.mpla_free_generic_matrix(C.byref(A), C.byref(inst))
+        if assem_pp is not None:
+            L.destroy_gaussian_kernel_system_assembler_object(assem_pp.data_ptr())
+        del w, pts, table, yh
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
     return out
 
 
@@ -542,9 +815,14 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-configs", action="store_true", help="only the headline workload")
     ap.add_argument("--served-requests", type=int, default=256)
-    ap.add_argument("--block-precisions", default="bf16x3,bf16")
-    ap.add_argument("--block-oracle-columns", action="store_true",
-                    help="also solve columns 0,21,42,63 with plain FP64 CG for the error report")
+    ap.add_argument("--served-saturated", type=int, default=96,
+                    help="requests of the stream offered at t=0 to measure each server mode's capacity")
+    ap.add_argument("--block-precisions", default="bf16x3,bf16,f32")
+    ap.add_argument("--no-block-oracle-columns", action="store_true",
+                    help="skip solving columns 0,21,42,63 with plain FP64 CG for the error report")
+    ap.add_argument("--only", default="",
+                    help="run only these configurations (comma-separated: dense, dense_large, block, served, "
+                         "reference_kernel_cg, hmatrix, cpu_dense) and print their results")
     ap.add_argument("--kernels-only", action="store_true",
                     help="launch only the measured kernels a few times (for rocprofv3 counter runs) and exit")
     args = ap.parse_args()
@@ -589,6 +867,18 @@ def main():
     if args.kernels_only:
         profile_kernels(K, table, stream_ptr)
         return
+    if args.only:
+        # one or more configurations alone (for profiler runs); no headline line
+        runs = {"dense": lambda: bench_dense_cg(K, L, inst, stream_ptr, ext),
+                "dense_large": lambda: bench_dense_large(K, L, inst, stream_ptr, ext),
+                "block": lambda: bench_block(K, L, inst, stream_ptr, args),
+                "served": lambda: bench_served(args),
+                "reference_kernel_cg": lambda: bench_reference_kernel_cg(K, L, inst, stream_ptr, ext),
+                "hmatrix": lambda: bench_hmatrix(K, L, inst, stream_ptr, ext),
+                "cpu_dense": lambda: cpu_dense(args)}
+        res = {name: runs[name]() for name in args.only.split(",")}
+        print(json.dumps({"only": res}), flush=True)
+        return
 
     def solve_mixed():
         reset_x()
@@ -601,6 +891,12 @@ def main():
         L.mpla_generic_conjugate_gradient_with_checkpoint_restart(
             C.byref(b), C.byref(A), C.byref(x), ITER_MAX, EPS, mpla.core("mpla_dgemv_core_kernel_fused"), 0, False,
             C.byref(inst))
+        return inst.last_iterations, inst.last_rel_res, inst.last_converged, inst.last_matvecs
+
+    def solve_plain_ordered():
+        reset_x()
+        L.mpla_kernel_conjugate_gradient_mixed(C.byref(b), C.byref(A), C.byref(x), ITER_MAX, EPS,
+                                               mpla.MPLA_PREC_F64, C.byref(inst))
         return inst.last_iterations, inst.last_rel_res, inst.last_converged, inst.last_matvecs
 
     def read_x():
@@ -625,6 +921,21 @@ def main():
     x_mixed = read_x()
     it_mixed, rel_mixed, conv_mixed, mv_mixed = infos[-1]
     log(f"mixed: {args.steps} solves in {wall:.3f} s, inner iterations {it_mixed}, true rel residual {rel_mixed:.3e}")
+    # outside the timed region: the mixed solution's residual with the exact
+    # FP64 product (every entry, caller order)
+    b_host = np.empty(N)
+    L.mpla_copy_distributed_vector_to_cpu(b_host.ctypes.data, C.byref(b), C.byref(inst))
+    desc_x = product_desc(table)
+    wbx = K.mplak_kernel_matvec_workspace_bytes(C.byref(desc_x))
+    wx = torch.empty(wbx // 8 + 1, dtype=torch.float64, device="cuda")
+    kx = torch.empty(N, dtype=torch.float64, device="cuda")
+    _native.check(K.mplak_kernel_matvec(C.byref(desc_x), _native.MPLAK_PREC_F64, x.data, kx.data_ptr(),
+                                        wx.data_ptr(), wbx, None, stream_ptr))
+    torch.cuda.synchronize()
+    exact_rel = float(np.linalg.norm(b_host - kx.cpu().numpy()) / np.linalg.norm(b_host))
+    del wx, kx
+    log(f"mixed: exact FP64 residual of the solution {exact_rel:.3e}")
+    assert exact_rel < EPS, f"mixed solution misses the tolerance on the exact operator: {exact_rel}"
 
     # ---- plain FP64 (reference algorithm on the GPU) ------------------------
     solve_plain()  # warm the FP64 path
@@ -639,6 +950,18 @@ def main():
     plain_sps = args.plain_steps / sum(plat)
     log(f"plain FP64: {args.plain_steps} solves, {1 / plain_sps:.3f} s each, iterations {it_plain}")
     err_vs_fp64 = float(np.linalg.norm(x_mixed - x_plain) / np.linalg.norm(x_plain))
+    # plain FP64 CG on Hilbert-ordered points with the FP64-exact screen
+    solve_plain_ordered()
+    torch.cuda.synchronize()
+    olat = []
+    for _ in range(args.plain_steps):
+        ts = time.perf_counter()
+        oinfo = solve_plain_ordered()
+        olat.append(time.perf_counter() - ts)
+    x_ord = read_x()
+    ord_sps = args.plain_steps / sum(olat)
+    log(f"plain FP64 (ordered, FP64-exact screen): {1 / ord_sps:.3f} s each, iterations {oinfo[0]}")
+    best_plain = max(plain_sps, ord_sps)
 
     # ---- dominant kernel: FP32 matrix-free product, HIP events on the stream --
     desc = product_desc(table)
@@ -737,6 +1060,17 @@ def main():
         configs["matern_block_n131072"] = bench_block(K, L, inst, stream_ptr, args)
         configs["served_stream"] = bench_served(args)
         configs["reference_benchmark_kernel_cg_n131072"] = bench_reference_kernel_cg(K, L, inst, stream_ptr, ext)
+        configs["hmatrix_n131072"] = bench_hmatrix(K, L, inst, stream_ptr, ext)
+        if not args.skip_cpu:
+            dense = cpu_dense(args)
+            configs["dense_cg_n4096"]["cpu"] = dense["dense_cg_n4096"]
+            configs["dense_dgemv_bicgstab"]["cpu"] = {"dgemv_65536": dense["dgemv_65536"],
+                                                      "bicgstab_32768": dense["bicgstab_32768"]}
+            blk = configs["matern_block_n131072"]
+            sp = next((blk[p]["shared_products"] for p in ("bf16x3", "f32", "bf16") if p in blk), 0)
+            blk["cpu"] = cpu_block(sp)
+        if cpu is not None:
+            configs["gaussian_n65536"] = {"cpu": cpu}
 
     value = args.steps / wall
     out = {
@@ -757,11 +1091,20 @@ def main():
                    "n": N, "dim": DIM, "length_scale": ELL, "nugget": NUGGET, "tolerance": EPS},
         "latency_ms": {"p50": percentile(lat, 50) * 1e3, "p99": percentile(lat, 99) * 1e3},
         "mixed": {"inner_iterations": it_mixed, "true_rel_residual": rel_mixed, "converged": bool(conv_mixed),
-                  "products": mv_mixed, "rel_error_vs_fp64_solution": err_vs_fp64},
+                  "products": mv_mixed, "rel_error_vs_fp64_solution": err_vs_fp64,
+                  "exact_rel_residual": exact_rel},
         "plain_fp64": {"solves_per_s": plain_sps, "iterations": it_plain, "rel_residual": rel_plain,
-                       "converged": bool(conv_plain),
+                       "converged": bool(conv_plain), "operator": "every entry (drop-in fused core)",
                        "latency_ms": {"p50": percentile(plat, 50) * 1e3, "p99": percentile(plat, 99) * 1e3}},
-        "speedup_mixed_vs_plain": value / plain_sps,
+        "plain_fp64_screened": {"solves_per_s": ord_sps, "iterations": oinfo[0], "rel_residual": oinfo[1],
+                                "converged": bool(oinfo[2]),
+                                "operator": "Hilbert-ordered points, FP64 product screened at 2^-(53+log2 N) "
+                                            "(mpla_kernel_conjugate_gradient_mixed with MPLA_PREC_F64)",
+                                "rel_diff_vs_all_entries_solution":
+                                    float(np.linalg.norm(x_ord - x_plain) / np.linalg.norm(x_plain)),
+                                "latency_ms": {"p50": percentile(olat, 50) * 1e3, "p99": percentile(olat, 99) * 1e3}},
+        "speedup_mixed_vs_plain": value / best_plain,
+        "speedup_basis": "the faster of plain_fp64 and plain_fp64_screened",
         "roofline": {"bound": "valu", "achieved": inner["executed_tflops"], "peak": PEAK_FP32 / 1e12,
                      "unit": "TFLOP/s", "frac": inner["executed_tflops"] * 1e12 / PEAK_FP32,
                      "traffic": committed_traffic("planned_inner_product_n65536"),

@@ -237,6 +237,83 @@ int mplak_gaussian_kernel_rhs(const double* const* points, int32_t dim, int64_t 
                               int64_t row_count, double* b, mplak_stream_t stream);
 
 /* ---------------------------------------------------------------------------
+ * Hierarchical-matrix (H-matrix) operator: the kernels behind the H-matrix
+ * plugin cores of hmglib_adapter.h, which replace hmglib's setup and product
+ * (src/hmglib_adapter.cu:6-132).  The host builds the cluster trees and the
+ * block partition; these kernels compress the admissible blocks and apply
+ * the operator.  Rows and columns are in cluster-tree order ("sorted").
+ * ------------------------------------------------------------------------ */
+
+#define MPLAK_HM_MAX_RANK 32
+
+typedef struct mplak_hm_block {
+  int64_t r0, m;    /* sorted rows [r0, r0 + m) */
+  int64_t c0, n;    /* sorted columns [c0, c0 + n) */
+  int64_t uoff;     /* U factor (m x kmax, column-major) at U + uoff */
+  int64_t voff;     /* V factor (n x kmax, column-major) at V + voff */
+} mplak_hm_block;
+
+typedef struct mplak_hm_chunk {
+  int32_t leaf;     /* row leaf whose near/far lists apply */
+  int32_t count;    /* rows, 1 .. 64 */
+  int64_t r0;       /* first sorted row */
+} mplak_hm_chunk;
+
+typedef struct mplak_hm_item {
+  int32_t block;    /* admissible block */
+  int32_t pad;
+  int64_t j0, count; /* its local columns [j0, j0 + count) */
+} mplak_hm_item;
+
+/* Closed-form entries: k(|p_i - p_j|) of MPLAK_KERNEL_* with the points'
+ * coordinates already multiplied by the kernel's coordinate scale
+ * (mplak_hm_gather_scaled).  The nugget is not part of these entries. */
+typedef struct mplak_hm_points {
+  const double* rows;  /* sorted row points: rows[d * row_ld + i] */
+  const double* cols;  /* sorted column points */
+  int64_t row_ld, col_ld;
+  int32_t dim;         /* 1 .. 4 */
+  int32_t kind;        /* MPLAK_KERNEL_* */
+} mplak_hm_points;
+
+/* out[d * ld + k] = scale(kind, length_scale) * points[d][offset + perm[k]]:
+ * the sorted, scaled coordinates mplak_hm_points expects. */
+int mplak_hm_gather_scaled(const double* const* points, int32_t dim, int64_t offset, const int64_t* perm, int64_t n,
+                           int32_t kind, double length_scale, double* out, int64_t ld, mplak_stream_t stream);
+
+/* Adaptive cross approximation with partial pivoting of every block
+ * (one workgroup each): rank[b] <= kmax <= MPLAK_HM_MAX_RANK terms, fewer
+ * when a pivot is exactly zero or, with eps > 0, once the newest term is
+ * below eps times the Frobenius norm of the approximation.  Deterministic. */
+int mplak_hm_aca(const mplak_hm_points* pts, const mplak_hm_block* blocks, int64_t nblocks, int32_t kmax,
+                 double eps, double* U, double* V, int32_t* rank, mplak_stream_t stream);
+
+/* part[i * kmax + l] = sum over the item's columns j of V_b(j, l) xs[c0_b + j]
+ * (l < rank[b]), then t[b * kmax + l] = sum of the partials of block b's
+ * items in item order (items of one block are consecutive; first_item has
+ * nblocks + 1 entries). */
+int mplak_hm_far_partials(const mplak_hm_block* blocks, const mplak_hm_item* items, int64_t nitems,
+                          const int32_t* rank, int32_t kmax, const double* V, const double* xs, double* part,
+                          mplak_stream_t stream);
+int mplak_hm_far_sum(const int64_t* first_item, int64_t nblocks, const int32_t* rank, int32_t kmax,
+                     const double* part, double* t, mplak_stream_t stream);
+
+/* ys[row] = sum over the row leaf's inadmissible column ranges (near_cols
+ * holds (c0, count) pairs, near_ptr the CSR offsets per leaf) of the
+ * entries times xs, plus sum over its admissible blocks (far_blk, CSR
+ * far_ptr) of U_b(row, :) t_b, for the rows of every chunk. */
+int mplak_hm_apply(const mplak_hm_points* pts, const mplak_hm_chunk* chunks, int64_t nchunks, const int64_t* near_ptr,
+                   const int64_t* near_cols, const int64_t* far_ptr, const int32_t* far_blk,
+                   const mplak_hm_block* blocks, const int32_t* rank, int32_t kmax, const double* U, const double* t,
+                   const double* xs, double* ys, mplak_stream_t stream);
+
+/* y[perm[k]] = ys[k] + nugget * x[perm[k] + shift] (the second term only
+ * where 0 <= perm[k] + shift < x_count): the sorted product back in the
+ * caller's row order with the nugget on the global diagonal. */
+int mplak_hm_scatter_nugget(const double* ys, const int64_t* perm, int64_t n, double nugget, const double* x,
+                            int64_t shift, int64_t x_count, double* y, mplak_stream_t stream);
+
+/* ---------------------------------------------------------------------------
  * Multi-right-hand-side product (no reference counterpart)
  * ------------------------------------------------------------------------ */
 

@@ -18,6 +18,7 @@ MPLAK_ERR_ARG = 1
 MPLAK_ERR_HIP = 2
 MPLAK_ERR_WORKSPACE = 3
 MPLAK_ERR_UNSUPPORTED = 4
+MPLAK_HM_MAX_RANK = 32
 MPLAK_KERNEL_GAUSSIAN = 0
 MPLAK_KERNEL_MATERN32 = 1
 MPLAK_PREC_F64 = 0
@@ -135,6 +136,12 @@ MPLAK_SIGNATURES = {
     "mplak_gather_f64": (C.c_int, [_vp, _vp, _vp, _i64, _vp]),
     "mplak_scatter_f64": (C.c_int, [_vp, _vp, _vp, _i64, _vp]),
     "mplak_gaussian_kernel_rhs": (C.c_int, [_vp, _i32, _i64, _i64, _vp, _vp]),
+    "mplak_hm_gather_scaled": (C.c_int, [_vp, _i32, _i64, _vp, _i64, _i32, _dbl, _vp, _i64, _vp]),
+    "mplak_hm_aca": (C.c_int, [_vp, _vp, _i64, _i32, _dbl, _vp, _vp, _vp, _vp]),
+    "mplak_hm_far_partials": (C.c_int, [_vp, _vp, _i64, _vp, _i32, _vp, _vp, _vp, _vp]),
+    "mplak_hm_far_sum": (C.c_int, [_vp, _i64, _vp, _i32, _vp, _vp, _vp]),
+    "mplak_hm_apply": (C.c_int, [_vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp]),
+    "mplak_hm_scatter_nugget": (C.c_int, [_vp, _vp, _i64, _dbl, _vp, _i64, _i64, _vp, _vp]),
     "mplak_kernel_mrhs_workspace_bytes": (_sz, [C.POINTER(mplak_kernel_desc), C.c_int, _i64]),
     "mplak_kernel_mrhs": (C.c_int, [C.POINTER(mplak_kernel_desc), C.c_int, _i64, _vp, _i64, _vp, _i64, _vp, _sz, _vp,
                                     _vp]),

@@ -1,0 +1,212 @@
+// H-matrix operator kernels with closed-form kernel entries (C ABI of
+// include/mplak.h): ACA compression of the admissible blocks, the far-field
+// coefficient products and the near + far apply.  The block partition comes
+// from the host (libmpla's H-matrix plugin, which replaces hmglib behind
+// src/hmglib_adapter.cu:6-132).
+
+#include "common.hpp"
+#include "entry.hpp"
+#include "hmatrix_kernels.hpp"
+
+namespace mplak {
+namespace {
+
+using hm::kHmThreads;
+
+__global__ __launch_bounds__(kBlock) void gather_scaled_kernel(const double* const* __restrict__ points, int dim,
+                                                               int64_t offset, const int64_t* __restrict__ perm,
+                                                               int64_t n, double scale, double* __restrict__ out,
+                                                               int64_t ld) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+    const int64_t i = offset + perm[k];
+    for (int d = 0; d < dim; ++d) out[d * ld + k] = scale * points[d][i];
+  }
+}
+
+// One workgroup per item: the item's columns against the block's rank terms.
+__global__ __launch_bounds__(kHmThreads) void far_partial_kernel(const mplak_hm_block* __restrict__ blocks,
+                                                                 const mplak_hm_item* __restrict__ items,
+                                                                 const int32_t* __restrict__ rank, int kmax,
+                                                                 const double* __restrict__ V,
+                                                                 const double* __restrict__ xs,
+                                                                 double* __restrict__ part) {
+  __shared__ double red[kHmThreads / kWave];
+  const mplak_hm_item it = items[blockIdx.x];
+  const mplak_hm_block B = blocks[it.block];
+  const int rb = rank[it.block];
+  const double* Vb = V + B.voff;
+  double* out = part + (int64_t)blockIdx.x * kmax;
+  for (int q = 0; q < rb; ++q) {
+    double s = 0.0;
+    for (int64_t j = it.j0 + threadIdx.x; j < it.j0 + it.count; j += kHmThreads)
+      s = fma(Vb[j + q * B.n], xs[B.c0 + j], s);
+    s = block_sum<kHmThreads>(s, red);
+    if (threadIdx.x == 0) out[q] = s;
+  }
+}
+
+// One thread per (block, term): the block's item partials in item order.
+__global__ __launch_bounds__(kBlock) void far_sum_kernel(const int64_t* __restrict__ first_item, int64_t nblocks,
+                                                         const int32_t* __restrict__ rank, int kmax,
+                                                         const double* __restrict__ part, double* __restrict__ t) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nblocks * kmax) return;
+  const int64_t b = g / kmax;
+  const int q = (int)(g % kmax);
+  double s = 0.0;
+  if (q < rank[b])
+    for (int64_t i = first_item[b]; i < first_item[b + 1]; ++i) s += part[i * kmax + q];
+  t[g] = s;
+}
+
+__global__ __launch_bounds__(kBlock) void scatter_nugget_kernel(const double* __restrict__ ys,
+                                                                const int64_t* __restrict__ perm, int64_t n,
+                                                                double nugget, const double* __restrict__ x,
+                                                                int64_t shift, int64_t x_count,
+                                                                double* __restrict__ y) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+    const int64_t i = perm[k];
+    const int64_t j = i + shift;
+    double v = ys[k];
+    if (nugget != 0.0 && j >= 0 && j < x_count) v = fma(nugget, x[j], v);
+    y[i] = v;
+  }
+}
+
+template <int KIND>
+int launch_aca(const mplak_hm_points* p, const mplak_hm_block* blocks, int64_t nblocks, int kmax, double eps,
+               double* U, double* V, int32_t* rank, hipStream_t s) {
+  hm::KernelEntries<KIND> e{p->rows, p->cols, p->row_ld, p->col_ld, p->dim};
+  hipLaunchKernelGGL(hm::aca_kernel<hm::KernelEntries<KIND>>, dim3((unsigned)nblocks), dim3(kHmThreads), 0, s, e,
+                     blocks, kmax, eps, U, V, rank);
+  return launch_status();
+}
+
+template <int KIND, int DIM>
+void launch_apply(const hm::KernelEntries<KIND>& e, int64_t nchunks, hipStream_t s, const mplak_hm_chunk* chunks,
+                  const int64_t* near_ptr, const int64_t* near_cols, const int64_t* far_ptr, const int32_t* far_blk,
+                  const mplak_hm_block* blocks, const int32_t* rank, int kmax, const double* U, const double* t,
+                  const double* xs, double* ys) {
+  hipLaunchKernelGGL((hm::apply_kernel<KIND, DIM>), dim3((unsigned)nchunks), dim3(kHmThreads), 0, s, e, chunks,
+                     near_ptr, near_cols, far_ptr, far_blk, blocks, rank, kmax, U, t, xs, ys);
+}
+
+template <int KIND>
+int apply_kind(const mplak_hm_points* p, const mplak_hm_chunk* chunks, int64_t nchunks, const int64_t* near_ptr,
+               const int64_t* near_cols, const int64_t* far_ptr, const int32_t* far_blk,
+               const mplak_hm_block* blocks, const int32_t* rank, int kmax, const double* U, const double* t,
+               const double* xs, double* ys, hipStream_t s) {
+  hm::KernelEntries<KIND> e{p->rows, p->cols, p->row_ld, p->col_ld, p->dim};
+  switch (p->dim) {
+    case 1: launch_apply<KIND, 1>(e, nchunks, s, chunks, near_ptr, near_cols, far_ptr, far_blk, blocks, rank, kmax,
+                                  U, t, xs, ys); break;
+    case 2: launch_apply<KIND, 2>(e, nchunks, s, chunks, near_ptr, near_cols, far_ptr, far_blk, blocks, rank, kmax,
+                                  U, t, xs, ys); break;
+    case 3: launch_apply<KIND, 3>(e, nchunks, s, chunks, near_ptr, near_cols, far_ptr, far_blk, blocks, rank, kmax,
+                                  U, t, xs, ys); break;
+    default: launch_apply<KIND, 0>(e, nchunks, s, chunks, near_ptr, near_cols, far_ptr, far_blk, blocks, rank, kmax,
+                                   U, t, xs, ys); break;
+  }
+  return launch_status();
+}
+
+int check_points(const mplak_hm_points* p) {
+  if (p == nullptr || p->rows == nullptr || p->cols == nullptr || p->dim < 1 || p->dim > hm::kHmMaxDim ||
+      p->row_ld < 0 || p->col_ld < 0)
+    return MPLAK_ERR_ARG;
+  if (p->kind != MPLAK_KERNEL_GAUSSIAN && p->kind != MPLAK_KERNEL_MATERN32) return MPLAK_ERR_ARG;
+  return MPLAK_OK;
+}
+
+int64_t grid_for(int64_t n) {
+  int64_t g = (n + kBlock - 1) / kBlock;
+  return g < 1 ? 1 : (g > 4096 ? 4096 : g);
+}
+
+}  // namespace
+}  // namespace mplak
+
+using namespace mplak;
+
+extern "C" {
+
+int mplak_hm_gather_scaled(const double* const* points, int32_t dim, int64_t offset, const int64_t* perm, int64_t n,
+                           int32_t kind, double length_scale, double* out, int64_t ld, mplak_stream_t stream) {
+  if (points == nullptr || perm == nullptr || out == nullptr || dim < 1 || n < 0 || offset < 0 || ld < n ||
+      !(length_scale > 0.0) || (kind != MPLAK_KERNEL_GAUSSIAN && kind != MPLAK_KERNEL_MATERN32))
+    return MPLAK_ERR_ARG;
+  if (n == 0) return MPLAK_OK;
+  hipLaunchKernelGGL(gather_scaled_kernel, dim3((unsigned)grid_for(n)), dim3(kBlock), 0, as_stream(stream), points,
+                     dim, offset, perm, n, coord_scale(kind, length_scale), out, ld);
+  return launch_status();
+}
+
+int mplak_hm_aca(const mplak_hm_points* pts, const mplak_hm_block* blocks, int64_t nblocks, int32_t kmax, double eps,
+                 double* U, double* V, int32_t* rank, mplak_stream_t stream) {
+  int st = check_points(pts);
+  if (st != MPLAK_OK) return st;
+  if (nblocks < 0 || kmax < 1 || kmax > MPLAK_HM_MAX_RANK || !(eps >= 0.0) || nblocks > 0x7fffffff) return MPLAK_ERR_ARG;
+  if (nblocks == 0) return MPLAK_OK;
+  if (blocks == nullptr || U == nullptr || V == nullptr || rank == nullptr) return MPLAK_ERR_ARG;
+  hipStream_t s = as_stream(stream);
+  if (pts->kind == MPLAK_KERNEL_GAUSSIAN)
+    return launch_aca<MPLAK_KERNEL_GAUSSIAN>(pts, blocks, nblocks, kmax, eps, U, V, rank, s);
+  return launch_aca<MPLAK_KERNEL_MATERN32>(pts, blocks, nblocks, kmax, eps, U, V, rank, s);
+}
+
+int mplak_hm_far_partials(const mplak_hm_block* blocks, const mplak_hm_item* items, int64_t nitems,
+                          const int32_t* rank, int32_t kmax, const double* V, const double* xs, double* part,
+                          mplak_stream_t stream) {
+  if (nitems < 0 || nitems > 0x7fffffff || kmax < 1 || kmax > MPLAK_HM_MAX_RANK) return MPLAK_ERR_ARG;
+  if (nitems == 0) return MPLAK_OK;
+  if (blocks == nullptr || items == nullptr || rank == nullptr || V == nullptr || xs == nullptr || part == nullptr)
+    return MPLAK_ERR_ARG;
+  hipLaunchKernelGGL(far_partial_kernel, dim3((unsigned)nitems), dim3(kHmThreads), 0, as_stream(stream), blocks,
+                     items, rank, kmax, V, xs, part);
+  return launch_status();
+}
+
+int mplak_hm_far_sum(const int64_t* first_item, int64_t nblocks, const int32_t* rank, int32_t kmax,
+                     const double* part, double* t, mplak_stream_t stream) {
+  if (nblocks < 0 || kmax < 1 || kmax > MPLAK_HM_MAX_RANK) return MPLAK_ERR_ARG;
+  if (nblocks == 0) return MPLAK_OK;
+  if (first_item == nullptr || rank == nullptr || part == nullptr || t == nullptr) return MPLAK_ERR_ARG;
+  const int64_t total = nblocks * kmax;
+  const int64_t g = (total + kBlock - 1) / kBlock;
+  if (g > 0x7fffffff) return MPLAK_ERR_ARG;
+  hipLaunchKernelGGL(far_sum_kernel, dim3((unsigned)g), dim3(kBlock), 0, as_stream(stream), first_item, nblocks, rank,
+                     kmax, part, t);
+  return launch_status();
+}
+
+int mplak_hm_apply(const mplak_hm_points* pts, const mplak_hm_chunk* chunks, int64_t nchunks, const int64_t* near_ptr,
+                   const int64_t* near_cols, const int64_t* far_ptr, const int32_t* far_blk,
+                   const mplak_hm_block* blocks, const int32_t* rank, int32_t kmax, const double* U, const double* t,
+                   const double* xs, double* ys, mplak_stream_t stream) {
+  int st = check_points(pts);
+  if (st != MPLAK_OK) return st;
+  if (nchunks < 0 || nchunks > 0x7fffffff || kmax < 1 || kmax > MPLAK_HM_MAX_RANK) return MPLAK_ERR_ARG;
+  if (nchunks == 0) return MPLAK_OK;
+  if (chunks == nullptr || near_ptr == nullptr || far_ptr == nullptr || xs == nullptr || ys == nullptr)
+    return MPLAK_ERR_ARG;
+  hipStream_t s = as_stream(stream);
+  if (pts->kind == MPLAK_KERNEL_GAUSSIAN)
+    return apply_kind<MPLAK_KERNEL_GAUSSIAN>(pts, chunks, nchunks, near_ptr, near_cols, far_ptr, far_blk, blocks,
+                                             rank, kmax, U, t, xs, ys, s);
+  return apply_kind<MPLAK_KERNEL_MATERN32>(pts, chunks, nchunks, near_ptr, near_cols, far_ptr, far_blk, blocks, rank,
+                                           kmax, U, t, xs, ys, s);
+}
+
+int mplak_hm_scatter_nugget(const double* ys, const int64_t* perm, int64_t n, double nugget, const double* x,
+                            int64_t shift, int64_t x_count, double* y, mplak_stream_t stream) {
+  if (n < 0 || x_count < 0) return MPLAK_ERR_ARG;
+  if (n == 0) return MPLAK_OK;
+  if (ys == nullptr || perm == nullptr || y == nullptr || (nugget != 0.0 && x == nullptr)) return MPLAK_ERR_ARG;
+  hipLaunchKernelGGL(scatter_nugget_kernel, dim3((unsigned)grid_for(n)), dim3(kBlock), 0, as_stream(stream), ys, perm,
+                     n, nugget, x, shift, x_count, y);
+  return launch_status();
+}
+
+}  // extern "C"

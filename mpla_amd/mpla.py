@@ -1,5 +1,6 @@
 """ctypes mirror of the drop-in API (include/mpla.h, kernel.h, mpla_kernel.h,
-generic_system_adapter.h, kernel_system_assembler.h) exported by libmpla.so.
+generic_system_adapter.h, kernel_system_assembler.h, hmglib_adapter.h)
+exported by libmpla.so.
 
 This is the binding a Python caller of MPLA would use; the struct layouts are
 checked against the C headers by tests/test_abi.py.  Device memory that the
@@ -224,4 +225,60 @@ MPLA_SIGNATURES.update({
     "mpla_server_clock": (C.c_double, [_vp]),
     "mpla_server_peak_bytes": (C.c_size_t, [_vp]),
     "mpla_halton_points": (C.c_int, [_vp, C.c_int32, C.c_int64, C.c_int64, C.c_int64]),
+})
+
+
+# ---- hmglib_adapter.h ----------------------------------------------------------
+
+
+class h_matrix_data(C.Structure):
+    _fields_ = [
+        ("point_count", C.c_int * 2),
+        ("dim", C.c_int),
+        ("bits", C.c_int),
+        ("eta", C.c_double),
+        ("max_level", C.c_int),
+        ("c_leaf", C.c_int),
+        ("k", C.c_int),
+        ("epsilon", C.c_double),
+        ("root_level_set_1", C.c_char),
+        ("root_level_set_2", C.c_char),
+        ("max_batched_dense_size", C.c_int),
+        ("max_batched_aca_size", C.c_int),
+        ("assem", C.c_void_p),
+        ("coords_d", C.c_void_p * 2),
+        ("point_ids_d", C.c_void_p * 2),
+        ("impl", C.c_void_p),
+    ]
+
+
+class mpla_hmatrix_info(C.Structure):
+    _fields_ = [
+        ("row_leaves", C.c_int64),
+        ("col_leaves", C.c_int64),
+        ("near_blocks", C.c_int64),
+        ("far_blocks", C.c_int64),
+        ("near_entries", C.c_int64),
+        ("far_entries", C.c_int64),
+        ("stored_terms", C.c_int64),
+        ("max_rank", C.c_int32),
+        ("k", C.c_int32),
+        ("mean_rank", C.c_double),
+        ("device_bytes", C.c_int64),
+    ]
+
+
+MPLA_SIGNATURES.update({
+    # all_coords[2] / all_point_ids[2] / global_point_count[2] are passed as pointers to 2-element arrays
+    "mpla_init_hmglib": (None, [_G, _vp, _vp, _vp, _vp, _dbl, _int, _int, _int, _int, C.c_char, C.c_char, _int, _int,
+                                _I]),
+    "mpla_init_hmglib_coords_from_host": (None, [_G, _vp, _vp, _vp, _dbl, _int, _int, _int, _int, _int, _int, _I]),
+    "mpla_destroy_hmglib": (None, [_G, _I]),
+    "mpla_dgemv_core_hmglib_h_matrix": (None, [_V, _G, _V, _I]),
+    "mpla_dgemv_core_hmglib_full_matrix": (None, [_V, _G, _V, _I]),
+    "set_gaussian_kernel_rhs": (None, [_vp, _P(h_matrix_data)]),
+    "mpla_init_hmatrix_kernel": (None, [_G, _vp, _int, _int, _dbl, _dbl, _dbl, _int, _int, _dbl, _I]),
+    "mpla_hmatrix_get_info": (None, [_G, _P(mpla_hmatrix_info)]),
+    "mpla_hmatrix_get_permutation": (None, [_G, _int, _P(C.c_int64)]),
+    "mpla_hmatrix_partition": (_int, [_vp, C.c_int64, _int, _dbl, _int, _P(C.c_int64), _P(C.c_int64)]),
 })

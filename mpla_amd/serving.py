@@ -106,3 +106,18 @@ def latency_stats(results):
         return {"p50_ms": float("nan"), "p99_ms": float("nan")}
     return {"p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
             "mean_ms": float(np.mean(lat) * 1e3)}
+
+
+def slowest(reqs, results, count=5):
+    """The `count` requests of longest latency: (latency ms, n, dim, kind,
+    nrhs, iterations, refinement steps, admission wait ms)."""
+    rows = []
+    for r, x in zip(reqs, results):
+        if x.status != mpla.MPLA_REQUEST_CONVERGED:
+            continue
+        rows.append({"latency_ms": (x.t_done - x.t_arrival) * 1e3, "n": r["n"], "dim": r["dim"],
+                     "kind": "gaussian" if r["kind"] == mpla.MPLA_KERNEL_GAUSSIAN else "matern32",
+                     "nrhs": r["nrhs"], "iterations": x.iterations, "refinement_steps": x.refinement_steps,
+                     "wait_ms": (x.t_admit - x.t_arrival) * 1e3})
+    rows.sort(key=lambda d: -d["latency_ms"])
+    return rows[:count]

@@ -82,6 +82,14 @@ def lib():
             "oracle_load_vector": (C.c_int, [C.c_char_p, vp, C.c_int32]),
             "oracle_threads": (C.c_int, []),
             "oracle_halton": (C.c_int, [vp, C.c_int, i64, i64, i64]),
+            "oracle_hm_create": (vp, [C.POINTER(oracle_kernel), dbl, C.c_int, C.c_int, dbl]),
+            "oracle_hm_free": (None, [vp]),
+            "oracle_hm_perm": (None, [vp, vp]),
+            "oracle_hm_counts": (None, [vp, vp]),
+            "oracle_hm_cluster": (C.c_int, [vp, C.c_int, vp]),
+            "oracle_hm_block": (C.c_int, [vp, C.c_int, C.c_int, vp]),
+            "oracle_hm_matvec": (None, [vp, vp, vp]),
+            "oracle_aca": (C.c_int, [vp, i64, i64, i64, C.c_int, dbl, vp, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -229,3 +237,69 @@ def halton(n, dim, first=0) -> np.ndarray:
 
 def threads() -> int:
     return lib().oracle_threads()
+
+
+class HMatrix:
+    """The oracle's H-matrix (cluster tree, block partition, ACA) of a kernel."""
+
+    def __init__(self, kernel: "Kernel", eta, c_leaf, kmax, eps=0.0):
+        self.kernel = kernel
+        self.n = kernel.n
+        self.dim = kernel.dim
+        self.h = lib().oracle_hm_create(C.byref(kernel.s), eta, c_leaf, kmax, eps)
+        if not self.h:
+            raise ValueError("invalid H-matrix arguments")
+
+    def close(self):
+        if self.h:
+            lib().oracle_hm_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def perm(self) -> np.ndarray:
+        p = np.empty(self.n, dtype=np.int64)
+        lib().oracle_hm_perm(self.h, _ptr(p))
+        return p
+
+    def counts(self) -> dict:
+        c = np.zeros(7, dtype=np.int64)
+        lib().oracle_hm_counts(self.h, _ptr(c))
+        keys = ("leaves", "near_blocks", "far_blocks", "near_entries", "far_entries", "stored_terms", "max_rank")
+        return dict(zip(keys, (int(v) for v in c)))
+
+    def clusters(self):
+        out = np.zeros(4 + 2 * self.dim)
+        count = lib().oracle_hm_cluster(self.h, 0, _ptr(out))
+        res = []
+        for c in range(count):
+            lib().oracle_hm_cluster(self.h, c, _ptr(out))
+            res.append((int(out[0]), int(out[1]), int(out[2]), int(out[3]), out[4:4 + self.dim].copy(),
+                        out[4 + self.dim:].copy()))
+        return res
+
+    def blocks(self, far):
+        rc = np.zeros(2, dtype=np.int32)
+        count = lib().oracle_hm_block(self.h, int(far), 0, _ptr(rc))
+        out = []
+        for b in range(count):
+            lib().oracle_hm_block(self.h, int(far), b, _ptr(rc))
+            out.append((int(rc[0]), int(rc[1])))
+        return out
+
+    def matvec(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty(self.n)
+        lib().oracle_hm_matvec(self.h, _ptr(x), _ptr(y))
+        return y
+
+
+def aca(A: np.ndarray, kmax, eps=0.0):
+    """ACA of an explicit matrix: (rank, U, V) with A ~ U V^T."""
+    A = np.asfortranarray(A, dtype=np.float64)
+    m, n = A.shape
+    U = np.zeros((m, kmax), order="F")
+    V = np.zeros((n, kmax), order="F")
+    r = lib().oracle_aca(_ptr(A), m, m, n, kmax, eps, _ptr(U), _ptr(V))
+    return r, U[:, :r], V[:, :r]

@@ -154,7 +154,7 @@ void oracle_daxpy(int64_t n, double alpha, const double* x, double* y) {
  * generalises them: Gaussian exp(-r^2/(2 l^2)) (l = 1/sqrt(2) gives exp(-r^2)),
  * Matern-3/2 (1+s)exp(-s) with s = sqrt(3) r / l (l = sqrt(3) gives the
  * reference's (1+r)exp(-r)).  The nugget on i == j follows the assembler's
- * regularization rule (src/kernel_system_assembler.h:55-68). */
+ * regularization rule (src/kernel_system_assembler.h:33-34). */
 
 double oracle_kernel_entry(const oracle_kernel* k, int64_t i, int64_t j) {
   double r2 = 0.0;
@@ -518,4 +518,361 @@ int oracle_halton(double* out, int dim, int64_t n, int64_t first, int64_t ld) {
     }
   }
   return 0;
+}
+
+/* ---- H-matrix operator (the H-matrix plugin, src/hmglib_adapter.cu:6-132) --
+ * The reference delegates setup and product to hmglib (P. Zaspel's GPU
+ * H-matrix library; version unpinned, linked by src/Makefile:63,78-81, not
+ * vendored).  Restated from the published construction of H-matrices
+ * (Hackbusch, "Hierarchical Matrices", Springer 2015, ch. 5-6; Bebendorf,
+ * Numer. Math. 86, 2000 for ACA), with the choices the build states:
+ *   cluster tree   a cluster of more than c_leaf points is split along the
+ *                  longest edge of its bounding box (lowest axis on ties)
+ *                  into the floor(n/2) points of smallest (coordinate, index)
+ *                  and the rest; a leaf lists its points by increasing index.
+ *   admissibility  min(diam B_s, diam B_t) <= eta dist(B_s, B_t), dist > 0.
+ *   block tree     admissible -> low rank; both leaves -> dense; otherwise the
+ *                  non-leaf side(s) are refined.
+ *   ACA            partial pivoting from the first row of the block: pivot
+ *                  column = arg max |row residual| (smallest index on ties),
+ *                  next pivot row = arg max |column residual| over unused rows;
+ *                  stops at kmax terms, a zero pivot, or (eps > 0) once
+ *                  |u_l||v_l| <= eps |S_l|_F.
+ * The product uses the oracle's kernel entries (libm exp) for the dense
+ * blocks and the ACA factors; sums in long double. */
+
+typedef struct {
+  int64_t lo, hi;
+  int child[2];
+  double bmin[8], bmax[8];
+} hm_cluster;
+
+struct oracle_hmatrix {
+  oracle_kernel k;
+  int64_t n;
+  int dim, kmax;
+  int64_t* perm;
+  hm_cluster* nodes;
+  int nnodes, nleaves;
+  int (*near)[2];
+  int nnear, cap_near;
+  int (*far)[2];
+  int nfar, cap_far;
+  double** U; /* per far block, m x rank column-major */
+  double** V; /* per far block, n x rank */
+  int* rank;
+};
+
+static const double* hm_key_coords;
+static int hm_key_cmp(const void* pa, const void* pb) {
+  const int64_t a = *(const int64_t*)pa, b = *(const int64_t*)pb;
+  const double ca = hm_key_coords[a], cb = hm_key_coords[b];
+  if (ca < cb) return -1;
+  if (ca > cb) return 1;
+  return a < b ? -1 : (a > b ? 1 : 0);
+}
+static int hm_index_cmp(const void* pa, const void* pb) {
+  const int64_t a = *(const int64_t*)pa, b = *(const int64_t*)pb;
+  return a < b ? -1 : (a > b ? 1 : 0);
+}
+
+static int hm_split(struct oracle_hmatrix* h, int id, int c_leaf) {
+  hm_cluster* c = &h->nodes[id];
+  const int64_t lo = c->lo, hi = c->hi, n = h->n;
+  for (int d = 0; d < h->dim; ++d) {
+    double mn = INFINITY, mx = -INFINITY;
+    for (int64_t q = lo; q < hi; ++q) {
+      const double v = h->k.points[(int64_t)d * n + h->perm[q]];
+      if (v < mn) mn = v;
+      if (v > mx) mx = v;
+    }
+    c->bmin[d] = mn;
+    c->bmax[d] = mx;
+  }
+  c->child[0] = c->child[1] = -1;
+  if (hi - lo <= c_leaf) {
+    qsort(h->perm + lo, (size_t)(hi - lo), sizeof(int64_t), hm_index_cmp);
+    return 0;
+  }
+  int axis = 0;
+  for (int d = 1; d < h->dim; ++d)
+    if (c->bmax[d] - c->bmin[d] > c->bmax[axis] - c->bmin[axis]) axis = d;
+  hm_key_coords = h->k.points + (int64_t)axis * n;
+  /* a full sort selects the same two sets as any selection of the median */
+  qsort(h->perm + lo, (size_t)(hi - lo), sizeof(int64_t), hm_key_cmp);
+  const int64_t mid = lo + (hi - lo) / 2;
+  const int c0 = h->nnodes;
+  h->nodes[c0].lo = lo;
+  h->nodes[c0].hi = mid;
+  h->nodes[c0 + 1].lo = mid;
+  h->nodes[c0 + 1].hi = hi;
+  h->nnodes += 2;
+  h->nodes[id].child[0] = c0;
+  h->nodes[id].child[1] = c0 + 1;
+  hm_split(h, c0, c_leaf);
+  hm_split(h, c0 + 1, c_leaf);
+  return 0;
+}
+
+static double hm_diam(const hm_cluster* c, int dim) {
+  double s = 0.0;
+  for (int d = 0; d < dim; ++d) s += (c->bmax[d] - c->bmin[d]) * (c->bmax[d] - c->bmin[d]);
+  return sqrt(s);
+}
+
+static double hm_dist(const hm_cluster* a, const hm_cluster* b, int dim) {
+  double s = 0.0;
+  for (int d = 0; d < dim; ++d) {
+    double g = a->bmin[d] - b->bmax[d];
+    if (b->bmin[d] - a->bmax[d] > g) g = b->bmin[d] - a->bmax[d];
+    if (g > 0.0) s += g * g;
+  }
+  return sqrt(s);
+}
+
+static void hm_push(int (**list)[2], int* cnt, int* cap, int a, int b) {
+  if (*cnt == *cap) {
+    *cap = *cap ? 2 * *cap : 256;
+    *list = realloc(*list, sizeof(int[2]) * (size_t)*cap);
+  }
+  (*list)[*cnt][0] = a;
+  (*list)[*cnt][1] = b;
+  ++*cnt;
+}
+
+static void hm_blocks(struct oracle_hmatrix* h, int a, int b, double eta) {
+  const hm_cluster* A = &h->nodes[a];
+  const hm_cluster* B = &h->nodes[b];
+  const double ds = hm_dist(A, B, h->dim);
+  const double da = hm_diam(A, h->dim), db = hm_diam(B, h->dim);
+  if (ds > 0.0 && (da < db ? da : db) <= eta * ds) {
+    hm_push(&h->far, &h->nfar, &h->cap_far, a, b);
+    return;
+  }
+  const int la = A->child[0] < 0, lb = B->child[0] < 0;
+  if (la && lb) {
+    hm_push(&h->near, &h->nnear, &h->cap_near, a, b);
+  } else if (la) {
+    hm_blocks(h, a, B->child[0], eta);
+    hm_blocks(h, a, B->child[1], eta);
+  } else if (lb) {
+    hm_blocks(h, A->child[0], b, eta);
+    hm_blocks(h, A->child[1], b, eta);
+  } else {
+    const int a0 = A->child[0], a1 = A->child[1], b0 = B->child[0], b1 = B->child[1];
+    hm_blocks(h, a0, b0, eta);
+    hm_blocks(h, a0, b1, eta);
+    hm_blocks(h, a1, b0, eta);
+    hm_blocks(h, a1, b1, eta);
+  }
+}
+
+/* entry (p, q) of the block with sorted rows r0 + p and columns c0 + q */
+typedef struct {
+  const struct oracle_hmatrix* h;
+  const double* dense; /* explicit matrix (oracle_aca) instead of the kernel */
+  int64_t ld, r0, c0;
+} hm_src;
+
+static double hm_entry(const hm_src* s, int64_t p, int64_t q) {
+  if (s->dense) return s->dense[p + q * s->ld];
+  return oracle_kernel_entry(&s->h->k, s->h->perm[s->r0 + p], s->h->perm[s->c0 + q]);
+}
+
+static int hm_aca(const hm_src* src, int64_t m, int64_t n, int kmax, double eps, double* U, double* V) {
+  int64_t* used = malloc(sizeof(int64_t) * (size_t)(kmax > 0 ? kmax : 1));
+  int64_t piv = 0;
+  int r = 0;
+  double s2 = 0.0;
+  for (int l = 0; l < kmax; ++l) {
+    int64_t js = -1;
+    double best = -1.0;
+    for (int64_t j = 0; j < n; ++j) {
+      double v = hm_entry(src, piv, j);
+      for (int q = 0; q < l; ++q) v -= U[piv + q * m] * V[j + q * n];
+      V[j + l * n] = v;
+      if (fabs(v) > best) {
+        best = fabs(v);
+        js = j;
+      }
+    }
+    const double delta = V[js + l * n];
+    if (!(fabs(delta) > 0.0) || !isfinite(delta)) break;
+    double nv2 = 0.0, nu2 = 0.0;
+    for (int64_t j = 0; j < n; ++j) {
+      V[j + l * n] /= delta;
+      nv2 += V[j + l * n] * V[j + l * n];
+    }
+    int64_t is = -1;
+    best = -1.0;
+    for (int64_t i = 0; i < m; ++i) {
+      double u = hm_entry(src, i, js);
+      for (int q = 0; q < l; ++q) u -= V[js + q * n] * U[i + q * m];
+      U[i + l * m] = u;
+      nu2 += u * u;
+      int fresh = i != piv;
+      for (int q = 0; q < l; ++q) fresh = fresh && used[q] != i;
+      if (fresh && fabs(u) > best) {
+        best = fabs(u);
+        is = i;
+      }
+    }
+    r = l + 1;
+    if (eps > 0.0) {
+      double cross = 0.0;
+      for (int q = 0; q < l; ++q) {
+        double du = 0.0, dv = 0.0;
+        for (int64_t i = 0; i < m; ++i) du += U[i + q * m] * U[i + l * m];
+        for (int64_t j = 0; j < n; ++j) dv += V[j + q * n] * V[j + l * n];
+        cross += du * dv;
+      }
+      s2 += 2.0 * cross + nu2 * nv2;
+      if (sqrt(nu2 * nv2) <= eps * sqrt(fabs(s2))) break;
+    }
+    used[l] = piv;
+    if (is < 0) break;
+    piv = is;
+  }
+  free(used);
+  return r;
+}
+
+int oracle_aca(const double* A, int64_t lda, int64_t m, int64_t n, int kmax, double eps, double* U, double* V) {
+  hm_src s = {NULL, A, lda, 0, 0};
+  return hm_aca(&s, m, n, kmax, eps, U, V);
+}
+
+struct oracle_hmatrix* oracle_hm_create(const oracle_kernel* k, double eta, int c_leaf, int kmax, double eps) {
+  if (k->dim < 1 || k->dim > 8 || c_leaf < 1 || kmax < 1 || !(eta > 0.0)) return NULL;
+  struct oracle_hmatrix* h = calloc(1, sizeof(*h));
+  h->k = *k;
+  h->n = k->n_points;
+  h->dim = k->dim;
+  h->kmax = kmax;
+  h->perm = malloc(sizeof(int64_t) * (size_t)(h->n > 0 ? h->n : 1));
+  for (int64_t i = 0; i < h->n; ++i) h->perm[i] = i;
+  h->nodes = calloc((size_t)(2 * (h->n / (c_leaf > 1 ? c_leaf / 2 : 1) + 2) + 4), sizeof(hm_cluster));
+  if (h->n > 0) {
+    h->nodes[0].lo = 0;
+    h->nodes[0].hi = h->n;
+    h->nnodes = 1;
+    hm_split(h, 0, c_leaf);
+    for (int i = 0; i < h->nnodes; ++i) h->nleaves += h->nodes[i].child[0] < 0;
+    hm_blocks(h, 0, 0, eta);
+  }
+  h->U = calloc((size_t)(h->nfar > 0 ? h->nfar : 1), sizeof(double*));
+  h->V = calloc((size_t)(h->nfar > 0 ? h->nfar : 1), sizeof(double*));
+  h->rank = calloc((size_t)(h->nfar > 0 ? h->nfar : 1), sizeof(int));
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int b = 0; b < h->nfar; ++b) {
+    const hm_cluster* A = &h->nodes[h->far[b][0]];
+    const hm_cluster* B = &h->nodes[h->far[b][1]];
+    const int64_t m = A->hi - A->lo, n = B->hi - B->lo;
+    h->U[b] = malloc(sizeof(double) * (size_t)(m * kmax));
+    h->V[b] = malloc(sizeof(double) * (size_t)(n * kmax));
+    hm_src s = {h, NULL, 0, A->lo, B->lo};
+    h->rank[b] = hm_aca(&s, m, n, kmax, eps, h->U[b], h->V[b]);
+  }
+  return h;
+}
+
+void oracle_hm_free(struct oracle_hmatrix* h) {
+  if (!h) return;
+  for (int b = 0; b < h->nfar; ++b) {
+    free(h->U[b]);
+    free(h->V[b]);
+  }
+  free(h->U);
+  free(h->V);
+  free(h->rank);
+  free(h->near);
+  free(h->far);
+  free(h->nodes);
+  free(h->perm);
+  free(h);
+}
+
+void oracle_hm_perm(const struct oracle_hmatrix* h, int64_t* perm) {
+  memcpy(perm, h->perm, sizeof(int64_t) * (size_t)h->n);
+}
+
+void oracle_hm_counts(const struct oracle_hmatrix* h, int64_t* counts) {
+  int64_t near_entries = 0, far_entries = 0, stored = 0;
+  int max_rank = 0;
+  for (int b = 0; b < h->nnear; ++b) {
+    const hm_cluster* A = &h->nodes[h->near[b][0]];
+    const hm_cluster* B = &h->nodes[h->near[b][1]];
+    near_entries += (A->hi - A->lo) * (B->hi - B->lo);
+  }
+  for (int b = 0; b < h->nfar; ++b) {
+    const hm_cluster* A = &h->nodes[h->far[b][0]];
+    const hm_cluster* B = &h->nodes[h->far[b][1]];
+    far_entries += (A->hi - A->lo) * (B->hi - B->lo);
+    stored += (A->hi - A->lo + B->hi - B->lo) * h->rank[b];
+    if (h->rank[b] > max_rank) max_rank = h->rank[b];
+  }
+  counts[0] = h->nleaves;
+  counts[1] = h->nnear;
+  counts[2] = h->nfar;
+  counts[3] = near_entries;
+  counts[4] = far_entries;
+  counts[5] = stored;
+  counts[6] = max_rank;
+}
+
+/* Cluster c of the tree: lo, hi, child0, child1, bmin[0..dim), bmax[0..dim)
+ * into out (4 + 2 dim doubles); returns the node count. */
+int oracle_hm_cluster(const struct oracle_hmatrix* h, int c, double* out) {
+  if (c >= 0 && c < h->nnodes) {
+    const hm_cluster* x = &h->nodes[c];
+    out[0] = (double)x->lo;
+    out[1] = (double)x->hi;
+    out[2] = x->child[0];
+    out[3] = x->child[1];
+    for (int d = 0; d < h->dim; ++d) {
+      out[4 + d] = x->bmin[d];
+      out[4 + h->dim + d] = x->bmax[d];
+    }
+  }
+  return h->nnodes;
+}
+
+/* Block b of the near (far = 0) or far (far = 1) list: its row and column
+ * clusters into rc[0..1]; returns the list length. */
+int oracle_hm_block(const struct oracle_hmatrix* h, int far, int b, int* rc) {
+  const int cnt = far ? h->nfar : h->nnear;
+  if (b >= 0 && b < cnt) {
+    rc[0] = far ? h->far[b][0] : h->near[b][0];
+    rc[1] = far ? h->far[b][1] : h->near[b][1];
+  }
+  return cnt;
+}
+
+/* y = H x in the caller's order (x, y indexed by point). */
+void oracle_hm_matvec(const struct oracle_hmatrix* h, const double* x, double* y) {
+  const int64_t n = h->n;
+  long double* acc = calloc((size_t)(n > 0 ? n : 1), sizeof(long double));
+  for (int b = 0; b < h->nnear; ++b) {
+    const hm_cluster* A = &h->nodes[h->near[b][0]];
+    const hm_cluster* B = &h->nodes[h->near[b][1]];
+#pragma omp parallel for schedule(static)
+    for (int64_t p = A->lo; p < A->hi; ++p) {
+      long double s = 0.0L;
+      for (int64_t q = B->lo; q < B->hi; ++q)
+        s += (long double)oracle_kernel_entry(&h->k, h->perm[p], h->perm[q]) * x[h->perm[q]];
+      acc[p] += s;
+    }
+  }
+  for (int b = 0; b < h->nfar; ++b) {
+    const hm_cluster* A = &h->nodes[h->far[b][0]];
+    const hm_cluster* B = &h->nodes[h->far[b][1]];
+    const int64_t m = A->hi - A->lo, nn = B->hi - B->lo;
+    for (int l = 0; l < h->rank[b]; ++l) {
+      long double t = 0.0L;
+      for (int64_t q = 0; q < nn; ++q) t += (long double)h->V[b][q + l * nn] * x[h->perm[B->lo + q]];
+      for (int64_t p = 0; p < m; ++p) acc[A->lo + p] += (long double)h->U[b][p + l * m] * t;
+    }
+  }
+  for (int64_t p = 0; p < n; ++p) y[h->perm[p]] = (double)acc[p];
+  free(acc);
 }

@@ -118,6 +118,26 @@ int oracle_load_vector(const char* path, double* x, int32_t n);
  * Returns 0, or -1 for an unsupported dim. */
 int oracle_halton(double* out, int dim, int64_t n, int64_t first, int64_t ld);
 
+/* ---- H-matrix operator (src/hmglib_adapter.cu:6-132; hmglib restated) --- */
+/* Cluster tree, block partition and ACA as described in oracle.c, for one
+ * point set used as rows and columns (the kernel k's points); entries are
+ * oracle_kernel_entry.  NULL for invalid arguments. */
+typedef struct oracle_hmatrix oracle_hmatrix;
+oracle_hmatrix* oracle_hm_create(const oracle_kernel* k, double eta, int c_leaf, int kmax, double eps);
+void oracle_hm_free(oracle_hmatrix* h);
+/* perm[k] = point at position k of the cluster-tree order. */
+void oracle_hm_perm(const oracle_hmatrix* h, int64_t* perm);
+/* counts: leaves, near blocks, far blocks, near entries, far entries,
+ * stored terms (sum (m + n) rank), largest rank. */
+void oracle_hm_counts(const oracle_hmatrix* h, int64_t* counts);
+int oracle_hm_cluster(const oracle_hmatrix* h, int c, double* out);
+int oracle_hm_block(const oracle_hmatrix* h, int far, int b, int* rc);
+/* y = H x in the caller's order. */
+void oracle_hm_matvec(const oracle_hmatrix* h, const double* x, double* y);
+/* ACA of an explicit m x n matrix (column-major, lda): returns the rank;
+ * U (m x kmax) and V (n x kmax) column-major receive the factors. */
+int oracle_aca(const double* A, int64_t lda, int64_t m, int64_t n, int kmax, double eps, double* U, double* V);
+
 /* Number of OpenMP threads the oracle's parallel loops use. */
 int oracle_threads(void);
 

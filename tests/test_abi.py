@@ -19,10 +19,12 @@ HEADERS = {
     "kernel_system_assembler.h": "libmpla.so",
     "mpla_server.h": "libmpla.so",
     "quadrature.h": "libmpla.so",
+    "hmglib_adapter.h": "libmpla.so",
 }
 
 DECL = re.compile(r"^\s*(?:extern\s+)?[A-Za-z_][\w\s\*]*?[\s\*]((?:mplak|mpla)_\w+|info|checkCUDAError|"
-                  r"(?:create|destroy)_gaussian_kernel_system_assembler_object)\s*\(", re.M)
+                  r"(?:create|destroy)_gaussian_kernel_system_assembler_object|set_gaussian_kernel_rhs)\s*\(",
+                  re.M)
 
 
 def declared(header):
@@ -80,7 +82,7 @@ def test_python_bindings_cover_headers():
     from mpla_amd import mpla
     names = set()
     for h in ("mpla.h", "kernel.h", "mpla_kernel.h", "generic_system_adapter.h", "kernel_system_assembler.h",
-              "mpla_server.h", "quadrature.h"):
+              "mpla_server.h", "quadrature.h", "hmglib_adapter.h"):
         names |= set(declared(h))
     assert sorted(names) == sorted(mpla.MPLA_SIGNATURES)
 
@@ -97,6 +99,8 @@ STRUCTS = {
     "mpla_server_config": None,
     "mpla_solve_request": None,
     "mpla_solve_result": None,
+    "h_matrix_data": None,
+    "mpla_hmatrix_info": None,
 }
 
 
@@ -105,7 +109,8 @@ def test_struct_layout_matches_headers(tmp_path):
     import subprocess
     from mpla_amd import mpla
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "mpla.h"', '#include "kernel.h"',
-             '#include "mpla_kernel.h"', '#include "mpla_server.h"', "int main(void) {"]
+             '#include "mpla_kernel.h"', '#include "mpla_server.h"', '#include "hmglib_adapter.h"',
+             "int main(void) {"]
     for sname in STRUCTS:
         cls = getattr(mpla, sname)
         tag = "" if sname.startswith(("mpla_server", "mpla_solve")) else "struct "
