@@ -30,6 +30,7 @@ typedef void* mplak_stream_t; /* a hipStream_t; NULL is the default stream */
 #define MPLAK_ERR_HIP 2         /* a HIP runtime call or kernel launch failed */
 #define MPLAK_ERR_WORKSPACE 3   /* caller workspace smaller than required */
 #define MPLAK_ERR_UNSUPPORTED 4 /* valid request this build does not implement */
+#define MPLAK_ERR_TOO_LARGE 5   /* beyond the fixed scratch of mplak_kernel_matvec_noalloc */
 
 /* Kernel functions k(r) of the matrix-free operator, r = |p_i - p_j| */
 #define MPLAK_KERNEL_GAUSSIAN 0 /* exp(-r^2 / (2 l^2)); l = 1/sqrt(2) is exp(-r^2) of src/kernel.cu:12-15 */
@@ -138,6 +139,20 @@ size_t mplak_kernel_matvec_workspace_bytes(const mplak_kernel_desc* desc);
 int mplak_kernel_matvec(const mplak_kernel_desc* desc, int precision, const double* x, double* y,
                         void* work, size_t work_bytes, const int32_t* skip_if,
                         mplak_stream_t stream);
+
+/* No reference counterpart: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  mplak_kernel_matvec with its workspace taken from a fixed
+ * device array of mplak_kernel_matvec_noalloc_max_bytes() bytes (80 MiB)
+ * that the library holds on each device it runs on: the same kernels, so y
+ * is bitwise that of mplak_kernel_matvec.  Calls no hipMalloc or other
+ * allocating routine.  Returns MPLAK_ERR_TOO_LARGE when the product's
+ * workspace (mplak_kernel_matvec_workspace_bytes) exceeds the array: square
+ * blocks up to n = 65536 fit.  All calls on one device share the array, so
+ * calls that overlap on one device (two streams, two threads) race. */
+int64_t mplak_kernel_matvec_noalloc_max_bytes(void);
+int mplak_kernel_matvec_noalloc(const mplak_kernel_desc* desc, int precision, const double* x, double* y,
+                                const int32_t* skip_if, mplak_stream_t stream);
 
 /* No reference counterpart (an approximation the reference does not make):
  * the product of mplak_kernel_matvec with negligible entries dropped, for

@@ -18,6 +18,7 @@ MPLAK_ERR_ARG = 1
 MPLAK_ERR_HIP = 2
 MPLAK_ERR_WORKSPACE = 3
 MPLAK_ERR_UNSUPPORTED = 4
+MPLAK_ERR_TOO_LARGE = 5
 MPLAK_HM_MAX_RANK = 32
 MPLAK_KERNEL_GAUSSIAN = 0
 MPLAK_KERNEL_MATERN32 = 1
@@ -117,6 +118,8 @@ MPLAK_SIGNATURES = {
     "mplak_dwaxpby_f64": (C.c_int, [_i64, _dbl, _vp, _dbl, _vp, _vp, _vp]),
     "mplak_kernel_matvec_workspace_bytes": (_sz, [C.POINTER(mplak_kernel_desc)]),
     "mplak_kernel_matvec": (C.c_int, [C.POINTER(mplak_kernel_desc), C.c_int, _vp, _vp, _vp, _sz, _vp, _vp]),
+    "mplak_kernel_matvec_noalloc_max_bytes": (_i64, []),
+    "mplak_kernel_matvec_noalloc": (C.c_int, [C.POINTER(mplak_kernel_desc), C.c_int, _vp, _vp, _vp, _vp]),
     "mplak_kernel_matvec_screened_workspace_bytes": (_sz, [C.POINTER(mplak_kernel_desc)]),
     "mplak_kernel_matvec_screened": (C.c_int, [C.POINTER(mplak_kernel_desc), C.c_int, _dbl, _vp, _vp, _vp, _sz,
                                                _vp, _vp]),

@@ -265,6 +265,7 @@ const char* mplak_status_string(int status) {
     case MPLAK_ERR_HIP: return "HIP runtime error";
     case MPLAK_ERR_WORKSPACE: return "workspace too small";
     case MPLAK_ERR_UNSUPPORTED: return "unsupported request";
+    case MPLAK_ERR_TOO_LARGE: return "problem needs more scratch than the no-allocation entry point holds";
     default: return "unknown status";
   }
 }

@@ -19,6 +19,7 @@
 #include <stdlib.h>
 
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 
 #include <rocprim/device/device_radix_sort.hpp>
@@ -26,8 +27,35 @@
 #include "common.hpp"
 #include "entry.hpp"
 
+// Scratch of mplak_kernel_matvec_noalloc: a fixed device array, so that the
+// entry point allocates nothing.  80 MiB holds the symmetric product's tile
+// partials up to n = 65536 (67.6 MB) and the plain product's slab partials
+// of the blocks that fit.  Each device that runs the library holds one copy
+// until the process ends; calls that overlap on one device share it and race.
+constexpr size_t kNoallocScratchBytes = size_t(80) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
 namespace mplak {
 namespace {
+
+// Device address of `scratch` on the current device, looked up once per
+// device with that device current.  The host-side name `scratch` is not a
+// device address and is never passed to a kernel or a copy.
+int noalloc_scratch(void** out) {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static void* ptrs[kMaxDevices] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return MPLAK_ERR_HIP;
+  std::lock_guard<std::mutex> lock(mu);
+  if (ptrs[dev] == nullptr) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || p == nullptr) return MPLAK_ERR_HIP;
+    ptrs[dev] = p;
+  }
+  *out = ptrs[dev];
+  return MPLAK_OK;
+}
 
 constexpr int kKmvTpt = 2;                     // target points per thread
 constexpr int kKmvTile = kBlock * kKmvTpt;     // 512 targets per workgroup
@@ -1841,6 +1869,22 @@ int mplak_kernel_matvec(const mplak_kernel_desc* d, int precision, const double*
                      p.slabs, part, d->row_offset, d->col_offset, d->col_count, d->nugget, x, y,
                      skip_if);
   return launch_status();
+}
+
+int64_t mplak_kernel_matvec_noalloc_max_bytes(void) { return (int64_t)kNoallocScratchBytes; }
+
+int mplak_kernel_matvec_noalloc(const mplak_kernel_desc* d, int precision, const double* x, double* y,
+                                const int32_t* skip_if, mplak_stream_t s) {
+  int st = check_desc(d);
+  if (st != MPLAK_OK) return st;
+  if (mplak_kernel_matvec_workspace_bytes(d) > kNoallocScratchBytes) return MPLAK_ERR_TOO_LARGE;
+  void* work = nullptr;
+  st = noalloc_scratch(&work);
+  if (st != MPLAK_OK) return st;
+  // the same kernels and launch shapes as mplak_kernel_matvec, on the
+  // device's fixed scratch: every partial the finish reads is written first,
+  // so nothing needs clearing between calls
+  return mplak_kernel_matvec(d, precision, x, y, work, kNoallocScratchBytes, skip_if, s);
 }
 
 int mplak_gaussian_kernel_rhs(const double* const* points, int32_t dim, int64_t row_offset,
