@@ -295,30 +295,28 @@ def bench_block(K, L, inst, stream_ptr, args):
                      "max_true_rel_residual": max(rel), "converged": bool(inst.last_converged)}
         log(f"Matern block solve {name}: {t:.2f} s for {R} RHS, {inst.last_iterations} shared products, max rel {max(rel):.2e}")
     if not args.no_block_oracle_columns and sols:
-        x = mpla.mpla_vector()
-        bv = mpla.mpla_vector()
-        L.mpla_init_vector(C.byref(x), C.byref(inst), n)
-        L.mpla_init_vector(C.byref(bv), C.byref(inst), n)
-        errs = {k: [] for k in sols}
+        # the FP64 reference of the four columns: plain batched FP64 CG (one
+        # shared FP64 product of the four columns per iteration) to 1e-12
+        B4 = torch.stack([Bd[c * n:(c + 1) * n] for c in cols]).reshape(-1).contiguous()
+        X4 = torch.zeros(len(cols) * n, dtype=torch.float64, device="cuda")
+        rel4 = (C.c_double * len(cols))()
+        torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for c in cols:
-            _native.check(K.mplak_copy_f64(bv.data, Bd.data_ptr() + 8 * c * n, n, stream_ptr))
-            _native.check(K.mplak_fill_f64(x.data, n, 0.0, stream_ptr))
-            L.mpla_kernel_conjugate_gradient_mixed(C.byref(bv), C.byref(A), C.byref(x), 20000, 1e-12,
-                                                   mpla.MPLA_PREC_F64, C.byref(inst))
-            xo = np.empty(n)
-            L.mpla_copy_distributed_vector_to_cpu(xo.ctypes.data, C.byref(x), C.byref(inst))
-            for k, X in sols.items():
-                errs[k].append(float(np.linalg.norm(X[:, c] - xo) / np.linalg.norm(xo)))
+        L.mpla_kernel_block_conjugate_gradient_mixed(B4.data_ptr(), X4.data_ptr(), len(cols), C.byref(A), 20000,
+                                                     1e-12, mpla.MPLA_PREC_F64, rel4, C.byref(inst))
         t_or = time.perf_counter() - t0
+        XO = X4.cpu().numpy().reshape((n, len(cols)), order="F")
+        errs = {k: [float(np.linalg.norm(X[:, c] - XO[:, q]) / np.linalg.norm(XO[:, q])) for q, c in enumerate(cols)]
+                for k, X in sols.items()}
         for k in sols:
             out[k]["rel_error_vs_fp64_columns_0_21_42_63"] = errs[k]
         out["fp64_columns"] = {"columns": list(cols), "tolerance": 1e-12, "seconds": t_or,
-                               "solver": "plain FP64 CG (mpla_kernel_conjugate_gradient_mixed, MPLA_PREC_F64)"}
-        log(f"Matern block: FP64 columns {cols} in {t_or:.1f} s; errors " +
+                               "iterations": inst.last_iterations, "max_rel_residual": max(rel4),
+                               "solver": "plain batched FP64 CG (mpla_kernel_block_conjugate_gradient_mixed, "
+                                         "MPLA_PREC_F64)"}
+        log(f"Matern block: FP64 columns {cols} in {t_or:.1f} s ({inst.last_iterations} iterations); errors " +
             ", ".join(f"{k} {max(v):.2e}" for k, v in errs.items()))
-        L.mpla_free_vector(C.byref(x), C.byref(inst))
-        L.mpla_free_vector(C.byref(bv), C.byref(inst))
+        del B4, X4
     L.mpla_free_generic_matrix(C.byref(A), C.byref(inst))
     del P, Bd, Xd
     torch.cuda.empty_cache()

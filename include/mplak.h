@@ -270,7 +270,7 @@ typedef struct mplak_hm_block {
 
 typedef struct mplak_hm_chunk {
   int32_t leaf;     /* row leaf whose near/far lists apply */
-  int32_t count;    /* rows, 1 .. 64 */
+  int32_t count;    /* rows, 1 .. 128 */
   int64_t r0;       /* first sorted row */
 } mplak_hm_chunk;
 
@@ -321,6 +321,18 @@ int mplak_hm_apply(const mplak_hm_points* pts, const mplak_hm_chunk* chunks, int
                    const int64_t* near_cols, const int64_t* far_ptr, const int32_t* far_blk,
                    const mplak_hm_block* blocks, const int32_t* rank, int32_t kmax, const double* U, const double* t,
                    const double* xs, double* ys, mplak_stream_t stream);
+
+/* The apply of mplak_hm_apply with the inadmissible blocks' entries read
+ * from `stored` instead of evaluated: near range k (CSR order) of leaf l
+ * holds entry (i, j) at stored[near_off[k] + (i - lo_l) + j m_l], with
+ * leaf_range[2 l] = lo_l and leaf_range[2 l + 1] = m_l (the leaf's sorted
+ * rows [lo_l, lo_l + m_l)).  For entries that are expensive to evaluate
+ * (a virtual assembler): the near field is then HBM-bound. */
+int mplak_hm_apply_stored(const mplak_hm_chunk* chunks, int64_t nchunks, const int64_t* leaf_range,
+                          const int64_t* near_ptr, const int64_t* near_cols, const int64_t* near_off,
+                          const double* stored, const int64_t* far_ptr, const int32_t* far_blk,
+                          const mplak_hm_block* blocks, const int32_t* rank, int32_t kmax, const double* U,
+                          const double* t, const double* xs, double* ys, mplak_stream_t stream);
 
 /* y[perm[k]] = ys[k] + nugget * x[perm[k] + shift] (the second term only
  * where 0 <= perm[k] + shift < x_count): the sorted product back in the

@@ -144,6 +144,8 @@ MPLAK_SIGNATURES = {
     "mplak_hm_far_partials": (C.c_int, [_vp, _vp, _i64, _vp, _i32, _vp, _vp, _vp, _vp]),
     "mplak_hm_far_sum": (C.c_int, [_vp, _i64, _vp, _i32, _vp, _vp, _vp]),
     "mplak_hm_apply": (C.c_int, [_vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp]),
+    "mplak_hm_apply_stored": (C.c_int, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp, _vp,
+                                        _vp, _vp]),
     "mplak_hm_scatter_nugget": (C.c_int, [_vp, _vp, _i64, _dbl, _vp, _i64, _i64, _vp, _vp]),
     "mplak_kernel_mrhs_workspace_bytes": (_sz, [C.POINTER(mplak_kernel_desc), C.c_int, _i64]),
     "mplak_kernel_mrhs": (C.c_int, [C.POINTER(mplak_kernel_desc), C.c_int, _i64, _vp, _i64, _vp, _i64, _vp, _sz, _vp,

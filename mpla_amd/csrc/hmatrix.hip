@@ -25,24 +25,28 @@ __global__ __launch_bounds__(kBlock) void gather_scaled_kernel(const double* con
 }
 
 // One workgroup per item: the item's columns against the block's rank terms.
+// Wave w takes the terms q = w, w + 4, ...: its lanes walk the item's columns
+// (coalesced loads of V's column q and of x), then a butterfly sums the
+// lanes.  No barriers; every sum has a fixed order.
 __global__ __launch_bounds__(kHmThreads) void far_partial_kernel(const mplak_hm_block* __restrict__ blocks,
                                                                  const mplak_hm_item* __restrict__ items,
                                                                  const int32_t* __restrict__ rank, int kmax,
                                                                  const double* __restrict__ V,
                                                                  const double* __restrict__ xs,
                                                                  double* __restrict__ part) {
-  __shared__ double red[kHmThreads / kWave];
   const mplak_hm_item it = items[blockIdx.x];
   const mplak_hm_block B = blocks[it.block];
   const int rb = rank[it.block];
   const double* Vb = V + B.voff;
+  const double* x = xs + B.c0;
+  const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
   double* out = part + (int64_t)blockIdx.x * kmax;
-  for (int q = 0; q < rb; ++q) {
+  for (int q = w; q < rb; q += kHmThreads / kWave) {
+    const double* v = Vb + (int64_t)q * B.n;
     double s = 0.0;
-    for (int64_t j = it.j0 + threadIdx.x; j < it.j0 + it.count; j += kHmThreads)
-      s = fma(Vb[j + q * B.n], xs[B.c0 + j], s);
-    s = block_sum<kHmThreads>(s, red);
-    if (threadIdx.x == 0) out[q] = s;
+    for (int64_t j = it.j0 + lane; j < it.j0 + it.count; j += kWave) s = fma(v[j], x[j], s);
+    s = wave_sum(s);
+    if (lane == 0) out[q] = s;
   }
 }
 
@@ -89,7 +93,7 @@ void launch_apply(const hm::KernelEntries<KIND>& e, int64_t nchunks, hipStream_t
                   const int64_t* near_ptr, const int64_t* near_cols, const int64_t* far_ptr, const int32_t* far_blk,
                   const mplak_hm_block* blocks, const int32_t* rank, int kmax, const double* U, const double* t,
                   const double* xs, double* ys) {
-  hipLaunchKernelGGL((hm::apply_kernel<KIND, DIM>), dim3((unsigned)nchunks), dim3(kHmThreads), 0, s, e, chunks,
+  hipLaunchKernelGGL((hm::apply_kernel<KIND, DIM>), dim3((unsigned)nchunks), dim3(hm::kHmApplyThreads), 0, s, e, chunks,
                      near_ptr, near_cols, far_ptr, far_blk, blocks, rank, kmax, U, t, xs, ys);
 }
 
@@ -197,6 +201,22 @@ int mplak_hm_apply(const mplak_hm_points* pts, const mplak_hm_chunk* chunks, int
                                              rank, kmax, U, t, xs, ys, s);
   return apply_kind<MPLAK_KERNEL_MATERN32>(pts, chunks, nchunks, near_ptr, near_cols, far_ptr, far_blk, blocks, rank,
                                            kmax, U, t, xs, ys, s);
+}
+
+int mplak_hm_apply_stored(const mplak_hm_chunk* chunks, int64_t nchunks, const int64_t* leaf_range,
+                          const int64_t* near_ptr, const int64_t* near_cols, const int64_t* near_off,
+                          const double* stored, const int64_t* far_ptr, const int32_t* far_blk,
+                          const mplak_hm_block* blocks, const int32_t* rank, int32_t kmax, const double* U,
+                          const double* t, const double* xs, double* ys, mplak_stream_t stream) {
+  if (nchunks < 0 || nchunks > 0x7fffffff || kmax < 1 || kmax > MPLAK_HM_MAX_RANK) return MPLAK_ERR_ARG;
+  if (nchunks == 0) return MPLAK_OK;
+  if (chunks == nullptr || leaf_range == nullptr || near_ptr == nullptr || far_ptr == nullptr || xs == nullptr ||
+      ys == nullptr)
+    return MPLAK_ERR_ARG;
+  hipLaunchKernelGGL(hm::apply_stored_kernel, dim3((unsigned)nchunks), dim3(hm::kHmApplyThreads), 0, as_stream(stream), chunks,
+                     leaf_range, near_ptr, near_cols, near_off, stored, far_ptr, far_blk, blocks, rank, kmax, U, t, xs,
+                     ys);
+  return launch_status();
 }
 
 int mplak_hm_scatter_nugget(const double* ys, const int64_t* perm, int64_t n, double nugget, const double* x,

@@ -19,9 +19,11 @@ namespace mplak {
 namespace hm {
 
 constexpr int kHmThreads = 256;                 // 4 waves
-constexpr int kHmRows = 64;                     // rows of one apply work item (one per lane)
-constexpr int kHmParts = kHmThreads / kHmRows;  // waves splitting one row's sources
-constexpr int kHmStage = 256;                   // sources staged in LDS at a time
+constexpr int kHmApplyThreads = 512;             // 8 waves per apply work item
+constexpr int kHmLaneRows = 2;                   // rows per lane
+constexpr int kHmRows = kHmLaneRows * kWave;     // rows of one apply work item
+constexpr int kHmParts = kHmApplyThreads / kWave;  // waves splitting one row's sources
+constexpr int kHmStage = 512;                   // sources staged in LDS at a time
 constexpr int kHmMaxDim = 4;
 
 // Signed value of largest magnitude and its index; ties go to the smaller
@@ -190,48 +192,69 @@ __global__ __launch_bounds__(kHmThreads) void aca_kernel(E e, const mplak_hm_blo
   aca_block(e, B, kmax, eps, U, V, rank + blockIdx.x, tab);
 }
 
-// Far field for one row of an apply item: sum over the leaf's admissible
+// Far field for the rows of an apply item: sum over the leaf's admissible
 // blocks b (those of part `part` of kHmParts) of U_b(row - r0_b, :) t_b.
-__device__ __forceinline__ double far_rows(int64_t row, bool valid, const int64_t* __restrict__ far_ptr,
-                                           const int32_t* __restrict__ far_blk, int32_t leaf,
-                                           const mplak_hm_block* __restrict__ blocks,
-                                           const int32_t* __restrict__ rank, int kmax, const double* __restrict__ U,
-                                           const double* __restrict__ t, int part) {
-  double acc = 0.0;
+__device__ __forceinline__ void far_rows(const int64_t (&row)[kHmLaneRows], const bool (&valid)[kHmLaneRows],
+                                         const int64_t* __restrict__ far_ptr, const int32_t* __restrict__ far_blk,
+                                         int32_t leaf, const mplak_hm_block* __restrict__ blocks,
+                                         const int32_t* __restrict__ rank, int kmax, const double* __restrict__ U,
+                                         const double* __restrict__ t, int part, double (&acc)[kHmLaneRows]) {
   const int64_t f0 = far_ptr[leaf], f1 = far_ptr[leaf + 1];
   for (int64_t f = f0 + part; f < f1; f += kHmParts) {
     const int32_t b = far_blk[f];
     const mplak_hm_block B = blocks[b];
     const int rb = rank[b];
     const double* tb = t + (int64_t)b * kmax;
-    if (valid) {
-      const double* u = U + B.uoff + (row - B.r0);
-      for (int q = 0; q < rb; ++q) acc = fma(u[q * B.m], tb[q], acc);
+#pragma unroll
+    for (int h = 0; h < kHmLaneRows; ++h) {
+      if (!valid[h]) continue;
+      const double* u = U + B.uoff + (row[h] - B.r0);
+      double a = acc[h];
+      for (int q = 0; q < rb; ++q) a = fma(u[q * B.m], tb[q], a);
+      acc[h] = a;
     }
   }
-  return acc;
 }
 
 // Adds the kHmParts partial sums of each row in part order and stores them.
-__device__ __forceinline__ void finish_rows(double acc, int64_t row, bool valid, double* __restrict__ ys,
+__device__ __forceinline__ void finish_rows(const double (&acc)[kHmLaneRows], const int64_t (&row)[kHmLaneRows],
+                                            const bool (&valid)[kHmLaneRows], double* __restrict__ ys,
                                             double (*red)[kHmRows]) {
-  const int lane = threadIdx.x % kHmRows, part = threadIdx.x / kHmRows;
-  red[part][lane] = acc;
-  __syncthreads();
-  if (part == 0 && valid) {
-    double s = red[0][lane];
+  const int lane = threadIdx.x % kWave, part = threadIdx.x / kWave;
 #pragma unroll
-    for (int p = 1; p < kHmParts; ++p) s += red[p][lane];
-    ys[row] = s;
+  for (int h = 0; h < kHmLaneRows; ++h) red[part][h * kWave + lane] = acc[h];
+  __syncthreads();
+  if (part == 0) {
+#pragma unroll
+    for (int h = 0; h < kHmLaneRows; ++h) {
+      if (!valid[h]) continue;
+      double v = red[0][h * kWave + lane];
+#pragma unroll
+      for (int p = 1; p < kHmParts; ++p) v += red[p][h * kWave + lane];
+      ys[row[h]] = v;
+    }
   }
 }
 
-// Near field plus far field of one apply item (up to 64 rows of one leaf),
-// closed-form entries: the leaf's inadmissible column ranges are staged in
-// LDS 256 sources at a time; wave p of the workgroup takes sources p, p+4,
-// ... of each stage for its 64 rows.
+// Row h of the lane within the item: c.r0 + 64 h + lane.
+__device__ __forceinline__ void item_rows(const mplak_hm_chunk& c, int64_t (&row)[kHmLaneRows],
+                                          bool (&valid)[kHmLaneRows]) {
+  const int lane = threadIdx.x % kWave;
+#pragma unroll
+  for (int h = 0; h < kHmLaneRows; ++h) {
+    valid[h] = h * kWave + lane < c.count;
+    row[h] = c.r0 + (valid[h] ? h * kWave + lane : 0);
+  }
+}
+
+// Near field plus far field of one apply item (up to 128 rows of one leaf,
+// two per lane), closed-form entries: the leaf's inadmissible column ranges
+// are staged in LDS 512 sources at a time; wave p of the workgroup's 8 takes
+// sources p, p + 8, ... of each stage, and every staged source serves both of
+// the lane's rows (half the LDS reads of one row per lane, at the same 8
+// waves per SIMD).  The eight partial sums of a row are added in wave order.
 template <int KIND, int DIM>
-__global__ __launch_bounds__(kHmThreads) void apply_kernel(
+__global__ __launch_bounds__(kHmApplyThreads) void apply_kernel(
     KernelEntries<KIND> e, const mplak_hm_chunk* __restrict__ chunks, const int64_t* __restrict__ near_ptr,
     const int64_t* __restrict__ near_cols, const int64_t* __restrict__ far_ptr, const int32_t* __restrict__ far_blk,
     const mplak_hm_block* __restrict__ blocks, const int32_t* __restrict__ rank, int kmax,
@@ -244,20 +267,25 @@ __global__ __launch_bounds__(kHmThreads) void apply_kernel(
   const int dim = DIM > 0 ? DIM : e.dim;
   fill_exp2_table(tab);
   const mplak_hm_chunk c = chunks[blockIdx.x];
-  const int lane = threadIdx.x % kHmRows, part = threadIdx.x / kHmRows;
-  const int64_t row = c.r0 + lane;
-  const bool valid = lane < c.count;
-  double p[D];
+  const int part = threadIdx.x / kWave;
+  int64_t row[kHmLaneRows];
+  bool valid[kHmLaneRows];
+  item_rows(c, row, valid);
+  double p[kHmLaneRows][D];
 #pragma unroll
-  for (int d = 0; d < D; ++d) p[d] = (d < dim && valid) ? e.rows[d * e.row_ld + row] : 0.0;
-  double acc = 0.0;
+  for (int h = 0; h < kHmLaneRows; ++h)
+#pragma unroll
+    for (int d = 0; d < D; ++d) p[h][d] = d < dim ? e.rows[d * e.row_ld + row[h]] : 0.0;
+  double acc[kHmLaneRows];
+#pragma unroll
+  for (int h = 0; h < kHmLaneRows; ++h) acc[h] = 0.0;
   const int64_t n0 = near_ptr[c.leaf], n1 = near_ptr[c.leaf + 1];
   for (int64_t k = n0; k < n1; ++k) {
     const int64_t c0 = near_cols[2 * k], cn = near_cols[2 * k + 1];
     for (int64_t base = 0; base < cn; base += kHmStage) {
       const int cnt = (int)(cn - base < kHmStage ? cn - base : kHmStage);
       __syncthreads();
-      if (threadIdx.x < cnt) {
+      if ((int)threadIdx.x < cnt) {
         const int64_t j = c0 + base + threadIdx.x;
 #pragma unroll
         for (int d = 0; d < D; ++d)
@@ -266,25 +294,32 @@ __global__ __launch_bounds__(kHmThreads) void apply_kernel(
       }
       __syncthreads();
       for (int s = part; s < cnt; s += kHmParts) {
-        double q = 0.0;
+        double q[D];
 #pragma unroll
-        for (int d = 0; d < D; ++d)
-          if (d < dim) {
-            const double df = p[d] - src[d][s];
-            q = fma(df, df, q);
-          }
-        acc = fma(kernel_entry<double, KIND>(q, tab), src[D][s], acc);
+        for (int d = 0; d < D; ++d) q[d] = d < dim ? src[d][s] : 0.0;
+        const double xv = src[D][s];
+#pragma unroll
+        for (int h = 0; h < kHmLaneRows; ++h) {
+          double r2 = 0.0;
+#pragma unroll
+          for (int d = 0; d < D; ++d)
+            if (d < dim) {
+              const double df = p[h][d] - q[d];
+              r2 = fma(df, df, r2);
+            }
+          acc[h] = fma(kernel_entry<double, KIND>(r2, tab), xv, acc[h]);
+        }
       }
     }
   }
-  acc += far_rows(row, valid, far_ptr, far_blk, c.leaf, blocks, rank, kmax, U, t, part);
+  far_rows(row, valid, far_ptr, far_blk, c.leaf, blocks, rank, kmax, U, t, part, acc);
   finish_rows(acc, row, valid, ys, red);
 }
 
 // The same with entries from an entry object evaluated per (row, column)
 // (e.g. a virtual system assembler): no staging.
 template <class E>
-__global__ __launch_bounds__(kHmThreads) void apply_generic_kernel(
+__global__ __launch_bounds__(kHmApplyThreads) void apply_generic_kernel(
     E e, const mplak_hm_chunk* __restrict__ chunks, const int64_t* __restrict__ near_ptr,
     const int64_t* __restrict__ near_cols, const int64_t* __restrict__ far_ptr, const int32_t* __restrict__ far_blk,
     const mplak_hm_block* __restrict__ blocks, const int32_t* __restrict__ rank, int kmax,
@@ -297,18 +332,80 @@ __global__ __launch_bounds__(kHmThreads) void apply_generic_kernel(
     __syncthreads();
   }
   const mplak_hm_chunk c = chunks[blockIdx.x];
-  const int lane = threadIdx.x % kHmRows, part = threadIdx.x / kHmRows;
-  const int64_t row = c.r0 + lane;
-  const bool valid = lane < c.count;
-  double acc = 0.0;
-  if (valid) {
-    const int64_t n0 = near_ptr[c.leaf], n1 = near_ptr[c.leaf + 1];
+  const int part = threadIdx.x / kWave;
+  int64_t row[kHmLaneRows];
+  bool valid[kHmLaneRows];
+  item_rows(c, row, valid);
+  double acc[kHmLaneRows];
+  const int64_t n0 = near_ptr[c.leaf], n1 = near_ptr[c.leaf + 1];
+#pragma unroll
+  for (int h = 0; h < kHmLaneRows; ++h) {
+    acc[h] = 0.0;
+    if (!valid[h]) continue;
     for (int64_t k = n0; k < n1; ++k) {
       const int64_t c0 = near_cols[2 * k], cn = near_cols[2 * k + 1];
-      for (int64_t s = part; s < cn; s += kHmParts) acc = fma(e(row, c0 + s, tab), xs[c0 + s], acc);
+      for (int64_t s = part; s < cn; s += kHmParts) acc[h] = fma(e(row[h], c0 + s, tab), xs[c0 + s], acc[h]);
     }
   }
-  acc += far_rows(row, valid, far_ptr, far_blk, c.leaf, blocks, rank, kmax, U, t, part);
+  far_rows(row, valid, far_ptr, far_blk, c.leaf, blocks, rank, kmax, U, t, part, acc);
+  finish_rows(acc, row, valid, ys, red);
+}
+
+// Stored near field: entry (i, j) of near range k of leaf l (rows
+// [lo_l, hi_l), columns [c0, c0 + cn)) at st[off_k + (i - lo_l) + j (hi_l -
+// lo_l)].  One workgroup per near range fills it from an entry object.
+template <class E>
+__global__ __launch_bounds__(kHmThreads) void near_fill_kernel(E e, const int64_t* __restrict__ range_leaf_lo,
+                                                               const int64_t* __restrict__ near_cols,
+                                                               const int64_t* __restrict__ near_off,
+                                                               double* __restrict__ st) {
+  __shared__ double tab[kExp2Table];
+  if (E::kTable) {
+    fill_exp2_table(tab);
+    __syncthreads();
+  }
+  const int64_t k = blockIdx.x;
+  const int64_t lo = range_leaf_lo[2 * k], m = range_leaf_lo[2 * k + 1];
+  const int64_t c0 = near_cols[2 * k], cn = near_cols[2 * k + 1];
+  double* out = st + near_off[k];
+  for (int64_t idx = threadIdx.x; idx < m * cn; idx += kHmThreads) {
+    const int64_t i = idx % m, j = idx / m;
+    out[idx] = e(lo + i, c0 + j, tab);
+  }
+}
+
+// Near field from the stored entries plus the far field (lanes are rows, so
+// every load of the stored block is coalesced; wave p takes every eighth
+// column).
+__global__ __launch_bounds__(kHmApplyThreads) void apply_stored_kernel(
+    const mplak_hm_chunk* __restrict__ chunks, const int64_t* __restrict__ leaf_range,
+    const int64_t* __restrict__ near_ptr, const int64_t* __restrict__ near_cols,
+    const int64_t* __restrict__ near_off, const double* __restrict__ st, const int64_t* __restrict__ far_ptr,
+    const int32_t* __restrict__ far_blk, const mplak_hm_block* __restrict__ blocks, const int32_t* __restrict__ rank,
+    int kmax, const double* __restrict__ U, const double* __restrict__ t, const double* __restrict__ xs,
+    double* __restrict__ ys) {
+  __shared__ double red[kHmParts][kHmRows];
+  const mplak_hm_chunk c = chunks[blockIdx.x];
+  const int part = threadIdx.x / kWave;
+  int64_t row[kHmLaneRows];
+  bool valid[kHmLaneRows];
+  item_rows(c, row, valid);
+  const int64_t lo = leaf_range[2 * c.leaf], m = leaf_range[2 * c.leaf + 1];
+  double acc[kHmLaneRows];
+#pragma unroll
+  for (int h = 0; h < kHmLaneRows; ++h) acc[h] = 0.0;
+  const int64_t n0 = near_ptr[c.leaf], n1 = near_ptr[c.leaf + 1];
+  for (int64_t k = n0; k < n1; ++k) {
+    const int64_t c0 = near_cols[2 * k], cn = near_cols[2 * k + 1];
+    const double* a = st + near_off[k];
+    for (int64_t s = part; s < cn; s += kHmParts) {
+      const double xv = xs[c0 + s];
+#pragma unroll
+      for (int h = 0; h < kHmLaneRows; ++h)
+        if (valid[h]) acc[h] = fma(a[(row[h] - lo) + s * m], xv, acc[h]);
+    }
+  }
+  far_rows(row, valid, far_ptr, far_blk, c.leaf, blocks, rank, kmax, U, t, part, acc);
   finish_rows(acc, row, valid, ys, red);
 }
 

@@ -143,6 +143,7 @@ using namespace mpla_hm;
 
 constexpr int kMaxDim = 4;
 constexpr int64_t kItemColumns = 4096;  // columns per far-field coefficient item
+constexpr int64_t kChunkRows = 128;     // rows per apply item (two per lane of a 64-lane wave)
 
 struct HMatrix {
   int dim = 0;
@@ -172,6 +173,9 @@ struct HMatrix {
   int32_t* far_blk = nullptr;
   int64_t *full_ptr = nullptr, *full_cols = nullptr, *nofar_ptr = nullptr;
   double *xs = nullptr, *ys = nullptr;
+  // stored near field (virtual entries): entries evaluated once at init
+  double* near_st = nullptr;
+  int64_t *near_off = nullptr, *leaf_range = nullptr;
   int64_t nblocks = 0, nitems = 0, nchunks = 0;
   double** coords_host[2] = {nullptr, nullptr};  // the h_matrix_data coords_d arrays
 };
@@ -287,7 +291,7 @@ void setup(mpla_generic_matrix* A, h_matrix_data* data, HMatrix* h, const std::v
     }
   }
   if (blocks.size() > 0x7fffffffu) fail(what, "too many admissible blocks");
-  // CSR lists per leaf, row chunks of up to 64 rows
+  // CSR lists per leaf, row chunks of up to 128 rows
   std::vector<int64_t> near_ptr{0}, near_cols, far_ptr{0}, full_ptr{0}, full_cols, nofar_ptr(nleaf + 1, 0);
   std::vector<int32_t> far_blk;
   std::vector<mplak_hm_chunk> chunks;
@@ -305,8 +309,8 @@ void setup(mpla_generic_matrix* A, h_matrix_data* data, HMatrix* h, const std::v
     }
     full_ptr.push_back((int64_t)full_cols.size() / 2);
     const Cluster& c = rt.nodes[rt.leaves[l]];
-    for (int64_t r0 = c.lo; r0 < c.hi; r0 += 64)
-      chunks.push_back(mplak_hm_chunk{(int32_t)l, (int32_t)std::min<int64_t>(64, c.hi - r0), r0});
+    for (int64_t r0 = c.lo; r0 < c.hi; r0 += kChunkRows)
+      chunks.push_back(mplak_hm_chunk{(int32_t)l, (int32_t)std::min<int64_t>(kChunkRows, c.hi - r0), r0});
   }
   h->info.row_leaves = nleaf;
   h->info.col_leaves = (int64_t)ct.leaves.size();
@@ -376,6 +380,35 @@ void setup(mpla_generic_matrix* A, h_matrix_data* data, HMatrix* h, const std::v
   } else {
     ck(virtual_aca(h->assem, h->gid_r, h->gid_c, h->blocks, h->nblocks, h->kmax, h->eps, h->U, h->V, h->rank, s),
        what);
+  }
+  if (!h->closed_form) {
+    // Virtual entries cost a device call each: evaluate the inadmissible
+    // blocks once and keep them (column-major per near range, leading
+    // dimension the leaf's row count), when they take at most half of the
+    // free device memory; the near field of a product is then HBM-bound.
+    hip_ck(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+    if (8.0 * (double)h->info.near_entries <= 0.5 * (double)free_b) {
+      std::vector<int64_t> range_leaf_lo, near_off, leaf_range;
+      int64_t off = 0;
+      for (int64_t l = 0; l < nleaf; ++l) {
+        const Cluster& c = rt.nodes[rt.leaves[l]];
+        leaf_range.push_back(c.lo);
+        leaf_range.push_back(c.hi - c.lo);
+        for (const auto& nc : near_of[l]) {
+          range_leaf_lo.push_back(c.lo);
+          range_leaf_lo.push_back(c.hi - c.lo);
+          near_off.push_back(off);
+          off += (c.hi - c.lo) * nc.second;
+        }
+      }
+      int64_t* d_rll = upload(h, range_leaf_lo, s);
+      h->near_off = upload(h, near_off, s);
+      h->leaf_range = upload(h, leaf_range, s);
+      h->near_st = dalloc<double>(h, (size_t)off);
+      ck(virtual_near_fill(h->assem, h->gid_r, h->gid_c, d_rll, h->near_cols, h->near_off, (int64_t)near_off.size(),
+                           h->near_st, s),
+         what);
+    }
   }
   std::vector<int32_t> ranks(blocks.size());
   if (!ranks.empty())
@@ -471,6 +504,10 @@ void apply(HMatrix* h, bool full, const double* x, double* y, mpla_instance* ins
     const mplak_hm_points p = points_of(h);
     ck(mplak_hm_apply(&p, h->chunks, h->nchunks, np, nc, fp, h->far_blk, h->blocks, h->rank, h->kmax, h->U, h->t,
                       h->xs, h->ys, s),
+       what);
+  } else if (!full && h->near_st != nullptr) {
+    ck(mplak_hm_apply_stored(h->chunks, h->nchunks, h->leaf_range, h->near_ptr, h->near_cols, h->near_off, h->near_st,
+                             fp, h->far_blk, h->blocks, h->rank, h->kmax, h->U, h->t, h->xs, h->ys, s),
        what);
   } else {
     ck(virtual_apply(h->assem, h->gid_r, h->gid_c, h->chunks, h->nchunks, np, nc, fp, h->far_blk, h->blocks, h->rank,

@@ -49,5 +49,9 @@ int virtual_apply(system_assembler* assem, const int64_t* row_ids, const int64_t
                   const int64_t* far_ptr, const int32_t* far_blk, const mplak_hm_block* blocks, const int32_t* rank,
                   int kmax, const double* U, const double* t, const double* xs, double* ys, void* stream);
 int virtual_rhs(system_assembler* assem, const int64_t* ids, int64_t n, double* b, void* stream);
+// stored near field (layout of mplak_hm_apply_stored): one workgroup per near range
+int virtual_near_fill(system_assembler* assem, const int64_t* row_ids, const int64_t* col_ids,
+                      const int64_t* range_leaf_lo, const int64_t* near_cols, const int64_t* near_off,
+                      int64_t nranges, double* stored, void* stream);
 
 }  // namespace mpla_hm

@@ -49,9 +49,19 @@ int virtual_apply(system_assembler* assem, const int64_t* row_ids, const int64_t
                   int kmax, const double* U, const double* t, const double* xs, double* ys, void* stream) {
   if (nchunks == 0) return MPLAK_OK;
   VirtualEntries e{assem, row_ids, col_ids};
-  hipLaunchKernelGGL(mplak::hm::apply_generic_kernel<VirtualEntries>, dim3((unsigned)nchunks), dim3(kHmThreads), 0,
+  hipLaunchKernelGGL(mplak::hm::apply_generic_kernel<VirtualEntries>, dim3((unsigned)nchunks), dim3(mplak::hm::kHmApplyThreads), 0,
                      static_cast<hipStream_t>(stream), e, chunks, near_ptr, near_cols, far_ptr, far_blk, blocks, rank,
                      kmax, U, t, xs, ys);
+  return status();
+}
+
+int virtual_near_fill(system_assembler* assem, const int64_t* row_ids, const int64_t* col_ids,
+                      const int64_t* range_leaf_lo, const int64_t* near_cols, const int64_t* near_off,
+                      int64_t nranges, double* stored, void* stream) {
+  if (nranges == 0) return MPLAK_OK;
+  VirtualEntries e{assem, row_ids, col_ids};
+  hipLaunchKernelGGL(mplak::hm::near_fill_kernel<VirtualEntries>, dim3((unsigned)nranges), dim3(kHmThreads), 0,
+                     static_cast<hipStream_t>(stream), e, range_leaf_lo, near_cols, near_off, stored);
   return status();
 }
 
