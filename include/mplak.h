@@ -487,7 +487,7 @@ typedef struct mplak_ragged_system {
   int32_t nrhs;
   int32_t precision;     /* MPLAK_PREC_F64 or MPLAK_PREC_F32 */
   int32_t rt, ctiles, tiles, slabs; /* planned */
-  double drop_log2;      /* > 0: drop every 64-target x 256-source block whose entries are all below
+  double drop_log2;      /* > 0: drop every 64-target x 64-source block whose entries are all below
                             2^-drop_log2, as mplak_kernel_matvec_screened (useful on spatially ordered
                             points; no reference counterpart); 0: exact product */
   double drop_s;         /* planned */

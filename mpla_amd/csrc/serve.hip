@@ -64,7 +64,8 @@ __global__ __launch_bounds__(kBlock) void ragged_product_kernel(const mplak_ragg
   const int64_t c1 = (c0 + S.slab_cols < n) ? c0 + S.slab_cols : n;
   // Screening (drop_s > 0): a wave skips a chunk when the bounding boxes of
   // its 64 targets and of the chunk's sources are further apart than drop_s,
-  // i.e. every entry of the block is below the system's drop level.
+  // i.e. every entry of the block is below the system's drop level, and
+  // within a kept chunk each 64-source part whose box is that far.
   const bool screen = S.drop_s > 0.0;
   const int lane = tid & (kWave - 1), w = tid / kWave;
   __shared__ T cbox[kBlock / kWave][2 * MD];
@@ -113,19 +114,38 @@ __global__ __launch_bounds__(kBlock) void ragged_product_kernel(const mplak_ragg
       }
       if (__builtin_amdgcn_readfirstlane((int)(d2 > (T)S.drop_s)) != 0) continue;
     }
+    // then per 64-source part of the chunk (wave q staged part q and its box):
+    // the wave skips the parts whose box is too far from its targets
+    uint32_t far_parts = 0;
+    if (screen) {
+#pragma unroll
+      for (int q = 0; q < kBlock / kWave; ++q) {
+        T d2 = T(0);
+#pragma unroll
+        for (int d = 0; d < MD; ++d) {
+          const T gap = fmax(T(0), fmax(cbox[q][d] - tbox[MD + d], tbox[d] - cbox[q][MD + d]));
+          d2 = fma(gap, gap, d2);
+        }
+        if (__builtin_amdgcn_readfirstlane((int)(d2 > (T)S.drop_s)) != 0) far_parts |= 1u << q;
+      }
+    }
     T cacc[RT];
 #pragma unroll
     for (int q = 0; q < RT; ++q) cacc[q] = T(0);
-    for (int jj = 0; jj < kChunk; ++jj) {
-      T s = T(0);
+#pragma unroll 1
+    for (int part = 0; part < kBlock / kWave; ++part) {
+      if ((far_parts >> part) & 1u) continue;
+      for (int jj = part * kWave; jj < (part + 1) * kWave; ++jj) {
+        T s = T(0);
 #pragma unroll
-      for (int d = 0; d < MD; ++d) {
-        const T df = pi[d] - lp[d][jj];
-        s = fma(df, df, s);
+        for (int d = 0; d < MD; ++d) {
+          const T df = pi[d] - lp[d][jj];
+          s = fma(df, df, s);
+        }
+        const T k = kernel_entry_rt<T>(S.kind, s, exp_tab);
+#pragma unroll
+        for (int q = 0; q < RT; ++q) cacc[q] = fma(k, lv[q][jj], cacc[q]);
       }
-      const T k = kernel_entry_rt<T>(S.kind, s, exp_tab);
-#pragma unroll
-      for (int q = 0; q < RT; ++q) cacc[q] = fma(k, lv[q][jj], cacc[q]);
     }
 #pragma unroll
     for (int q = 0; q < RT; ++q) acc[q] += (double)cacc[q];

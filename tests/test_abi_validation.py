@@ -111,3 +111,27 @@ def test_multi_rhs_validation(lib):
     assert lib.mplak_kernel_mrhs(C.byref(d), 0, 0, None, 10, None, 10, None, 0, None, None) == OK
     assert lib.mplak_bcg_workspace_bytes(10, 0) >= 0
     assert lib.mplak_bcg_start(10, 0, None, None, None, None, None, None, 1, 1e-8, 10, 1.0, None, 0, None) == ARG
+
+
+def test_hmatrix_kernels_validate_before_launch(lib):
+    """The H-matrix kernels (mplak_hm_*) refuse malformed arguments and accept
+    empty work without touching the device."""
+    assert lib.mplak_hm_aca(None, None, 0, 20, 0.0, None, None, None, None) == ARG  # no point description
+    assert lib.mplak_hm_far_partials(None, None, 0, None, 20, None, None, None, None) == OK
+    assert lib.mplak_hm_far_partials(None, None, -1, None, 20, None, None, None, None) == ARG
+    assert lib.mplak_hm_far_partials(None, None, 1, None, 33, None, None, None, None) == ARG  # rank > 32
+    assert lib.mplak_hm_far_sum(None, 0, None, 20, None, None, None) == OK
+    assert lib.mplak_hm_far_sum(None, 3, None, 0, None, None, None) == ARG
+    assert lib.mplak_hm_apply_stored(None, 0, None, None, None, None, None, None, None, None, None, 20, None, None,
+                                     None, None, None) == OK
+    assert lib.mplak_hm_apply_stored(None, 2, None, None, None, None, None, None, None, None, None, 20, None, None,
+                                     None, None, None) == ARG
+    assert lib.mplak_hm_scatter_nugget(None, None, 0, 0.0, None, 0, 0, None, None) == OK
+    assert lib.mplak_hm_scatter_nugget(None, None, 4, 1.0, None, 0, 4, None, None) == ARG
+    assert lib.mplak_hm_gather_scaled(None, 3, 0, None, 10, 0, 0.5, None, 10, None) == ARG
+
+
+def test_noalloc_refuses_malformed_descriptors(lib):
+    d = desc(rows=-1)
+    assert lib.mplak_kernel_matvec_noalloc(C.byref(d), 0, None, None, None, None) == ARG
+    assert lib.mplak_kernel_matvec_noalloc_max_bytes() == 80 << 20

@@ -344,3 +344,36 @@ def test_hmatrix_cg_reaches_tolerance(s, oracle):
               f"solution error {np.linalg.norm(xs - xo) / np.linalg.norm(xo):.2e}")
     finally:
         s.lib.mpla_destroy_hmglib(C.byref(A), s.I)
+
+
+@pytest.mark.gpu
+def test_hmatrix_two_point_sets(s, oracle):
+    """Rows and columns from different point sets (the reference's
+    all_coords[0] / all_coords[1]): the product against the dense product
+    between the two sets (the oracle on the concatenated points, rows 0..m,
+    columns m..m+n), within the 2-D compression bar."""
+    m, n, dim = 3000, 2200, 2
+    pr = oracle.uniform(dim * m, 79, 0).reshape(dim, m)
+    pc = oracle.uniform(dim * n, 79, 1).reshape(dim, n) * 0.8 + 0.1
+    xh = oracle.uniform(n, 79, 2, -1, 1)
+    A = s.generic(m, n, 0)
+    data = mpla.h_matrix_data()
+    A.data = C.addressof(data)
+    R, Cc = _Coords(pr), _Coords(pc)
+    coords = (C.c_void_p * 2)(C.cast(R.ptrs, C.c_void_p), C.cast(Cc.ptrs, C.c_void_p))
+    s.lib.mpla_init_hmatrix_kernel(C.byref(A), coords, dim, 0, 0.5, 0.0, 1.5, 128, 20, 0.0, s.I)
+    try:
+        x = s.vector(n, xh)
+        y = s.vector(m, np.zeros(m))
+        for core in ("mpla_dgemv_core_hmglib_h_matrix", "mpla_dgemv_core_hmglib_full_matrix"):
+            s.lib.mpla_generic_dgemv(C.byref(y), C.byref(A), C.byref(x), mpla.core(core), s.I)
+            got = s.get(y)
+            both = np.ascontiguousarray(np.concatenate([pr, pc], axis=1))
+            ref = oracle.Kernel(both, 0, 0.5).matvec(xh, row_off=0, rows=m, col_off=m, cols=n)
+            bar = 1e-8 if core.endswith("h_matrix") else 1e-12
+            assert np.linalg.norm(got - ref) <= bar * np.linalg.norm(ref), core
+        info = mpla.mpla_hmatrix_info()
+        s.lib.mpla_hmatrix_get_info(C.byref(A), C.byref(info))
+        assert info.near_entries + info.far_entries == m * n and info.far_blocks > 0
+    finally:
+        s.lib.mpla_destroy_hmglib(C.byref(A), s.I)
