@@ -1127,6 +1127,17 @@ def main():
         "cpu_baseline": cpu,
         "configs": configs,
     }
+    kv = configs.get("matern_block_n131072", {}).get("kv_product", {})
+    if "bf16" in kv:
+        k16 = kv["bf16"]
+        out["roofline_mrhs"] = {
+            "bound": "valu", "kernel": "mrhs_mfma_kernel (BF16 MFMA, FP32 entries), Matern N=131072, R=64",
+            "achieved": k16["entry_tflops"], "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
+            "frac": k16["entry_frac_valu"], "launch_ms": k16["ms"],
+            "contraction_tflops": k16["contraction_tflops"], "contraction_frac_mfma": k16["contraction_frac_mfma"],
+            "basis": "entry generation 16 FLOP x N^2 per K*V (the bound: each entry is used by 64 columns on the "
+                     "matrix cores); contraction 2 N^2 R against the 2.5 PF dense BF16 MFMA peak",
+            "by_precision": kv}
     if dgemv is not None:
         out["roofline_dgemv"] = {"bound": "hbm", "achieved": dgemv["gbps"], "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                                  "frac": dgemv["frac_hbm"], "traffic": committed_traffic("dgemv_n65536")}

@@ -369,6 +369,7 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
   ChunkRunner runner(inst, true, chunk, iteration, normalize, ordered ? &rt->mixed_chunk : nullptr, key);
 
   bool stagnated = false;
+  std::vector<int> step_chunks;
   for (;;) {
     // r = b - A_f64 x, relative to ||b||: the true FP64 residual
     if (x_zero) {
@@ -418,10 +419,22 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
     hip_ck(hipMemcpyAsync(st, rt->pinned, sizeof(inner), hipMemcpyHostToDevice, rt->stream), "hipMemcpyAsync");
     ck(mplak_fill_f64(e, n, 0.0, s), "mixed CG");
     mplak_cg_state ist = inner;
+    // the chunks this step took in the last solve of this size are enqueued
+    // before the state is first read back (one host round trip instead of one
+    // per chunk when the solves are alike); then one chunk per read
+    int hint = 1;
+    if (rt->mixed_hint_n == n && outer < (int)rt->mixed_hint_chunks.size())
+      hint = std::max(1, rt->mixed_hint_chunks[outer]);
+    int chunks_run = 0;
     while (ist.done == MPLAK_RUNNING) {
-      runner.run();
+      const int batch = chunks_run == 0 ? hint : 1;
+      for (int c = 0; c < batch; ++c) runner.run();
+      chunks_run += batch;
       ist = *static_cast<mplak_cg_state*>(read_back(inst, st, sizeof(mplak_cg_state)));
     }
+    // chunks this step needed: the iterations it ran (+1 for the converging
+    // product of the single-reduction form) over the chunk length
+    step_chunks.push_back((ist.k + 1 + chunk - 1) / chunk);
     total_inner += ist.k;
     // the single-reduction form applies the operator once more than it
     // updates when it converges (the product whose <r,r> meets the test)
@@ -441,6 +454,10 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
   }
 
   unpermute();
+  if (ordered) {
+    rt->mixed_hint_n = n;
+    rt->mixed_hint_chunks = step_chunks;
+  }
   int iterations = total_inner;
   double rel = cur.rel;
   bool converged = cur.rel < epsilon;
