@@ -487,21 +487,37 @@ typedef struct mplak_ragged_system {
   int32_t nrhs;
   int32_t precision;     /* MPLAK_PREC_F64 or MPLAK_PREC_F32 */
   int32_t rt, ctiles, tiles, slabs; /* planned */
-  double drop_log2;      /* > 0: drop every 64-target x 64-source block whose entries are all below
-                            2^-drop_log2, as mplak_kernel_matvec_screened (useful on spatially ordered
-                            points; no reference counterpart); 0: exact product */
+  double drop_log2;      /* > 0: drop every block of entries that are all below 2^-drop_log2 (64 targets
+                            x 64 sources in the slab scheme, 128 x 64 in the pair scheme), as
+                            mplak_kernel_matvec_screened (useful on spatially ordered points; no
+                            reference counterpart); 0: exact product */
   double drop_s;         /* planned */
+  int32_t scheme;        /* planned: MPLAK_RAGGED_SLABS or MPLAK_RAGGED_PAIRS */
+  int32_t reserved;
 } mplak_ragged_system;
+
+/* Work split of a ragged product.
+ * SLABS: item = (target tile of 256 rows, source slab, column tile); every
+ *   entry is evaluated once per column tile.
+ * PAIRS: the points in tiles of 512 (`tiles` of them), item = (tile pair
+ *   I <= J as pair index J (J + 1) / 2 + I, column tile); an off-diagonal
+ *   block is evaluated once and used for the rows of I (sums over J) and the
+ *   rows of J (sums over I).  Planned for n >= 8192; MPLAK_RAGGED_PAIRS=0 / 1
+ *   selects SLABS / PAIRS for every n. */
+#define MPLAK_RAGGED_SLABS 0
+#define MPLAK_RAGGED_PAIRS 1
 
 typedef struct mplak_ragged_item {
   int32_t sys;   /* index into the system array */
-  int32_t tile;  /* product: target tile of 256 rows; finish: block of 256 entries; CG: vector block */
-  int32_t slab;  /* product: source slab; CG: column */
+  int32_t tile;  /* product: target tile of 256 rows (SLABS) or tile pair (PAIRS); finish: block of 256
+                    entries; CG: vector block */
+  int32_t slab;  /* product: source slab (SLABS; 0 for PAIRS); CG: column */
   int32_t ctile; /* product: column tile of rt right-hand sides */
 } mplak_ragged_item;
 
 /* Host only: fills the planned fields of *s from (n, nrhs, precision, kind,
- * length_scale) and returns the bytes the system's `part` buffer needs. */
+ * length_scale, drop_log2) and returns the bytes the system's `part` buffer
+ * needs. */
 size_t mplak_ragged_plan(mplak_ragged_system* s);
 
 /* Host only: number of product / finish work items of a planned system; when
@@ -509,13 +525,14 @@ size_t mplak_ragged_plan(mplak_ragged_system* s);
 int64_t mplak_ragged_product_items(const mplak_ragged_system* s, mplak_ragged_item* out, int32_t sys_index);
 int64_t mplak_ragged_finish_items(const mplak_ragged_system* s, mplak_ragged_item* out, int32_t sys_index);
 
-/* Product partials for `count` items whose systems share rt and precision
- * (systems and items are device arrays); then mplak_ragged_finish adds the
- * slabs in order and the nugget term: Y = (K + nugget I) V per system.
- * dim: 1..4 when every system of the launch has that dimension (a kernel
- * specialised for it), 0 for any mix of dimensions. */
+/* Product partials for `count` items whose systems share rt, precision and
+ * scheme (systems and items are device arrays); then mplak_ragged_finish adds
+ * the partials in a fixed order and the nugget term: Y = (K + nugget I) V per
+ * system.  dim: 1..4 when every system of the launch has that dimension (a
+ * kernel specialised for it; PAIRS items of a system of another dimension
+ * are left undone), 0 for any mix of dimensions (SLABS only). */
 int mplak_ragged_product(const mplak_ragged_system* systems, const mplak_ragged_item* items, int64_t count, int rt,
-                         int dim, int precision, mplak_stream_t stream);
+                         int dim, int precision, int scheme, mplak_stream_t stream);
 int mplak_ragged_finish(const mplak_ragged_system* systems, const mplak_ragged_item* items, int64_t count,
                         mplak_stream_t stream);
 

@@ -39,6 +39,8 @@ MPLAK_RAGGED_DZ = 3
 MPLAK_RAGGED_UPDATE = 4
 MPLAK_RAGGED_DIRECT = 5
 MPLAK_RAGGED_AXPY = 6
+MPLAK_RAGGED_SLABS = 0
+MPLAK_RAGGED_PAIRS = 1
 MPLAK_RAGGED_FILL = 7
 
 
@@ -83,6 +85,37 @@ class mplak_cg_state(C.Structure):
 
 class mplak_bcg_group(C.Structure):
     _fields_ = [("all_done", C.c_int32), ("active", C.c_int32), ("pad", C.c_int32 * 2)]
+
+
+class mplak_ragged_system(C.Structure):
+    _fields_ = [
+        ("points", C.c_void_p),
+        ("V", C.c_void_p),
+        ("Y", C.c_void_p),
+        ("part", C.c_void_p),
+        ("skip", C.c_void_p),
+        ("n", C.c_int64),
+        ("slab_cols", C.c_int64),
+        ("length_scale", C.c_double),
+        ("nugget", C.c_double),
+        ("scale", C.c_double),
+        ("dim", C.c_int32),
+        ("kind", C.c_int32),
+        ("nrhs", C.c_int32),
+        ("precision", C.c_int32),
+        ("rt", C.c_int32),
+        ("ctiles", C.c_int32),
+        ("tiles", C.c_int32),
+        ("slabs", C.c_int32),
+        ("drop_log2", C.c_double),
+        ("drop_s", C.c_double),
+        ("scheme", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+class mplak_ragged_item(C.Structure):
+    _fields_ = [("sys", C.c_int32), ("tile", C.c_int32), ("slab", C.c_int32), ("ctile", C.c_int32)]
 
 
 class mplak_bicgstab_state(C.Structure):
@@ -163,7 +196,7 @@ MPLAK_SIGNATURES = {
     "mplak_ragged_plan": (_sz, [_vp]),
     "mplak_ragged_product_items": (_i64, [_vp, _vp, _i32]),
     "mplak_ragged_finish_items": (_i64, [_vp, _vp, _i32]),
-    "mplak_ragged_product": (C.c_int, [_vp, _vp, _i64, C.c_int, C.c_int, C.c_int, _vp]),
+    "mplak_ragged_product": (C.c_int, [_vp, _vp, _i64, C.c_int, C.c_int, C.c_int, C.c_int, _vp]),
     "mplak_ragged_finish": (C.c_int, [_vp, _vp, _i64, _vp]),
     "mplak_ragged_cg_blocks": (_i32, [_i64]),
     "mplak_ragged_cg_part_bytes": (_sz, [_i64, _i32]),

@@ -128,6 +128,7 @@ struct Upload {
     double value;
     size_t first;
     int64_t count;
+    int scheme = MPLAK_RAGGED_SLABS;  // product work split
   };
   std::vector<Launch> launches;
   void clear() {
@@ -202,7 +203,7 @@ void submit_upload(mpla_server* s, Upload& u) {
   for (const auto& L : u.launches) {
     const mplak_ragged_item* it = dit + L.first;
     if (L.kind == 0)
-      ck(mplak_ragged_product(dps, it, L.count, L.rt, L.op, L.precision, s->stream), "server product");
+      ck(mplak_ragged_product(dps, it, L.count, L.rt, L.op, L.precision, L.scheme, s->stream), "server product");
     else if (L.kind == 1)
       ck(mplak_ragged_finish(dps, it, L.count, s->stream), "server finish");
     else
@@ -255,20 +256,22 @@ void add_products(Upload& u, const std::vector<mplak_ragged_system>& systems, in
   if (systems.empty()) return;
   const size_t base = u.psys.size();
   for (const auto& sy : systems) u.psys.push_back(sy);
-  // one launch per (rt, dim) class: kernels specialised for both
-  for (int rt : {1, 2, 4, 8})
-    for (int dim = 1; dim <= MPLA_SERVER_MAX_DIM; ++dim) {
-      const size_t first = u.items.size();
-      for (size_t k = 0; k < systems.size(); ++k) {
-        if (systems[k].rt != rt || systems[k].dim != dim) continue;
-        const int64_t c = mplak_ragged_product_items(&systems[k], nullptr, 0);
-        const size_t at = u.items.size();
-        u.items.resize(at + (size_t)c);
-        mplak_ragged_product_items(&systems[k], u.items.data() + at, (int32_t)(base + k));
+  // one launch per (scheme, rt, dim) class: kernels specialised for each;
+  // the pair scheme (large systems, heavier items) first
+  for (int scheme : {MPLAK_RAGGED_PAIRS, MPLAK_RAGGED_SLABS})
+    for (int rt : {1, 2, 4, 8})
+      for (int dim = 1; dim <= MPLA_SERVER_MAX_DIM; ++dim) {
+        const size_t first = u.items.size();
+        for (size_t k = 0; k < systems.size(); ++k) {
+          if (systems[k].scheme != scheme || systems[k].rt != rt || systems[k].dim != dim) continue;
+          const int64_t c = mplak_ragged_product_items(&systems[k], nullptr, 0);
+          const size_t at = u.items.size();
+          u.items.resize(at + (size_t)c);
+          mplak_ragged_product_items(&systems[k], u.items.data() + at, (int32_t)(base + k));
+        }
+        const int64_t count = (int64_t)(u.items.size() - first);
+        if (count > 0) u.launches.push_back({0, dim, rt, precision, 0, 0.0, first, count, scheme});
       }
-      const int64_t count = (int64_t)(u.items.size() - first);
-      if (count > 0) u.launches.push_back({0, dim, rt, precision, 0, 0.0, first, count});
-    }
   const size_t first = u.items.size();
   for (size_t k = 0; k < systems.size(); ++k) {
     const int64_t c = mplak_ragged_finish_items(&systems[k], nullptr, 0);

@@ -268,36 +268,6 @@ inline int sym_mode() {
   return atoi(v) != 0 ? 1 : 0;
 }
 
-template <typename T>
-struct Quad;
-template <>
-struct Quad<float> {
-  typedef float4 type;
-};
-template <>
-struct Quad<double> {
-  typedef double4 type;
-};
-
-__device__ __forceinline__ float wave_rotate(float v) {
-  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x134 /* wave_rol:1 */, 0xF, 0xF, false));
-}
-__device__ __forceinline__ double wave_rotate(double v) {
-  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x134, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x134, 0xF, 0xF, false);
-  return __hiloint2double(hi, lo);
-}
-
-__device__ __forceinline__ int64_t sym_pair_index(int64_t I, int64_t J) { return J * (J + 1) / 2 + I; }
-
-// Tile pair p = J (J + 1) / 2 + I, I <= J.
-__device__ __forceinline__ void decode_pair(int64_t p, int64_t& I, int64_t& J) {
-  J = (int64_t)((sqrt(8.0 * (double)p + 1.0) - 1.0) * 0.5);
-  while (J * (J + 1) / 2 > p) --J;
-  while ((J + 1) * (J + 2) / 2 <= p) ++J;
-  I = p - J * (J + 1) / 2;
-}
-
 // Squared distance between two axis-aligned boxes (lo at [0..2], hi at [3..5]).
 template <typename T, int DIM>
 __device__ __forceinline__ T box_gap2(const T* a, const T* b) {

@@ -9,13 +9,17 @@
 // precision), never on what else is in the batch, so a system's arithmetic,
 // and therefore its result, is bitwise the same alone and in any mix.
 //
-// Products: the VALU scheme of the multi-RHS product (one target per thread,
-// RT columns per thread, sources through LDS, FP64 slab partials) with the
-// kernel function, dimension (<= 4) and scale read from the system record.
+// Products, two work splits (mplak_ragged_plan picks one per system from its
+// n alone): SLABS, the VALU scheme of the multi-RHS product (one target per
+// thread, RT columns per thread, sources through LDS, FP64 slab partials);
+// PAIRS, the symmetric tile-pair form (each off-diagonal entry evaluated once
+// for both of its rows).  The kernel function, dimension (<= 4) and scale
+// come from the system record.
 // CG: the batched recurrences of the multi-RHS solver with per-column
 // scalars in a state pool, items (system, column, vector block).
 
 #include <math.h>
+#include <stdlib.h>
 
 #include "common.hpp"
 #include "entry.hpp"
@@ -157,6 +161,251 @@ __global__ __launch_bounds__(kBlock) void ragged_product_kernel(const mplak_ragg
   }
 }
 
+// ---- pair scheme ------------------------------------------------------------
+//
+// The system's points in tiles of 512; item (pair p = (I, J), I <= J, column
+// tile).  Each wave holds 128 targets of tile I (two per lane) and walks the
+// eight 64-source groups of tile J, all 512 sources staged in LDS with their
+// RT values.  Off the diagonal, lane l takes source (l + s) mod 64 at step s:
+// one evaluation of each entry feeds the row sums of its two targets and a
+// column sum per right-hand side that rotates one lane per step, so after the
+// group lane l holds the column sums of source l over the wave's targets.
+// The four waves' column sums of a group are added in wave order (FP64) and
+// stored; the diagonal pair evaluates its whole block for the row sums.
+//
+// Partials (entry precision T): rows of I at part_i[(p R + r) 512 + k], rows
+// of J at part_j[...] (same shape, after part_i), one flag per pair after
+// both: 0 when every 128 x 64 block of the pair is screened out (nothing else
+// written), 1 otherwise.  The finish adds, for a row of tile X, the pairs
+// (X, P) over P = 0 .. tiles - 1 in order.
+
+constexpr int kPairTile = 2 * kBlock;
+constexpr int kPairGroups = kPairTile / kWave;
+constexpr int64_t kRagPairMinN = 8192;
+
+__host__ __device__ inline int64_t pair_count(int64_t tiles) { return tiles * (tiles + 1) / 2; }
+
+template <typename T>
+__device__ __forceinline__ T* pair_rows(const mplak_ragged_system& S) {
+  return reinterpret_cast<T*>(S.part);
+}
+template <typename T>
+__device__ __forceinline__ int32_t* pair_flags(const mplak_ragged_system& S) {
+  const int64_t bytes = 2 * pair_count(S.tiles) * (int64_t)S.nrhs * kPairTile * (int64_t)sizeof(T);
+  return reinterpret_cast<int32_t*>(reinterpret_cast<char*>(S.part) + (bytes + 255) / 256 * 256);
+}
+
+template <typename T, int RT, int DIM>
+__global__ __launch_bounds__(kBlock) void ragged_pair_kernel(const mplak_ragged_system* __restrict__ systems,
+                                                             const mplak_ragged_item* __restrict__ items) {
+  typedef typename Quad<T>::type Q;
+  const mplak_ragged_item it = items[blockIdx.x];
+  const mplak_ragged_system S = systems[it.sys];
+  // a launch specialised for DIM reads DIM coordinates: other systems are not its items
+  if ((S.skip != nullptr && *S.skip != 0) || S.dim != DIM) return;
+  __shared__ Q src[kPairTile];       // scaled source coordinates
+  __shared__ T sval[kPairTile][RT];  // source values, column tile
+  __shared__ T gbox[kPairGroups][2 * DIM];
+  __shared__ T red[kBlock / kWave][RT][kWave];
+  __shared__ uint32_t wkeep[kBlock / kWave];
+  __shared__ double exp_tab[Exp2Lds<T>::size];
+  if (sizeof(T) == 8) fill_exp2_table(exp_tab);  // first read after the staging barrier
+
+  const int64_t n = S.n, R = S.nrhs;
+  const int64_t p = it.tile;
+  int64_t I, J;
+  decode_pair(p, I, J);
+  const int64_t r0 = (int64_t)it.ctile * RT;
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), w = tid / kWave;
+  const double scale = S.scale;
+
+  // every global load of the staging is issued before the first use
+  double tc[2][DIM], tv[2][RT], sc[2][DIM], sv[2][RT];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int64_t i = I * kPairTile + w * 2 * kWave + h * kWave + lane;
+    const int64_t ic = i < n ? i : n - 1;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) tc[h][d] = S.points[d * n + ic];
+#pragma unroll
+    for (int q = 0; q < RT; ++q) tv[h][q] = (i < n && r0 + q < R) ? S.V[(r0 + q) * n + i] : 0.0;
+  }
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int64_t j = J * kPairTile + u * kBlock + tid;
+    const int64_t jc = j < n ? j : n - 1;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) sc[u][d] = S.points[d * n + jc];
+#pragma unroll
+    for (int q = 0; q < RT; ++q) sv[u][q] = (j < n && r0 + q < R) ? S.V[(r0 + q) * n + j] : 0.0;
+  }
+  T pi[2][DIM], xi[2][RT];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) pi[h][d] = (T)(tc[h][d] * scale);
+#pragma unroll
+    for (int q = 0; q < RT; ++q) xi[h][q] = (T)tv[h][q];
+  }
+  const bool screen = S.drop_s > 0.0;
+  T tbox[2 * DIM];
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int k = u * kBlock + tid;  // group k / 64 = 4 u + w: one group per wave and pass
+    T c[4] = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) c[d] = (T)(sc[u][d] * scale);
+    Q q;
+    q.x = c[0], q.y = c[1], q.z = c[2], q.w = c[3];
+    src[k] = q;
+#pragma unroll
+    for (int r = 0; r < RT; ++r) sval[k][r] = (T)sv[u][r];
+    if (screen) {  // padded sources repeat point n - 1 and do not widen the box
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        const T lo = wave_min(c[d]), hi = wave_max(c[d]);
+        if (lane == 0) {
+          gbox[k / kWave][d] = lo;
+          gbox[k / kWave][DIM + d] = hi;
+        }
+      }
+    }
+  }
+  if (screen) {
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      tbox[d] = wave_min(fmin(pi[0][d], pi[1][d]));
+      tbox[DIM + d] = wave_max(fmax(pi[0][d], pi[1][d]));
+    }
+  }
+  __syncthreads();
+  // groups this wave keeps (bit g); a pair nobody keeps writes only its flag
+  uint32_t keep = (1u << kPairGroups) - 1u;
+  if (screen) {
+    keep = 0u;
+#pragma unroll
+    for (int g = 0; g < kPairGroups; ++g) {
+      T d2 = T(0);
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        const T gap = fmax(T(0), fmax(gbox[g][d] - tbox[DIM + d], tbox[d] - gbox[g][DIM + d]));
+        d2 = fma(gap, gap, d2);
+      }
+      if (__builtin_amdgcn_readfirstlane((int)(d2 > (T)S.drop_s)) == 0) keep |= 1u << g;
+    }
+    if (lane == 0) wkeep[w] = keep;
+    __syncthreads();
+    const uint32_t any = wkeep[0] | wkeep[1] | wkeep[2] | wkeep[3];
+    if (any == 0u) {
+      if (tid == 0) pair_flags<T>(S)[p] = 0;
+      return;
+    }
+  }
+  if (tid == 0) pair_flags<T>(S)[p] = 1;
+
+  T* part_i = pair_rows<T>(S);
+  T* part_j = part_i + pair_count(S.tiles) * R * kPairTile;
+  const int kind = S.kind;
+  T acc[2][RT];
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int q = 0; q < RT; ++q) acc[h][q] = T(0);
+  auto entries = [&](const Q& q, T (&e)[2]) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const T dx = pi[h][0] - q.x;
+      T s = dx * dx;
+      if constexpr (DIM > 1) {
+        const T dy = pi[h][1] - q.y;
+        s = fma(dy, dy, s);
+      }
+      if constexpr (DIM > 2) {
+        const T dz = pi[h][2] - q.z;
+        s = fma(dz, dz, s);
+      }
+      if constexpr (DIM > 3) {
+        const T dw = pi[h][3] - q.w;
+        s = fma(dw, dw, s);
+      }
+      e[h] = kernel_entry_rt<T>(kind, s, exp_tab);
+    }
+  };
+
+  if (I == J) {
+#pragma unroll 1
+    for (int g = 0; g < kPairGroups; ++g) {
+      if (((keep >> g) & 1u) == 0u) continue;
+#pragma unroll 2
+      for (int s = 0; s < kWave; ++s) {
+        const int jj = g * kWave + s;
+        T e[2];
+        entries(src[jj], e);
+#pragma unroll
+        for (int q = 0; q < RT; ++q) {
+          const T v = sval[jj][q];
+          acc[0][q] = fma(e[0], v, acc[0][q]);
+          acc[1][q] = fma(e[1], v, acc[1][q]);
+        }
+      }
+    }
+  } else {
+#pragma unroll 1
+    for (int g = 0; g < kPairGroups; ++g) {
+      T t[RT];
+#pragma unroll
+      for (int q = 0; q < RT; ++q) t[q] = T(0);
+      if ((keep >> g) & 1u) {
+#pragma unroll 2
+        for (int s = 0; s < kWave; ++s) {
+          const int jj = g * kWave + ((lane + s) & (kWave - 1));
+          T e[2];
+          entries(src[jj], e);
+#pragma unroll
+          for (int q = 0; q < RT; ++q) {
+            const T v = sval[jj][q];
+            acc[0][q] = fma(e[0], v, acc[0][q]);
+            acc[1][q] = fma(e[1], v, acc[1][q]);
+            t[q] = fma(e[1], xi[1][q], fma(e[0], xi[0][q], t[q]));
+            t[q] = wave_rotate(t[q]);
+          }
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < RT; ++q) red[w][q][lane] = t[q];
+      __syncthreads();
+      for (int v = tid; v < RT * kWave; v += kBlock) {
+        const int q = v / kWave, l = v % kWave;
+        double s = (double)red[0][q][l];
+#pragma unroll
+        for (int ww = 1; ww < kBlock / kWave; ++ww) s += (double)red[ww][q][l];
+        if (r0 + q < R) part_j[(p * R + r0 + q) * kPairTile + g * kWave + l] = (T)s;
+      }
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int q = 0; q < RT; ++q)
+      if (r0 + q < R) part_i[(p * R + r0 + q) * kPairTile + w * 2 * kWave + h * kWave + lane] = acc[h][q];
+}
+
+template <typename T>
+__device__ __forceinline__ double pair_finish_row(const mplak_ragged_system& S, int64_t i, int64_t r) {
+  const int64_t X = i / kPairTile, t = i % kPairTile, R = S.nrhs;
+  const T* part_i = pair_rows<T>(S);
+  const T* part_j = part_i + pair_count(S.tiles) * R * kPairTile;
+  const int32_t* flags = pair_flags<T>(S);
+  double s = 0.0;
+  for (int64_t P = 0; P < S.tiles; ++P) {
+    const int64_t p = X <= P ? sym_pair_index(X, P) : sym_pair_index(P, X);
+    if (flags[p] == 0) continue;
+    s += (double)(X <= P ? part_i[(p * R + r) * kPairTile + t] : part_j[(p * R + r) * kPairTile + t]);
+  }
+  return s;
+}
+
 // items: (system, block) over n * nrhs entries of Y, 256 per block
 __global__ __launch_bounds__(kBlock) void ragged_finish_kernel(const mplak_ragged_system* __restrict__ systems,
                                                                const mplak_ragged_item* __restrict__ items) {
@@ -167,8 +416,13 @@ __global__ __launch_bounds__(kBlock) void ragged_finish_kernel(const mplak_ragge
   const int64_t n = S.n, R = S.nrhs;
   if (g >= n * R) return;
   const int64_t i = g % n, r = g / n;
-  double s = S.part[r * n + i];
-  for (int64_t k = 1; k < S.slabs; ++k) s += S.part[(k * R + r) * n + i];
+  double s;
+  if (S.scheme == MPLAK_RAGGED_PAIRS) {
+    s = S.precision == MPLAK_PREC_F64 ? pair_finish_row<double>(S, i, r) : pair_finish_row<float>(S, i, r);
+  } else {
+    s = S.part[r * n + i];
+    for (int64_t k = 1; k < S.slabs; ++k) s += S.part[(k * R + r) * n + i];
+  }
   if (S.nugget != 0.0) s = fma(S.nugget, S.V[r * n + i], s);
   S.Y[r * n + i] = s;
 }
@@ -367,6 +621,39 @@ void launch_product(int rt, int dim, const mplak_ragged_system* sys, const mplak
   }
 }
 
+template <typename T, int DIM>
+void launch_pairs_dim(int rt, const mplak_ragged_system* sys, const mplak_ragged_item* items, int64_t count,
+                      hipStream_t st) {
+  const dim3 g((unsigned)count), b(kBlock);
+  switch (rt) {
+    case 1: hipLaunchKernelGGL((ragged_pair_kernel<T, 1, DIM>), g, b, 0, st, sys, items); break;
+    case 2: hipLaunchKernelGGL((ragged_pair_kernel<T, 2, DIM>), g, b, 0, st, sys, items); break;
+    case 4: hipLaunchKernelGGL((ragged_pair_kernel<T, 4, DIM>), g, b, 0, st, sys, items); break;
+    default: hipLaunchKernelGGL((ragged_pair_kernel<T, 8, DIM>), g, b, 0, st, sys, items); break;
+  }
+}
+
+template <typename T>
+void launch_pairs(int rt, int dim, const mplak_ragged_system* sys, const mplak_ragged_item* items, int64_t count,
+                  hipStream_t st) {
+  switch (dim) {
+    case 1: launch_pairs_dim<T, 1>(rt, sys, items, count, st); break;
+    case 2: launch_pairs_dim<T, 2>(rt, sys, items, count, st); break;
+    case 3: launch_pairs_dim<T, 3>(rt, sys, items, count, st); break;
+    default: launch_pairs_dim<T, 4>(rt, sys, items, count, st); break;
+  }
+}
+
+// PAIRS for n >= kRagPairMinN unless MPLAK_RAGGED_PAIRS says otherwise
+// (0: never, 1: every n); the pair index must fit an item's 32-bit field.
+bool use_pairs(int64_t n) {
+  const char* v = getenv("MPLAK_RAGGED_PAIRS");
+  bool on = n >= kRagPairMinN;
+  if (v != nullptr && *v != '\0') on = atoi(v) != 0;
+  const int64_t tiles = (n + kPairTile - 1) / kPairTile;
+  return on && pair_count(tiles) <= 0x7fffffff;
+}
+
 
 }  // namespace
 }  // namespace mplak
@@ -389,11 +676,31 @@ size_t mplak_ragged_plan(mplak_ragged_system* s) {
   s->slabs = (int32_t)((s->n + s->slab_cols - 1) / s->slab_cols);
   s->scale = coord_scale(s->kind, s->length_scale);
   s->drop_s = s->drop_log2 > 0.0 ? drop_distance2(s->kind, s->drop_log2) : 0.0;
+  s->reserved = 0;
+  if (use_pairs(s->n)) {
+    s->scheme = MPLAK_RAGGED_PAIRS;
+    s->tiles = (int32_t)((s->n + kPairTile - 1) / kPairTile);
+    s->slabs = 0;
+    s->slab_cols = 0;
+    const size_t es = s->precision == MPLAK_PREC_F64 ? sizeof(double) : sizeof(float);
+    const size_t pairs = (size_t)pair_count(s->tiles);
+    return align_up(2 * pairs * (size_t)s->nrhs * kPairTile * es, 256) + align_up(pairs * sizeof(int32_t), 256);
+  }
+  s->scheme = MPLAK_RAGGED_SLABS;
   return (size_t)s->slabs * (size_t)s->nrhs * (size_t)s->n * sizeof(double);
 }
 
 int64_t mplak_ragged_product_items(const mplak_ragged_system* s, mplak_ragged_item* out, int32_t sys_index) {
   if (s == nullptr || s->tiles <= 0) return 0;
+  if (s->scheme == MPLAK_RAGGED_PAIRS) {
+    const int64_t pairs = pair_count(s->tiles);
+    if (out) {
+      int64_t k = 0;
+      for (int32_t c = 0; c < s->ctiles; ++c)
+        for (int64_t q = 0; q < pairs; ++q) out[k++] = mplak_ragged_item{sys_index, (int32_t)q, 0, c};
+    }
+    return pairs * s->ctiles;
+  }
   const int64_t count = (int64_t)s->tiles * s->slabs * s->ctiles;
   if (out) {
     int64_t k = 0;
@@ -413,14 +720,22 @@ int64_t mplak_ragged_finish_items(const mplak_ragged_system* s, mplak_ragged_ite
 }
 
 int mplak_ragged_product(const mplak_ragged_system* systems, const mplak_ragged_item* items, int64_t count, int rt,
-                         int dim, int precision, mplak_stream_t s) {
+                         int dim, int precision, int scheme, mplak_stream_t s) {
   if (count <= 0) return MPLAK_OK;
   if (!systems || !items || count > 0x7fffffff || dim < 0 || dim > kMaxRaggedDim) return MPLAK_ERR_ARG;
+  if (scheme != MPLAK_RAGGED_SLABS && scheme != MPLAK_RAGGED_PAIRS) return MPLAK_ERR_ARG;
+  if (scheme == MPLAK_RAGGED_PAIRS && dim == 0) return MPLAK_ERR_ARG;
   hipStream_t hs = as_stream(s);
-  if (precision == MPLAK_PREC_F64)
+  if (scheme == MPLAK_RAGGED_PAIRS) {
+    if (precision == MPLAK_PREC_F64)
+      launch_pairs<double>(rt, dim, systems, items, count, hs);
+    else
+      launch_pairs<float>(rt, dim, systems, items, count, hs);
+  } else if (precision == MPLAK_PREC_F64) {
     launch_product<double>(rt, dim, systems, items, count, hs);
-  else
+  } else {
     launch_product<float>(rt, dim, systems, items, count, hs);
+  }
   return launch_status();
 }
 

@@ -131,6 +131,35 @@ def test_struct_layout_matches_headers(tmp_path):
             assert int(got[f"{sname}.{fname}"]) == getattr(cls, fname).offset, (sname, fname)
 
 
+MPLAK_STRUCTS = ["mplak_kernel_desc", "mplak_cg_state", "mplak_bcg_group", "mplak_bicgstab_state",
+                 "mplak_ragged_system", "mplak_ragged_item"]
+
+
+def test_mplak_struct_layout_matches_header(tmp_path):
+    """The kernel library's records (ctypes in mpla_amd._native) against gcc's
+    sizeof/offsetof of include/mplak.h."""
+    import subprocess
+    from mpla_amd import _native
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "mplak.h"', "int main(void) {"]
+    for sname in MPLAK_STRUCTS:
+        cls = getattr(_native, sname)
+        lines.append(f'printf("{sname} size %zu\\n", sizeof({sname}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'printf("{sname}.{fname} %zu\\n", offsetof({sname}, {fname}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout_k.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout_k"
+    subprocess.run(["gcc", "-std=c11", "-I", INCLUDE, str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = dict(line.rsplit(" ", 1) for line in out if line)
+    for sname in MPLAK_STRUCTS:
+        cls = getattr(_native, sname)
+        assert int(got[f"{sname} size"]) == C.sizeof(cls), sname
+        for fname, _ in cls._fields_:
+            assert int(got[f"{sname}.{fname}"]) == getattr(cls, fname).offset, (sname, fname)
+
+
 def test_block_distribution_host_only():
     """The product's distribution arithmetic (no GPU needed) against hand cases
     of src/mpla.cpp:157-209."""
@@ -148,13 +177,17 @@ def test_block_distribution_host_only():
 
 def test_server_request_bytes_host_only():
     """Admission arithmetic is host-only: device block = points + 6 vectors +
-    slab partials + CG partials, rounded to 1 MiB."""
+    product partials + CG partials, rounded to 1 MiB.  Below 8192 points the
+    partials are FP64 slab sums; from 8192 on, FP64 tile-pair sums (two
+    512-row blocks per pair of 512-point tiles and right-hand side)."""
     from mpla_amd import mpla
     lib = mpla.lib()
     small = lib.mpla_server_request_bytes(1024, 2, 1)
     big = lib.mpla_server_request_bytes(65536, 3, 8)
     assert small % (1 << 20) == 0 and small >= (1 << 20)
-    assert big >= 8 * (3 * 65536 + 6 * 65536 * 8)
-    assert big < 64 * (1 << 20)  # ~25 MB of vectors plus partials
+    assert small < 4 * (1 << 20)
+    pairs = 128 * 129 // 2
+    assert big >= 8 * (3 * 65536 + 6 * 65536 * 8) + 2 * pairs * 8 * 512 * 8
+    assert big < 8 * (3 * 65536 + 6 * 65536 * 8) + 2 * pairs * 8 * 512 * 8 + (8 << 20)
     assert lib.mpla_server_request_bytes(65536, 3, 4) < big
     assert lib.mpla_server_request_bytes(0, 3, 1) == 0

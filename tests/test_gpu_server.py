@@ -43,10 +43,15 @@ def test_server_single_request_vs_oracle(oracle):
     assert np.linalg.norm(sols[0][:, 0] - xo) <= 1e-6 * np.linalg.norm(xo)
 
 
+@pytest.mark.parametrize("pairs", [None, "1"])
 @pytest.mark.parametrize("mode", [mpla.MPLA_SERVE_MIXED_BATCHED, mpla.MPLA_SERVE_FP64_SERIAL])
-def test_server_mix_equals_alone(oracle, mode):
-    """Requests that overlap on the GPU give the same bits as each request alone."""
+def test_server_mix_equals_alone(oracle, monkeypatch, mode, pairs):
+    """Requests that overlap on the GPU give the same bits as each request
+    alone, with the products split by size (slabs at these n) and with every
+    product on the tile-pair split (MPLAK_RAGGED_PAIRS=1)."""
     require_gpu()
+    if pairs is not None:
+        monkeypatch.setenv("MPLAK_RAGGED_PAIRS", pairs)
     reqs = small_stream(10, seed=11, rate=2000.0)
     with serving.Server(mode=mode) as srv:
         res_mix, sol_mix = srv.run_trace(reqs)
