@@ -21,6 +21,8 @@
 #include <math.h>
 #include <stdlib.h>
 
+#include <type_traits>
+
 #include "common.hpp"
 #include "entry.hpp"
 
@@ -305,60 +307,43 @@ __global__ __launch_bounds__(kBlock) void ragged_pair_kernel(const mplak_ragged_
 
   T* part_i = pair_rows<T>(S);
   T* part_j = part_i + pair_count(S.tiles) * R * kPairTile;
-  const int kind = S.kind;
-  T acc[2][RT];
+  // the compute loop specialised per kernel function (the kind is uniform
+  // over the workgroup: no per-entry branch)
+  auto run = [&](auto kind_c) {
+    constexpr int KIND = decltype(kind_c)::value;
+    T acc[2][RT];
 #pragma unroll
-  for (int h = 0; h < 2; ++h)
+    for (int h = 0; h < 2; ++h)
 #pragma unroll
-    for (int q = 0; q < RT; ++q) acc[h][q] = T(0);
-  auto entries = [&](const Q& q, T (&e)[2]) {
+      for (int q = 0; q < RT; ++q) acc[h][q] = T(0);
+    auto entries = [&](const Q& q, T (&e)[2]) {
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const T dx = pi[h][0] - q.x;
-      T s = dx * dx;
-      if constexpr (DIM > 1) {
-        const T dy = pi[h][1] - q.y;
-        s = fma(dy, dy, s);
-      }
-      if constexpr (DIM > 2) {
-        const T dz = pi[h][2] - q.z;
-        s = fma(dz, dz, s);
-      }
-      if constexpr (DIM > 3) {
-        const T dw = pi[h][3] - q.w;
-        s = fma(dw, dw, s);
-      }
-      e[h] = kernel_entry_rt<T>(kind, s, exp_tab);
-    }
-  };
-
-  if (I == J) {
-#pragma unroll 1
-    for (int g = 0; g < kPairGroups; ++g) {
-      if (((keep >> g) & 1u) == 0u) continue;
-#pragma unroll 2
-      for (int s = 0; s < kWave; ++s) {
-        const int jj = g * kWave + s;
-        T e[2];
-        entries(src[jj], e);
-#pragma unroll
-        for (int q = 0; q < RT; ++q) {
-          const T v = sval[jj][q];
-          acc[0][q] = fma(e[0], v, acc[0][q]);
-          acc[1][q] = fma(e[1], v, acc[1][q]);
+      for (int h = 0; h < 2; ++h) {
+        const T dx = pi[h][0] - q.x;
+        T s = dx * dx;
+        if constexpr (DIM > 1) {
+          const T dy = pi[h][1] - q.y;
+          s = fma(dy, dy, s);
         }
+        if constexpr (DIM > 2) {
+          const T dz = pi[h][2] - q.z;
+          s = fma(dz, dz, s);
+        }
+        if constexpr (DIM > 3) {
+          const T dw = pi[h][3] - q.w;
+          s = fma(dw, dw, s);
+        }
+        e[h] = kernel_entry<T, KIND>(s, exp_tab);
       }
-    }
-  } else {
+    };
+
+    if (I == J) {
 #pragma unroll 1
-    for (int g = 0; g < kPairGroups; ++g) {
-      T t[RT];
-#pragma unroll
-      for (int q = 0; q < RT; ++q) t[q] = T(0);
-      if ((keep >> g) & 1u) {
+      for (int g = 0; g < kPairGroups; ++g) {
+        if (((keep >> g) & 1u) == 0u) continue;
 #pragma unroll 2
         for (int s = 0; s < kWave; ++s) {
-          const int jj = g * kWave + ((lane + s) & (kWave - 1));
+          const int jj = g * kWave + s;
           T e[2];
           entries(src[jj], e);
 #pragma unroll
@@ -366,29 +351,54 @@ __global__ __launch_bounds__(kBlock) void ragged_pair_kernel(const mplak_ragged_
             const T v = sval[jj][q];
             acc[0][q] = fma(e[0], v, acc[0][q]);
             acc[1][q] = fma(e[1], v, acc[1][q]);
-            t[q] = fma(e[1], xi[1][q], fma(e[0], xi[0][q], t[q]));
-            t[q] = wave_rotate(t[q]);
           }
         }
       }
+    } else {
+#pragma unroll 1
+      for (int g = 0; g < kPairGroups; ++g) {
+        T t[RT];
 #pragma unroll
-      for (int q = 0; q < RT; ++q) red[w][q][lane] = t[q];
-      __syncthreads();
-      for (int v = tid; v < RT * kWave; v += kBlock) {
-        const int q = v / kWave, l = v % kWave;
-        double s = (double)red[0][q][l];
+        for (int q = 0; q < RT; ++q) t[q] = T(0);
+        if ((keep >> g) & 1u) {
+#pragma unroll 2
+          for (int s = 0; s < kWave; ++s) {
+            const int jj = g * kWave + ((lane + s) & (kWave - 1));
+            T e[2];
+            entries(src[jj], e);
 #pragma unroll
-        for (int ww = 1; ww < kBlock / kWave; ++ww) s += (double)red[ww][q][l];
-        if (r0 + q < R) part_j[(p * R + r0 + q) * kPairTile + g * kWave + l] = (T)s;
+            for (int q = 0; q < RT; ++q) {
+              const T v = sval[jj][q];
+              acc[0][q] = fma(e[0], v, acc[0][q]);
+              acc[1][q] = fma(e[1], v, acc[1][q]);
+              t[q] = fma(e[1], xi[1][q], fma(e[0], xi[0][q], t[q]));
+              t[q] = wave_rotate(t[q]);
+            }
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < RT; ++q) red[w][q][lane] = t[q];
+        __syncthreads();
+        for (int v = tid; v < RT * kWave; v += kBlock) {
+          const int q = v / kWave, l = v % kWave;
+          double s = (double)red[0][q][l];
+#pragma unroll
+          for (int ww = 1; ww < kBlock / kWave; ++ww) s += (double)red[ww][q][l];
+          if (r0 + q < R) part_j[(p * R + r0 + q) * kPairTile + g * kWave + l] = (T)s;
+        }
+        __syncthreads();
       }
-      __syncthreads();
     }
-  }
 #pragma unroll
-  for (int h = 0; h < 2; ++h)
+    for (int h = 0; h < 2; ++h)
 #pragma unroll
-    for (int q = 0; q < RT; ++q)
-      if (r0 + q < R) part_i[(p * R + r0 + q) * kPairTile + w * 2 * kWave + h * kWave + lane] = acc[h][q];
+      for (int q = 0; q < RT; ++q)
+        if (r0 + q < R) part_i[(p * R + r0 + q) * kPairTile + w * 2 * kWave + h * kWave + lane] = acc[h][q];
+  };
+  if (S.kind == MPLAK_KERNEL_GAUSSIAN)
+    run(std::integral_constant<int, MPLAK_KERNEL_GAUSSIAN>{});
+  else
+    run(std::integral_constant<int, MPLAK_KERNEL_MATERN32>{});
 }
 
 template <typename T>
