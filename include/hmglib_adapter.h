@@ -99,6 +99,8 @@ struct mpla_hmatrix_info
 	int32_t max_rank, k;
 	double mean_rank;
 	int64_t device_bytes;         /* device memory the operator holds */
+	int64_t near_pairs;           /* > 0: symmetric near field, this many chunk pairs each
+	                                 evaluated once for both sides (0: per row) */
 };
 
 void mpla_hmatrix_get_info(struct mpla_generic_matrix* A, struct mpla_hmatrix_info* info);

@@ -280,6 +280,13 @@ typedef struct mplak_hm_item {
   int64_t j0, count; /* its local columns [j0, j0 + count) */
 } mplak_hm_item;
 
+/* A pair of near chunks A <= B (sorted rows [a0, a0 + am) and [b0, b0 + bm),
+ * at most 512 each) of a symmetric operator's inadmissible blocks. */
+typedef struct mplak_hm_pair {
+  int64_t a0, am;
+  int64_t b0, bm;
+} mplak_hm_pair;
+
 /* Closed-form entries: k(|p_i - p_j|) of MPLAK_KERNEL_* with the points'
  * coordinates already multiplied by the kernel's coordinate scale
  * (mplak_hm_gather_scaled).  The nugget is not part of these entries. */
@@ -321,6 +328,27 @@ int mplak_hm_apply(const mplak_hm_points* pts, const mplak_hm_chunk* chunks, int
                    const int64_t* near_cols, const int64_t* far_ptr, const int32_t* far_blk,
                    const mplak_hm_block* blocks, const int32_t* rank, int32_t kmax, const double* U, const double* t,
                    const double* xs, double* ys, mplak_stream_t stream);
+
+/* Symmetric near field (rows and columns the same sorted point set, so
+ * pts->rows and pts->cols hold the same coordinates): for pair k,
+ *   part[(2 k) 512 + i]     = sum_{j < bm} K(a0 + i, b0 + j) xs[b0 + j]  (i < am),
+ *   part[(2 k + 1) 512 + j] = sum_{i < am} K(a0 + i, b0 + j) xs[a0 + i]  (j < bm, a0 != b0),
+ * one workgroup per pair; each entry of an off-diagonal pair is evaluated
+ * once for both.  No reference counterpart (hmglib evaluates every
+ * inadmissible block per row). */
+int mplak_hm_near_pairs(const mplak_hm_points* pts, const mplak_hm_pair* pairs, int64_t npairs, const double* xs,
+                        double* part, mplak_stream_t stream);
+
+/* The apply of mplak_hm_apply with the near field taken from those
+ * partials: chunk c lies in near chunk chunk_near[c], whose first row is
+ * near_start[chunk_near[c]]; its near terms are the 512-row partial blocks
+ * ent[k] (2 pair + side), k in [ent_ptr[nc], ent_ptr[nc + 1]), added in a
+ * fixed order, then the far field as in mplak_hm_apply. */
+int mplak_hm_apply_pairs(const mplak_hm_chunk* chunks, int64_t nchunks, const int64_t* chunk_near,
+                         const int64_t* near_start, const int64_t* ent_ptr, const int64_t* ent, const double* part,
+                         const int64_t* far_ptr, const int32_t* far_blk, const mplak_hm_block* blocks,
+                         const int32_t* rank, int32_t kmax, const double* U, const double* t, double* ys,
+                         mplak_stream_t stream);
 
 /* The apply of mplak_hm_apply with the inadmissible blocks' entries read
  * from `stored` instead of evaluated: near range k (CSR order) of leaf l

@@ -177,6 +177,9 @@ MPLAK_SIGNATURES = {
     "mplak_hm_far_partials": (C.c_int, [_vp, _vp, _i64, _vp, _i32, _vp, _vp, _vp, _vp]),
     "mplak_hm_far_sum": (C.c_int, [_vp, _i64, _vp, _i32, _vp, _vp, _vp]),
     "mplak_hm_apply": (C.c_int, [_vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp]),
+    "mplak_hm_near_pairs": (C.c_int, [_vp, _vp, _i64, _vp, _vp, _vp]),
+    "mplak_hm_apply_pairs": (C.c_int, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp, _vp,
+                                       _vp]),
     "mplak_hm_apply_stored": (C.c_int, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp, _vp,
                                         _vp, _vp]),
     "mplak_hm_scatter_nugget": (C.c_int, [_vp, _vp, _i64, _dbl, _vp, _i64, _i64, _vp, _vp]),

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 
@@ -144,6 +145,7 @@ using namespace mpla_hm;
 constexpr int kMaxDim = 4;
 constexpr int64_t kItemColumns = 4096;  // columns per far-field coefficient item
 constexpr int64_t kChunkRows = 128;     // rows per apply item (two per lane of a 64-lane wave)
+constexpr int64_t kNearChunk = 512;     // rows per chunk of the symmetric near field (mplak_hm_pair)
 
 struct HMatrix {
   int dim = 0;
@@ -173,6 +175,13 @@ struct HMatrix {
   int32_t* far_blk = nullptr;
   int64_t *full_ptr = nullptr, *full_cols = nullptr, *nofar_ptr = nullptr;
   double *xs = nullptr, *ys = nullptr;
+  // symmetric near field (closed form, the same points on both sides):
+  // pairs of near chunks, their partial rows, and per near chunk the list
+  // of partial blocks (2 pair + side) that make up its rows
+  mplak_hm_pair* pairs = nullptr;
+  int64_t npairs = 0;
+  double* pair_part = nullptr;
+  int64_t *chunk_near = nullptr, *near_start = nullptr, *ent_ptr = nullptr, *ent = nullptr;
   // stored near field (virtual entries): entries evaluated once at init
   double* near_st = nullptr;
   int64_t *near_off = nullptr, *leaf_range = nullptr;
@@ -225,6 +234,73 @@ std::vector<double> host_coords(double* const* dev_coords, int dim, int64_t n, h
              "H-matrix coordinates");
   hip_ck(hipStreamSynchronize(s), "H-matrix coordinates");
   return c;
+}
+
+// Symmetric near field: when rows and columns are the same points (the two
+// trees, and so the leaves, coincide) and the near set is symmetric, the
+// inadmissible blocks become pairs of near chunks (512-row pieces of the
+// leaves) A <= B, each evaluated once for the rows of both
+// (mplak_hm_near_pairs).  MPLA_HM_SYMMETRIC=0 keeps the per-row form.
+void build_pairs(HMatrix* h, const std::vector<int32_t>& leaf_of, const std::vector<mplak_hm_chunk>& chunks,
+                 hipStream_t s) {
+  const char* v = getenv("MPLA_HM_SYMMETRIC");
+  if (v != nullptr && *v != '\0' && atoi(v) == 0) return;
+  const Tree& rt = h->rt;
+  const Tree& ct = h->ct;
+  if (rt.leaves.size() != ct.leaves.size()) return;
+  const int64_t nleaf = (int64_t)rt.leaves.size();
+  for (int64_t l = 0; l < nleaf; ++l)
+    if (rt.nodes[rt.leaves[l]].lo != ct.nodes[ct.leaves[l]].lo || rt.nodes[rt.leaves[l]].hi != ct.nodes[ct.leaves[l]].hi)
+      return;
+  std::vector<int32_t> cleaf_of(ct.nodes.size(), -1);
+  for (size_t i = 0; i < ct.leaves.size(); ++i) cleaf_of[ct.leaves[i]] = (int32_t)i;
+  std::vector<std::pair<int32_t, int32_t>> near;
+  for (const auto& nb : h->part.near) near.push_back({leaf_of[nb.first], cleaf_of[nb.second]});
+  std::sort(near.begin(), near.end());
+  for (const auto& ab : near)
+    if (!std::binary_search(near.begin(), near.end(), std::make_pair(ab.second, ab.first))) return;
+  std::vector<int64_t> first_chunk(nleaf + 1), near_start, near_len;
+  for (int64_t l = 0; l < nleaf; ++l) {
+    first_chunk[l] = (int64_t)near_start.size();
+    const Cluster& c = rt.nodes[rt.leaves[l]];
+    for (int64_t r0 = c.lo; r0 < c.hi; r0 += kNearChunk) {
+      near_start.push_back(r0);
+      near_len.push_back(std::min(kNearChunk, c.hi - r0));
+    }
+  }
+  first_chunk[nleaf] = (int64_t)near_start.size();
+  std::vector<mplak_hm_pair> pairs;
+  std::vector<std::vector<int64_t>> ents(near_start.size());
+  for (const auto& ab : near) {
+    if (ab.first > ab.second) continue;
+    for (int64_t ca = first_chunk[ab.first]; ca < first_chunk[ab.first + 1]; ++ca)
+      for (int64_t cb = first_chunk[ab.second]; cb < first_chunk[ab.second + 1]; ++cb) {
+        if (ab.first == ab.second && cb < ca) continue;
+        const int64_t k = (int64_t)pairs.size();
+        pairs.push_back(mplak_hm_pair{near_start[ca], near_len[ca], near_start[cb], near_len[cb]});
+        ents[ca].push_back(2 * k);
+        if (cb != ca) ents[cb].push_back(2 * k + 1);
+      }
+  }
+  if (pairs.empty() || pairs.size() > 0x7fffffffu) return;
+  std::vector<int64_t> ent_ptr{0}, ent;
+  for (const auto& e : ents) {
+    ent.insert(ent.end(), e.begin(), e.end());
+    ent_ptr.push_back((int64_t)ent.size());
+  }
+  std::vector<int64_t> chunk_near;
+  for (const auto& c : chunks) {
+    const int64_t lo = rt.nodes[rt.leaves[c.leaf]].lo;
+    chunk_near.push_back(first_chunk[c.leaf] + (c.r0 - lo) / kNearChunk);
+  }
+  h->pairs = upload(h, pairs, s);
+  h->npairs = (int64_t)pairs.size();
+  h->near_start = upload(h, near_start, s);
+  h->ent_ptr = upload(h, ent_ptr, s);
+  h->ent = upload(h, ent, s);
+  h->chunk_near = upload(h, chunk_near, s);
+  h->pair_part = dalloc<double>(h, 2 * pairs.size() * (size_t)kNearChunk);
+  h->info.near_pairs = h->npairs;
 }
 
 // Everything after the coordinates and ids are in h_matrix_data: trees,
@@ -351,6 +427,7 @@ void setup(mpla_generic_matrix* A, h_matrix_data* data, HMatrix* h, const std::v
   h->nblocks = (int64_t)blocks.size();
   h->nitems = (int64_t)items.size();
   h->nchunks = (int64_t)chunks.size();
+  if (h->closed_form && h->nr == h->nc && cr == cc) build_pairs(h, leaf_of, chunks, s);
   h->rank = dalloc<int32_t>(h, blocks.size());
   h->U = dalloc<double>(h, (size_t)uoff);
   h->V = dalloc<double>(h, (size_t)voff);
@@ -500,7 +577,13 @@ void apply(HMatrix* h, bool full, const double* x, double* y, mpla_instance* ins
   const int64_t* np = full ? h->full_ptr : h->near_ptr;
   const int64_t* nc = full ? h->full_cols : h->near_cols;
   const int64_t* fp = far ? h->far_ptr : h->nofar_ptr;
-  if (h->closed_form) {
+  if (h->closed_form && !full && h->npairs > 0) {
+    const mplak_hm_points p = points_of(h);
+    ck(mplak_hm_near_pairs(&p, h->pairs, h->npairs, h->xs, h->pair_part, s), what);
+    ck(mplak_hm_apply_pairs(h->chunks, h->nchunks, h->chunk_near, h->near_start, h->ent_ptr, h->ent, h->pair_part,
+                            fp, h->far_blk, h->blocks, h->rank, h->kmax, h->U, h->t, h->ys, s),
+       what);
+  } else if (h->closed_form) {
     const mplak_hm_points p = points_of(h);
     ck(mplak_hm_apply(&p, h->chunks, h->nchunks, np, nc, fp, h->far_blk, h->blocks, h->rank, h->kmax, h->U, h->t,
                       h->xs, h->ys, s),

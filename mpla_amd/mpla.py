@@ -265,6 +265,7 @@ class mpla_hmatrix_info(C.Structure):
         ("k", C.c_int32),
         ("mean_rank", C.c_double),
         ("device_bytes", C.c_int64),
+        ("near_pairs", C.c_int64),
     ]
 
 

@@ -375,5 +375,38 @@ def test_hmatrix_two_point_sets(s, oracle):
         info = mpla.mpla_hmatrix_info()
         s.lib.mpla_hmatrix_get_info(C.byref(A), C.byref(info))
         assert info.near_entries + info.far_entries == m * n and info.far_blocks > 0
+        assert info.near_pairs == 0  # two point sets: the near field stays per row
     finally:
         s.lib.mpla_destroy_hmglib(C.byref(A), s.I)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dim,n,kind,ell,c_leaf", [(2, 4000, 0, 1.0 / np.sqrt(2.0), 256), (3, 5000, 1, 0.5, 200),
+                                                   (1, 6000, 0, 0.05, 1000)])
+def test_hmatrix_symmetric_near_field(s, oracle, monkeypatch, dim, n, kind, ell, c_leaf):
+    """One point set on both sides: the near field as pairs of near chunks,
+    each entry evaluated once for both of its rows (leaves of more than 512
+    points split into 512-row chunks), against the per-row near field
+    (MPLA_HM_SYMMETRIC=0) and the oracle's exact product.  The two forms add
+    the same entries in different orders: within 1e-13 of |K| |x| per row."""
+    pts = oracle.uniform(dim * n, 83, dim).reshape(dim, n)
+    xh = oracle.uniform(n, 83, 7, -1, 1)
+    k = oracle.Kernel(pts, kind, ell)
+    ref = k.matvec(xh)
+    scale = k.matvec(np.abs(xh))
+    out, pairs = {}, {}
+    for sym in ("1", "0"):
+        monkeypatch.setenv("MPLA_HM_SYMMETRIC", sym)
+        print(f"hmatrix symmetric={sym}: dim {dim} n {n} kind {kind} c_leaf {c_leaf}", flush=True)
+        A, data, P = _init_kernel(s, pts, kind, ell, 0.0, 1.5, c_leaf, 20, 0.0)
+        try:
+            out[sym] = _apply(s, A, "mpla_dgemv_core_hmglib_h_matrix", xh)
+            info = mpla.mpla_hmatrix_info()
+            s.lib.mpla_hmatrix_get_info(C.byref(A), C.byref(info))
+            pairs[sym] = info.near_pairs
+        finally:
+            s.lib.mpla_destroy_hmglib(C.byref(A), s.I)
+    assert pairs["1"] > 0 and pairs["0"] == 0
+    assert np.all(np.abs(out["1"] - out["0"]) <= 1e-13 * scale)
+    bar = 1e-5 if dim == 3 else 1e-8
+    assert np.linalg.norm(out["1"] - ref) <= bar * np.linalg.norm(ref)
