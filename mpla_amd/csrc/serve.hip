@@ -407,12 +407,25 @@ __device__ __forceinline__ double pair_finish_row(const mplak_ragged_system& S, 
   const T* part_i = pair_rows<T>(S);
   const T* part_j = part_i + pair_count(S.tiles) * R * kPairTile;
   const int32_t* flags = pair_flags<T>(S);
-  double s = 0.0;
-  for (int64_t P = 0; P < S.tiles; ++P) {
+  // the partial of pair (X, P), or 0 for a pair whose flag says it wrote
+  // nothing (its block is then never used, whatever the memory holds)
+  auto term = [&](int64_t P) -> double {
     const int64_t p = X <= P ? sym_pair_index(X, P) : sym_pair_index(P, X);
-    if (flags[p] == 0) continue;
-    s += (double)(X <= P ? part_i[(p * R + r) * kPairTile + t] : part_j[(p * R + r) * kPairTile + t]);
+    const T v = (X <= P ? part_i : part_j)[(p * R + r) * kPairTile + t];
+    return flags[p] != 0 ? (double)v : 0.0;
+  };
+  // loads batched ahead of the additions, which stay in tile order
+  constexpr int kBatch = 8;
+  double s = 0.0;
+  int64_t P = 0;
+  for (; P + kBatch <= S.tiles; P += kBatch) {
+    double v[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) v[u] = term(P + u);
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) s += v[u];
   }
+  for (; P < S.tiles; ++P) s += term(P);
   return s;
 }
 
