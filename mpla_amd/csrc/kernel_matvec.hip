@@ -1104,6 +1104,8 @@ __global__ __launch_bounds__(kBlock) void kmv_sym_finish_kernel(
   for (int64_t row0 = (int64_t)blockIdx.x * kFinishRows; row0 < n; row0 += (int64_t)gridDim.x * kFinishRows) {
     const int64_t i = row0 + r;
     const int64_t X = row0 / kSymTile, t = i % kSymTile;  // rows past n read their tile's padding partials
+    // x of the row, loaded before the partials so that its latency overlaps theirs
+    const double xi = (k == 0 && i < n) ? x[i] : 0.0;
     double s = 0.0;
     if (LIST) {
       const uint32_t* lst = partners + (X * kFinishSlices + k) * pcap;
@@ -1132,7 +1134,6 @@ __global__ __launch_bounds__(kBlock) void kmv_sym_finish_kernel(
       double v = red[0][r];
 #pragma unroll
       for (int q = 1; q < kFinishSlices; ++q) v += red[q][r];
-      const double xi = x[i];
       if (nugget != 0.0) v = fma(nugget, xi, v);
       y[i] = v;
       dxx = fma(xi, xi, dxx);
