@@ -191,3 +191,42 @@ def test_server_request_bytes_host_only():
     assert big < 8 * (3 * 65536 + 6 * 65536 * 8) + 2 * pairs * 8 * 512 * 8 + (8 << 20)
     assert lib.mpla_server_request_bytes(65536, 3, 4) < big
     assert lib.mpla_server_request_bytes(0, 3, 1) == 0
+
+
+def test_ragged_plan_and_items_host_only(monkeypatch):
+    """The served products' work split is host arithmetic (no GPU): slabs
+    below 8192 points, tile pairs from 8192 on (or as MPLAK_RAGGED_PAIRS
+    says), with the partial-buffer size and the item list of each split."""
+    from mpla_amd import _native
+    lib = _native.mplak()
+
+    def plan(n, nrhs, prec):
+        s = _native.mplak_ragged_system()
+        s.n, s.dim, s.kind, s.nrhs, s.precision, s.length_scale = n, 3, _native.MPLAK_KERNEL_GAUSSIAN, nrhs, prec, 0.1
+        nbytes = lib.mplak_ragged_plan(C.byref(s))
+        cnt = lib.mplak_ragged_product_items(C.byref(s), None, 5)
+        items = (_native.mplak_ragged_item * cnt)()
+        assert lib.mplak_ragged_product_items(C.byref(s), items, 5) == cnt
+        return s, nbytes, list(items)
+
+    def align(v):
+        return -(-v // 256) * 256
+
+    monkeypatch.delenv("MPLAK_RAGGED_PAIRS", raising=False)
+    s, nbytes, items = plan(9000, 5, _native.MPLAK_PREC_F64)
+    tiles = -(-9000 // 512)
+    pairs = tiles * (tiles + 1) // 2
+    assert s.scheme == _native.MPLAK_RAGGED_PAIRS and s.tiles == tiles and s.rt == 8 and s.ctiles == 1
+    assert nbytes == align(2 * pairs * 5 * 512 * 8) + align(pairs * 4)
+    assert [(it.sys, it.tile, it.slab, it.ctile) for it in items] == [(5, p, 0, 0) for p in range(pairs)]
+    s32, nbytes32, _ = plan(9000, 5, _native.MPLAK_PREC_F32)
+    assert nbytes32 == align(2 * pairs * 5 * 512 * 4) + align(pairs * 4)
+    s, nbytes, items = plan(9000, 10, _native.MPLAK_PREC_F64)  # two column tiles
+    assert s.ctiles == 2 and len(items) == 2 * pairs and items[-1].ctile == 1
+    s, nbytes, items = plan(5000, 3, _native.MPLAK_PREC_F64)
+    assert s.scheme == _native.MPLAK_RAGGED_SLABS and nbytes == s.slabs * 3 * 5000 * 8
+    assert len(items) == s.tiles * s.slabs * s.ctiles
+    monkeypatch.setenv("MPLAK_RAGGED_PAIRS", "1")
+    assert plan(5000, 3, _native.MPLAK_PREC_F64)[0].scheme == _native.MPLAK_RAGGED_PAIRS
+    monkeypatch.setenv("MPLAK_RAGGED_PAIRS", "0")
+    assert plan(65536, 8, _native.MPLAK_PREC_F64)[0].scheme == _native.MPLAK_RAGGED_SLABS
