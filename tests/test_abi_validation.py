@@ -129,6 +129,28 @@ def test_hmatrix_kernels_validate_before_launch(lib):
     assert lib.mplak_hm_scatter_nugget(None, None, 0, 0.0, None, 0, 0, None, None) == OK
     assert lib.mplak_hm_scatter_nugget(None, None, 4, 1.0, None, 0, 4, None, None) == ARG
     assert lib.mplak_hm_gather_scaled(None, 3, 0, None, 10, 0, 0.5, None, 10, None) == ARG
+    # symmetric near field
+    assert lib.mplak_hm_near_pairs(None, None, 0, None, None, None) == ARG  # no point description
+    assert lib.mplak_hm_apply_pairs(None, 0, None, None, None, None, None, None, None, None, None, 20, None, None,
+                                    None, None) == OK
+    assert lib.mplak_hm_apply_pairs(None, 2, None, None, None, None, None, None, None, None, None, 20, None, None,
+                                    None, None) == ARG
+    assert lib.mplak_hm_apply_pairs(None, 2, None, None, None, None, None, None, None, None, None, 33, None, None,
+                                    None, None) == ARG
+
+
+def test_ragged_product_validates_before_launch(lib):
+    """The served products refuse an unknown work split, the tile-pair split
+    without a single dimension, and missing arrays; empty launches succeed."""
+    fake = C.c_void_p(0x1000)  # never dereferenced by the paths under test
+    pairs, slabs = _native.MPLAK_RAGGED_PAIRS, _native.MPLAK_RAGGED_SLABS
+    assert lib.mplak_ragged_product(None, None, 0, 8, 3, 0, pairs, None) == OK
+    assert lib.mplak_ragged_product(None, None, 1, 8, 3, 0, pairs, None) == ARG
+    assert lib.mplak_ragged_product(fake, fake, 1, 8, 0, 0, pairs, None) == ARG  # PAIRS needs one dimension
+    assert lib.mplak_ragged_product(fake, fake, 1, 8, 3, 0, 7, None) == ARG  # unknown split
+    assert lib.mplak_ragged_product(fake, fake, 1, 8, 5, 0, slabs, None) == ARG  # dimension above 4
+    assert lib.mplak_ragged_finish(None, None, 0, None) == OK
+    assert lib.mplak_ragged_finish(None, None, 3, None) == ARG
 
 
 def test_noalloc_refuses_malformed_descriptors(lib):
