@@ -74,15 +74,71 @@ __device__ __forceinline__ T block_sum(T v, T* scratch) {
   return s;
 }
 
+// This thread's share of a fixed-order sum of `count` partials: p[t],
+// p[t + NT], p[t + 2 NT], ... added in that order from +0.0.  The loads go
+// out kPartialBatch at a time (one round trip for up to kPartialBatch * NT
+// partials instead of one per partial); a batch past the end adds +0.0,
+// which leaves the sum as it is (a sum that starts at +0.0 is never -0.0 in
+// round-to-nearest), so the bits are those of the one-by-one loop.
+constexpr int kPartialBatch = 4;
+
+template <int NT>
+__device__ __forceinline__ double thread_partials(const double* __restrict__ p, int count) {
+  double s = 0.0;
+  for (int i0 = threadIdx.x; i0 < count; i0 += kPartialBatch * NT) {
+    double v[kPartialBatch];
+#pragma unroll
+    for (int u = 0; u < kPartialBatch; ++u) {
+      const int i = i0 + u * NT;
+      v[u] = i < count ? p[i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kPartialBatch; ++u) s += v[u];
+  }
+  return s;
+}
+
+// The same for two partial arrays of equal length, their loads in flight
+// together.
+template <int NT>
+__device__ __forceinline__ void thread_partials2(const double* __restrict__ p, const double* __restrict__ q,
+                                                 int count, double& sp, double& sq) {
+  sp = 0.0;
+  sq = 0.0;
+  for (int i0 = threadIdx.x; i0 < count; i0 += kPartialBatch * NT) {
+    double vp[kPartialBatch], vq[kPartialBatch];
+#pragma unroll
+    for (int u = 0; u < kPartialBatch; ++u) {
+      const int i = i0 + u * NT;
+      vp[u] = i < count ? p[i] : 0.0;
+      vq[u] = i < count ? q[i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kPartialBatch; ++u) {
+      sp += vp[u];
+      sq += vq[u];
+    }
+  }
+}
+
 // Sum of `count` partials laid out contiguously, evaluated by one block in a
 // fixed order (strided per-thread sums, then block_sum).  Every block that
 // calls this on the same data obtains the same bits.
 template <int NT>
 __device__ __forceinline__ double reduce_partials(const double* __restrict__ p, int count,
                                                   double* scratch) {
-  double s = 0.0;
-  for (int i = threadIdx.x; i < count; i += NT) s += p[i];
-  return block_sum<NT>(s, scratch);
+  return block_sum<NT>(thread_partials<NT>(p, count), scratch);
+}
+
+// Both sums of two partial arrays of equal length (the bits of two
+// reduce_partials calls).
+template <int NT>
+__device__ __forceinline__ void reduce_partials2(const double* __restrict__ p, const double* __restrict__ q,
+                                                 int count, double* scratch, double& sp, double& sq) {
+  double a, b;
+  thread_partials2<NT>(p, q, count, a, b);
+  sp = block_sum<NT>(a, scratch);
+  sq = block_sum<NT>(b, scratch);
 }
 
 // Symmetric products -------------------------------------------------------

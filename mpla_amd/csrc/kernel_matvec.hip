@@ -1104,25 +1104,35 @@ __global__ __launch_bounds__(kBlock) void kmv_sym_finish_kernel(
   for (int64_t row0 = (int64_t)blockIdx.x * kFinishRows; row0 < n; row0 += (int64_t)gridDim.x * kFinishRows) {
     const int64_t i = row0 + r;
     const int64_t X = row0 / kSymTile, t = i % kSymTile;  // rows past n read their tile's padding partials
-    // x of the row, loaded before the partials so that its latency overlaps theirs
+    // The slice's partner count and its first 64 partners (clamped index,
+    // no dependence on the count), then x of the row: all in flight
+    // together, and before the partials.
+    const uint32_t* lst = LIST ? partners + (X * kFinishSlices + k) * pcap : nullptr;
+    const int craw = LIST ? pcount[X * kFinishSlices + k] : 0;
+    const int head = LIST ? (int)lst[r < pcap ? r : pcap - 1] : 0;
     const double xi = (k == 0 && i < n) ? x[i] : 0.0;
     double s = 0.0;
     if (LIST) {
-      const uint32_t* lst = partners + (X * kFinishSlices + k) * pcap;
-      const int cnt = __builtin_amdgcn_readfirstlane(pcount[X * kFinishSlices + k]);
+      // keeps the list load here instead of after the count is known
+      asm volatile("" ::"v"(head));
+      // The partials go out kFinishBatch at a time, a batch past the count
+      // adding +0.0 (which leaves a sum that started at +0.0 unchanged), so
+      // the bits are those of adding one by one.
+      const int cnt = __builtin_amdgcn_readfirstlane(craw);
       for (int q0 = 0; q0 < cnt; q0 += kWave) {
         const int m = cnt - q0 < kWave ? cnt - q0 : kWave;
-        const int mine = r < m ? (int)lst[q0 + r] : 0;
-        int q = 0;
-        for (; q + kFinishBatch <= m; q += kFinishBatch) {
+        const int mine = q0 == 0 ? head : (int)lst[q0 + r < pcap ? q0 + r : pcap - 1];
+        for (int q = 0; q < m; q += kFinishBatch) {
           double v[kFinishBatch];
 #pragma unroll
-          for (int u = 0; u < kFinishBatch; ++u)
-            v[u] = sym_partial(part_i, part_j, X, (int64_t)__builtin_amdgcn_readlane(mine, q + u), t);
+          for (int u = 0; u < kFinishBatch; ++u) {
+            const int qu = q + u < m ? q + u : m - 1;
+            const double pv = sym_partial(part_i, part_j, X, (int64_t)__builtin_amdgcn_readlane(mine, qu), t);
+            v[u] = q + u < m ? pv : 0.0;
+          }
 #pragma unroll
           for (int u = 0; u < kFinishBatch; ++u) s += v[u];
         }
-        for (; q < m; ++q) s += sym_partial(part_i, part_j, X, (int64_t)__builtin_amdgcn_readlane(mine, q), t);
       }
     } else {
       for (int64_t P = k; P < ntiles; P += kFinishSlices) s += sym_partial(part_i, part_j, X, P, t);

@@ -66,8 +66,8 @@ __global__ __launch_bounds__(kBlock) void cg_start_final_kernel(const double* __
                                                                 double eps, int iter_max,
                                                                 double n_global) {
   __shared__ double scratch[kBlock / kWave];
-  const double rr = reduce_partials<kBlock>(part, g, scratch);
-  const double bb = reduce_partials<kBlock>(part + g, g, scratch);
+  double rr, bb;
+  reduce_partials2<kBlock>(part, part + g, g, scratch, rr, bb);
   if (threadIdx.x == 0) {
     mplak_cg_state s;
     s.rr = rr;
@@ -141,8 +141,8 @@ __global__ __launch_bounds__(kBlock) void cg_direct_kernel(int64_t n, const doub
   }
   __shared__ double scratch[kBlock / kWave];
   const int g = gridDim.x;
-  const double dz = reduce_partials<kBlock>(part, g, scratch);
-  const double rr_new = reduce_partials<kBlock>(part + g, g, scratch);
+  double dz, rr_new;
+  reduce_partials2<kBlock>(part, part + g, g, scratch, dz, rr_new);
   const double alpha = s_in.rr / dz;
   const double beta = rr_new / s_in.rr;
   const int k = s_in.k + 1;
@@ -208,8 +208,8 @@ __global__ __launch_bounds__(kBlock) void cg1_update_kernel(int64_t n, double* _
     sv[u] = in_range && !first ? s[i] : 0.0;
     xv[u] = in_range ? x[i] : 0.0;
   }
-  const double gam = reduce_partials<kBlock>(dots, nd, scratch);
-  const double del = reduce_partials<kBlock>(dots + nd, nd, scratch);
+  double gam, del;
+  reduce_partials2<kBlock>(dots, dots + nd, nd, scratch, gam, del);
   const double rel = stop_value(s_in.rule, gam, s_in.bnorm2, s_in.n_global);
   const bool conv = rel < s_in.eps;
   const double beta = first ? 0.0 : gam / s_in.rr;
@@ -318,8 +318,8 @@ __global__ __launch_bounds__(kBlock) void bicg_start_final_kernel(const double* 
                                                                   double eps, int iter_max,
                                                                   int true_residual) {
   __shared__ double scratch[kBlock / kWave];
-  const double rr = reduce_partials<kBlock>(part, g, scratch);
-  const double bb = reduce_partials<kBlock>(part + g, g, scratch);
+  double rr, bb;
+  reduce_partials2<kBlock>(part, part + g, g, scratch, rr, bb);
   if (threadIdx.x == 0) {
     mplak_bicgstab_state s;
     s.rho = rr;  // <r_hat, r> with r_hat = r
