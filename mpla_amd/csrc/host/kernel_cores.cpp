@@ -369,7 +369,7 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
   ChunkRunner runner(inst, true, chunk, iteration, normalize, ordered ? &rt->mixed_chunk : nullptr, key);
 
   bool stagnated = false;
-  std::vector<int> step_chunks;
+  std::vector<int> step_iters;
   for (;;) {
     // r = b - A_f64 x, relative to ||b||: the true FP64 residual
     if (x_zero) {
@@ -419,22 +419,31 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
     hip_ck(hipMemcpyAsync(st, rt->pinned, sizeof(inner), hipMemcpyHostToDevice, rt->stream), "hipMemcpyAsync");
     ck(mplak_fill_f64(e, n, 0.0, s), "mixed CG");
     mplak_cg_state ist = inner;
-    // the chunks this step took in the last solve of this size are enqueued
-    // before the state is first read back (one host round trip instead of one
-    // per chunk when the solves are alike); then one chunk per read
-    int hint = 1;
-    if (rt->mixed_hint_n == n && outer < (int)rt->mixed_hint_chunks.size())
-      hint = std::max(1, rt->mixed_hint_chunks[outer]);
-    int chunks_run = 0;
+    // the iterations this step took in the last solve of this size are
+    // enqueued before the state is first read back: whole chunks, then the
+    // rest one by one (not captured), so that a solve like the last one reads
+    // the state once and launches nothing past its convergence; then one
+    // chunk per read
+    int hint = 0;
+    if (rt->mixed_hint_n == n && outer < (int)rt->mixed_hint_iters.size())
+      hint = std::max(0, rt->mixed_hint_iters[outer]);
+    bool first_read = true;
     while (ist.done == MPLAK_RUNNING) {
-      const int batch = chunks_run == 0 ? hint : 1;
-      for (int c = 0; c < batch; ++c) runner.run();
-      chunks_run += batch;
+      if (first_read && hint > 0) {
+        for (int c = 0; c < hint / chunk; ++c) runner.run();
+        if (hint % chunk != 0) {
+          ChunkRunner rest(inst, false, hint % chunk, iteration, normalize);
+          rest.run();
+        }
+      } else {
+        runner.run();
+      }
+      first_read = false;
       ist = *static_cast<mplak_cg_state*>(read_back(inst, st, sizeof(mplak_cg_state)));
     }
-    // chunks this step needed: the iterations it ran (+1 for the converging
-    // product of the single-reduction form) over the chunk length
-    step_chunks.push_back((ist.k + 1 + chunk - 1) / chunk);
+    // iterations this step needed: its updates, +1 for the converging
+    // product of the single-reduction form
+    step_iters.push_back(ist.k + 1);
     total_inner += ist.k;
     // the single-reduction form applies the operator once more than it
     // updates when it converges (the product whose <r,r> meets the test)
@@ -456,7 +465,7 @@ void mpla_kernel_conjugate_gradient_mixed(struct mpla_vector* b, struct mpla_gen
   unpermute();
   if (ordered) {
     rt->mixed_hint_n = n;
-    rt->mixed_hint_chunks = step_chunks;
+    rt->mixed_hint_iters = step_iters;
   }
   int iterations = total_inner;
   double rel = cur.rel;

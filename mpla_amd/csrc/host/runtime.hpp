@@ -46,11 +46,11 @@ struct Runtime {
   size_t arena_bytes[kArenaCount] = {};
   void* pinned = nullptr;  // kPinnedBytes of page-locked host memory
   GraphSlot mixed_chunk;   // the mixed solver's inner-iteration chunk
-  // chunks each refinement step of the last mixed solve of this size took
-  // to converge: a later solve launches that many before its first look at
-  // the state (launches past convergence return at once)
+  // iterations each refinement step of the last mixed solve of this size
+  // took to converge: a later solve launches that many before its first look
+  // at the state (launches past convergence return at once)
   int64_t mixed_hint_n = -1;
-  std::vector<int> mixed_hint_chunks;
+  std::vector<int> mixed_hint_iters;
 };
 
 constexpr size_t kPinnedBytes = 65536;
