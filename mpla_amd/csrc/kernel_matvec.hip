@@ -1474,16 +1474,18 @@ __global__ __launch_bounds__(kBlock) void plan_tmask_kernel(const double* __rest
   const int w = (int)(id % kTmaskWords) / kSymGroups, g = (int)(id % kSymGroups);
   int64_t I, J;
   decode_pair(p, I, J);
+  // the thread's 8 target slabs and 4 source slabs, each loaded once
+  T tb[8][6], sb[4][6];
+#pragma unroll
+  for (int a = 0; a < 8; ++a) load_box(sboxes + ((size_t)I * kSlabs + 4 * a + w) * 6, tb[a]);
+#pragma unroll
+  for (int b = 0; b < 4; ++b) load_box(sboxes + ((size_t)J * kSlabs + 4 * g + b) * 6, sb[b]);
   uint32_t bits = 0;
-  for (int b = 0; b < 4; ++b) {
-    T sb[6];
-    load_box(sboxes + ((size_t)J * kSlabs + 4 * g + b) * 6, sb);
-    for (int a = 0; a < 8; ++a) {
-      T tb[6];
-      load_box(sboxes + ((size_t)I * kSlabs + 4 * a + w) * 6, tb);
-      if (box_gap2<T, DIM>(tb, sb) <= (T)drop_s) bits |= 1u << (8 * b + a);
-    }
-  }
+#pragma unroll
+  for (int b = 0; b < 4; ++b)
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+      if (box_gap2<T, DIM>(tb[a], sb[b]) <= (T)drop_s) bits |= 1u << (8 * b + a);
   tmask[id] = bits;
 }
 
@@ -1498,21 +1500,26 @@ __global__ __launch_bounds__(kBlock) void plan_pair_kernel(const double* __restr
   if (p >= np) return;
   int64_t I, J;
   decode_pair(p, I, J);
+  // the 8 group boxes of each tile, each loaded once
+  T ib[kSymGroups][6], jb[kSymGroups][6];
+#pragma unroll
+  for (int g = 0; g < kSymGroups; ++g) {
+    load_box(boxes + ((size_t)I * kSymGroups + g) * 6, ib[g]);
+    load_box(boxes + ((size_t)J * kSymGroups + g) * 6, jb[g]);
+  }
   int kept = 0;
   uint32_t bits = 0;
+#pragma unroll
   for (int w = 0; w < kBlock / kWave; ++w) {
-    T a[6], b[6], t[6];
-    load_box(boxes + ((size_t)I * kSymGroups + 2 * w) * 6, a);
-    load_box(boxes + ((size_t)I * kSymGroups + 2 * w + 1) * 6, b);
+    T t[6];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      t[d] = fmin(a[d], b[d]);
-      t[3 + d] = fmax(a[3 + d], b[3 + d]);
+      t[d] = fmin(ib[2 * w][d], ib[2 * w + 1][d]);
+      t[3 + d] = fmax(ib[2 * w][3 + d], ib[2 * w + 1][3 + d]);
     }
+#pragma unroll
     for (int g = 0; g < kSymGroups; ++g) {
-      T gb[6];
-      load_box(boxes + ((size_t)J * kSymGroups + g) * 6, gb);
-      if (box_gap2<T, DIM>(t, gb) <= (T)drop_s) {
+      if (box_gap2<T, DIM>(t, jb[g]) <= (T)drop_s) {
         kept += 1;
         bits |= 1u << (8 * w + g);
       }
@@ -1582,21 +1589,31 @@ __global__ __launch_bounds__(kBlock) void plan_count_kernel(const uint64_t* __re
 }
 
 // Partner lists of the finish: for tile X and slice k, the tiles P = k (mod 4)
-// in ascending order whose pair with X is listed (one thread per (tile,
+// in ascending order whose pair with X is listed (one wave per (tile,
 // slice); keys are the unsorted per-pair keys).
 __global__ __launch_bounds__(kBlock) void plan_partner_kernel(const uint64_t* __restrict__ keys, int64_t nt,
                                                               int64_t pcap, uint32_t* __restrict__ partners,
                                                               int32_t* __restrict__ pcount) {
-  const int64_t id = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  // one wave per list (tile X, slice k): lane l tests P = k + 4 (q0 + l), and
+  // the kept ones are written in P order (ballot, then each lane's rank)
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t id = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kWave;
   if (id >= nt * kFinishSlices) return;
   const int64_t X = id / kFinishSlices, k = id % kFinishSlices;
   uint32_t* lst = partners + id * pcap;
   int32_t c = 0;
-  for (int64_t P = k; P < nt; P += kFinishSlices) {
-    const int64_t p = P < X ? sym_pair_index(P, X) : sym_pair_index(X, P);
-    if ((keys[p] >> 32) != kPlanDropped) lst[c++] = (uint32_t)P;
+  for (int64_t P0 = k; P0 < nt; P0 += kFinishSlices * kWave) {
+    const int64_t P = P0 + kFinishSlices * lane;
+    bool kept = false;
+    if (P < nt) {
+      const int64_t p = P < X ? sym_pair_index(P, X) : sym_pair_index(X, P);
+      kept = (keys[p] >> 32) != kPlanDropped;
+    }
+    const uint64_t m = __ballot(kept);
+    if (kept) lst[c + __popcll(m & ((1ull << lane) - 1ull))] = (uint32_t)P;
+    c += __popcll(m);
   }
-  pcount[id] = c;
+  if (lane == 0) pcount[id] = c;
 }
 
 struct PlanPtrs {
@@ -1712,7 +1729,7 @@ int mplak_screen_plan_build(const mplak_kernel_desc* d, int precision, double dr
     launch_plan_geometry<float>(hs, d, scale, drop_s, q);
   st = launch_status();
   if (st != MPLAK_OK) return st;
-  hipLaunchKernelGGL(plan_partner_kernel, dim3((unsigned)((nt * kFinishSlices + kBlock - 1) / kBlock)), dim3(kBlock),
+  hipLaunchKernelGGL(plan_partner_kernel, dim3((unsigned)((nt * kFinishSlices * kWave + kBlock - 1) / kBlock)), dim3(kBlock),
                      0, hs, kin, nt, L.pcap, reinterpret_cast<uint32_t*>(base + L.partners),
                      reinterpret_cast<int32_t*>(base + L.pcount));
   st = launch_status();

@@ -270,6 +270,49 @@ def build_plan(lib, desc, prec, drop):
     return plan, wb
 
 
+def _plan_offsets(n):
+    """Byte offsets of the pair count, the pair list and the unsorted keys
+    in a screening plan of n points (the plan's fixed layout: 512-point
+    tiles, 256-byte aligned regions)."""
+    a = lambda v: -(-v // 256) * 256
+    nt = -(-n // 512)
+    np_ = nt * (nt + 1) // 2
+    tinfo = 256
+    boxes = tinfo + a(16 * nt)
+    sboxes = boxes + a(8 * 6 * 8 * nt)
+    lst = sboxes + a(8 * 6 * 32 * nt)
+    bmask = lst + a(4 * np_)
+    tmask = bmask + a(4 * np_)
+    keys_in = tmask + a(4 * 32 * np_)  # 32 tile-mask words per pair
+    return np_, 0, lst, keys_in
+
+
+@pytest.mark.parametrize("n,drop,prec", [(20000, 17.0, 1), (65536, 15.0, 1), (65536, 6.0, 1), (3000, 6.0, 1),
+                                         (511, 20.0, 1), (13000, 55.0, 0)])
+def test_plan_list_is_the_stable_class_order(lib, oracle, n, drop, prec):
+    """White-box check of a screening plan's pair list: every per-pair key
+    carries its pair index, and the list holds the pair indices in ascending
+    key class (bits 32-38: heaviest pairs first, pairs with nothing kept
+    last), equal classes in index order; the count is the number of pairs
+    before the dropped class."""
+    pts = ordered_points(lib, oracle, n, 3, 151)
+    P = dev(pts)
+    T = point_table(P)
+    d = kernel_desc(P, T, 0, 0.05, nugget=1.0)
+    np_, off_count, off_list, off_keys = _plan_offsets(n)
+    plan, wb = build_plan(lib, d, prec, drop)
+    raw = host(plan).view(np.uint8)
+    count = int(raw[off_count:off_count + 8].view(np.int64)[0])
+    lst = raw[off_list:off_list + 4 * np_].view(np.uint32)
+    keys = raw[off_keys:off_keys + 8 * np_].view(np.uint64)
+    assert np.array_equal(keys & np.uint64(0xffffffff), np.arange(np_, dtype=np.uint64))
+    cls = (keys >> np.uint64(32)).astype(np.int64)
+    assert np.all(cls <= 127)
+    assert np.array_equal(lst, np.argsort(cls, kind="stable").astype(np.uint32))
+    assert count == int((cls != 127).sum())
+    assert 0 < count <= np_
+
+
 def planned(lib, desc, prec, drop, plan, wb, X):
     y = empty(desc.row_count)
     ck(lib.mplak_kernel_matvec_planned(C.byref(desc), prec, drop, plan.data_ptr(), wb, X.data_ptr(), y.data_ptr(),
