@@ -414,18 +414,22 @@ __device__ __forceinline__ double pair_finish_row(const mplak_ragged_system& S, 
     const T v = (X <= P ? part_i : part_j)[(p * R + r) * kPairTile + t];
     return flags[p] != 0 ? (double)v : 0.0;
   };
-  // loads batched ahead of the additions, which stay in tile order
+  // loads batched ahead of the additions, which stay in tile order; the
+  // last batch adds +0.0 past the last tile, which leaves a sum that started
+  // at +0.0 unchanged, so the bits are those of adding tile by tile
   constexpr int kBatch = 8;
   double s = 0.0;
-  int64_t P = 0;
-  for (; P + kBatch <= S.tiles; P += kBatch) {
+  for (int64_t P = 0; P < S.tiles; P += kBatch) {
     double v[kBatch];
 #pragma unroll
-    for (int u = 0; u < kBatch; ++u) v[u] = term(P + u);
+    for (int u = 0; u < kBatch; ++u) {
+      const int64_t Q = P + u < S.tiles ? P + u : S.tiles - 1;
+      const double t = term(Q);
+      v[u] = P + u < S.tiles ? t : 0.0;
+    }
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) s += v[u];
   }
-  for (; P < S.tiles; ++P) s += term(P);
   return s;
 }
 
@@ -488,8 +492,8 @@ __global__ __launch_bounds__(kBlock) void ragged_start_final_kernel(const mplak_
   const mplak_ragged_item it = items[blockIdx.x];
   const mplak_ragged_cg S = systems[it.sys];
   const int G = S.G, c = it.slab;
-  const double rr = reduce_partials<kBlock>(S.part + (2 * c) * G, G, scratch);
-  const double bb = reduce_partials<kBlock>(S.part + (2 * c + 1) * G, G, scratch);
+  double rr, bb;
+  reduce_partials2<kBlock>(S.part + (2 * c) * G, S.part + (2 * c + 1) * G, G, scratch, rr, bb);
   if (threadIdx.x == 0) {
     mplak_cg_state s;
     s.rr = rr;
@@ -577,8 +581,8 @@ __global__ __launch_bounds__(kBlock) void ragged_direct_kernel(const mplak_ragge
   }
   __shared__ double scratch[kBlock / kWave];
   const int G = S.G;
-  const double dz = reduce_partials<kBlock>(S.part + c * G, G, scratch);
-  const double rr_new = reduce_partials<kBlock>(S.part + (S.nrhs + c) * G, G, scratch);
+  double dz, rr_new;
+  reduce_partials2<kBlock>(S.part + c * G, S.part + (S.nrhs + c) * G, G, scratch, dz, rr_new);
   const double beta = rr_new / s_in.rr;
   const int k = s_in.k + 1;
   const double rel = sqrt(rr_new) / sqrt(s_in.bnorm2);

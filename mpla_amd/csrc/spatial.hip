@@ -114,17 +114,37 @@ __global__ __launch_bounds__(kBlock) void hilbert_key_kernel(const double* const
                                                              int64_t offset, int64_t n,
                                                              const double* __restrict__ part, int nparts,
                                                              uint64_t* __restrict__ keys) {
+  // the bounding box from the per-block partial boxes, reduced by the whole
+  // block (minima and maxima are exact, so any order gives the same box)
   __shared__ double box[2 * kMaxDim];
-  if (threadIdx.x < dim) {
+  __shared__ double red[2][kBlock / kWave];
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  for (int d = 0; d < dim; ++d) {
     double lo = INFINITY, hi = -INFINITY;
-    for (int b = 0; b < nparts; ++b) {
-      lo = fmin(lo, part[(size_t)b * 2 * dim + threadIdx.x]);
-      hi = fmax(hi, part[(size_t)b * 2 * dim + dim + threadIdx.x]);
+    for (int b = threadIdx.x; b < nparts; b += kBlock) {
+      lo = fmin(lo, part[(size_t)b * 2 * dim + d]);
+      hi = fmax(hi, part[(size_t)b * 2 * dim + dim + d]);
     }
-    box[threadIdx.x] = lo;
-    box[kMaxDim + threadIdx.x] = hi;
+#pragma unroll
+    for (int m = 1; m < kWave; m <<= 1) {
+      lo = fmin(lo, __shfl_xor(lo, m));
+      hi = fmax(hi, __shfl_xor(hi, m));
+    }
+    if (lane == 0) {
+      red[0][w] = lo;
+      red[1][w] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int k = 1; k < kBlock / kWave; ++k) {
+        lo = fmin(lo, red[0][k]);
+        hi = fmax(hi, red[1][k]);
+      }
+      box[d] = lo;
+      box[kMaxDim + d] = hi;
+    }
+    __syncthreads();
   }
-  __syncthreads();
   const int bits = 30 / dim;
   const double cells = (double)(1u << bits);
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
