@@ -712,7 +712,11 @@ __device__ __forceinline__ void sym_mfma_pair(const double* const* __restrict__ 
       // bit b: tile (a, b) kept (bit 8 b + a of gm)
       const uint32_t cm = ((gm >> a) & 1u) | ((gm >> (7 + a)) & 2u) | ((gm >> (14 + a)) & 4u) | ((gm >> (21 + a)) & 8u);
       if (cm == 0) continue;
-      const f32x4 xm = DIAG ? f32x4{0.0f, 0.0f, 0.0f, 0.0f} : *reinterpret_cast<const f32x4*>(&xt[64 * a]);
+      const f32x4 xm = *reinterpret_cast<const f32x4*>(&xt[64 * a]);
+      // diagonal pair: the tile of target slab 4 a + w against source slab
+      // 4 g + b is on the diagonal for a = g, b = w (rows only); the plan
+      // keeps no tile below it
+      const int dtile = DIAG && a == g ? w : -1;
       f32x4 s[4];
 #pragma unroll
       for (int b = 0; b < 4; ++b)
@@ -726,11 +730,10 @@ __device__ __forceinline__ void sym_mfma_pair(const double* const* __restrict__ 
           if (KIND == MPLAK_KERNEL_MATERN32) sv = fmaxf(sv, 0.0f);  // sqrt of a rounding-negative s
           const float e = kernel_entry<float, KIND>(sv, nullptr);
           accR[a][v] = fmaf(e, xn[b], accR[a][v]);
-          if (!DIAG) accC[b] = fmaf(e, xm[v], accC[b]);
+          if (b != dtile) accC[b] = fmaf(e, xm[v], accC[b]);
         }
       }
     }
-    if (DIAG) return 0.0f;
     // lane 16 r + c keeps the sum of source 16 r + c of the group
     return rows4_reduce_scatter(accC, r4);
   };
@@ -749,7 +752,11 @@ __device__ __forceinline__ void sym_mfma_pair(const double* const* __restrict__ 
 #pragma unroll 1
     for (int g = 0; g < kSymGroups; ++g) {
       const uint32_t gm = group_mask(g);
-      if (gm != 0) mfma_group(g, gm, std::true_type{});
+      if (gm == 0) continue;
+      const float v = mfma_group(g, gm, std::true_type{});
+#pragma unroll
+      for (int q = 0; q < kSymGroups; ++q)
+        if (q == g) tg[q] = v;
     }
   } else {
 #pragma unroll 1
@@ -771,6 +778,30 @@ __device__ __forceinline__ void sym_mfma_pair(const double* const* __restrict__ 
     for (int v = 0; v < 4; ++v) rv[4 * a + v] = accR[a][v];
   }
   row16_reduce_scatter(rv, c16, out);
+  if (I == J) {
+    // the column sums of the upper tiles belong to the same rows: the four
+    // waves' sums in wave order (FP64), then added to each row's sum
+    __syncthreads();  // every wave is done reading the staged tiles
+#pragma unroll
+    for (int g = 0; g < kSymGroups; ++g) mf[w * kSymTile + g * kWave + lane] = tg[g];
+    __syncthreads();
+    static_assert((kBlock / kWave) * kSymTile + 2 * kSymTile <= kMfScale, "column sums clear of the row scales");
+    double* cs = reinterpret_cast<double*>(mf + (kBlock / kWave) * kSymTile);
+    for (int k = tid; k < kSymTile; k += kBlock) {
+      double sum = (double)mf[k];
+#pragma unroll
+      for (int q = 1; q < kBlock / kWave; ++q) sum += (double)mf[q * kSymTile + k];
+      cs[k] = sum;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int k = 256 * h + 64 * (c16 >> 2) + 16 * w + 4 * r4 + (c16 & 3);
+      const float v = FACTOR ? out[h] * mf[kMfScale + k] : out[h];
+      part_i[p * kSymTile + k] = (P)((double)v + cs[k]);
+    }
+    return;
+  }
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     const int k = 256 * h + 64 * (c16 >> 2) + 16 * w + 4 * r4 + (c16 & 3);
@@ -1480,12 +1511,17 @@ __global__ __launch_bounds__(kBlock) void plan_tmask_kernel(const double* __rest
   for (int a = 0; a < 8; ++a) load_box(sboxes + ((size_t)I * kSlabs + 4 * a + w) * 6, tb[a]);
 #pragma unroll
   for (int b = 0; b < 4; ++b) load_box(sboxes + ((size_t)J * kSlabs + 4 * g + b) * 6, sb[b]);
+  // FP32 plans, diagonal pairs: only tiles with target slab <= source slab;
+  // the matrix-core form adds each strictly upper tile to both its rows and
+  // its columns (the lower tile is its transpose)
+  const bool upper_only = sizeof(T) == 4 && I == J;
   uint32_t bits = 0;
 #pragma unroll
   for (int b = 0; b < 4; ++b)
 #pragma unroll
     for (int a = 0; a < 8; ++a)
-      if (box_gap2<T, DIM>(tb[a], sb[b]) <= (T)drop_s) bits |= 1u << (8 * b + a);
+      if (!(upper_only && 4 * a + w > 4 * g + b) && box_gap2<T, DIM>(tb[a], sb[b]) <= (T)drop_s)
+        bits |= 1u << (8 * b + a);
   tmask[id] = bits;
 }
 

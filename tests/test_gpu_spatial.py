@@ -395,8 +395,10 @@ def plan_counts_numpy(pts, l, drop):
     128-point wave target sets per 512-point tile (padding clamped to the last
     point), squared box gaps with FP32 rounding (fma emulated in FP64, exact
     for these products), compact tiles (half-diagonal^2 <= 49), and per tile
-    pair I <= J the kept 128 x 64 blocks and kept 16 x 16 tiles.  Returns the
-    five plan counts."""
+    pair I <= J the kept 128 x 64 blocks and kept 16 x 16 tiles (on a
+    diagonal pair only those with target slab <= source slab: the FP32
+    matrix-core form adds each strictly upper tile to its rows and its
+    columns).  Returns the five plan counts."""
     dim, n = pts.shape
     scale = math.sqrt(math.log2(math.e)) / (math.sqrt(2.0) * l)
     f = np.float32
@@ -429,7 +431,8 @@ def plan_counts_numpy(pts, l, drop):
     for J in range(nt):
         for I in range(J + 1):
             blocks = int(db[4 * I:4 * I + 4, 8 * J:8 * J + 8].sum())
-            tiles = int(dt[32 * I:32 * I + 32, 32 * J:32 * J + 32].sum())
+            block = dt[32 * I:32 * I + 32, 32 * J:32 * J + 32]
+            tiles = int((np.triu(block) if I == J else block).sum())
             if (tiles if compact[I] else blocks) == 0:
                 continue
             c[0] += 1
