@@ -258,6 +258,22 @@ void dispatch_dim(dim3 grid, hipStream_t st, bool coinc, const mplak_kernel_desc
 // ---------------------------------------------------------------------------
 
 constexpr int kSymTile = 2 * kBlock;  // 512 points per tile
+
+// The finish: 64 rows per workgroup, one partner slice per wave.
+constexpr int kFinishRows = kWave;
+constexpr int kFinishSlices = kBlock / kWave;
+constexpr int kFinishBatch = 8;  // independent partial loads in flight per wave
+
+// Partial rows of a plan's listed pairs ("slotted" layout): tile X's
+// partner lists in slices k = P mod kFinishSlices, pcap rows of 512 each,
+// hold the rows of X of its listed pairs (X, P) in list order, so the finish
+// reads a list's partials without the list.  slots[2 p] is the row of pair
+// p = (I, J) in I's list, slots[2 p + 1] its row in J's list (I < J).
+template <typename P>
+__device__ __forceinline__ P* slot_row(P* part, int64_t X, int64_t partner, int32_t slot, int64_t pcap) {
+  return part + ((X * kFinishSlices + (partner & (kFinishSlices - 1))) * pcap + slot) * kSymTile;
+}
+
 constexpr int kSymMinN = 12288;       // crossover measured on MI355X (plain kernel faster below)
 
 // MPLAK_KMV_SYMMETRIC=0 turns the symmetric product off, =1 uses it for every
@@ -305,8 +321,8 @@ using SymSrc = typename Quad<T>::type[(sizeof(T) == 4 ? 2 : 1) * kWave];
 template <typename T, int KIND, int DIM, bool COINC, bool SCREEN, bool MASKED, typename P>
 __device__ __forceinline__ void sym_valu_pair(const double* const* __restrict__ points, int64_t off, int64_t n,
                                               double scale, double coinc_thr, double coinc_shift, double drop_s,
-                                              const double* __restrict__ x, P* __restrict__ part_i,
-                                              P* __restrict__ part_j, int64_t p, uint32_t pmask,
+                                              const double* __restrict__ x, P* __restrict__ rows_i,
+                                              P* __restrict__ rows_j, int64_t p, uint32_t pmask,
                                               SymSrc<T>* src, T (*gbox)[6], T* own_red, const double* exp_tab) {
   typedef typename Quad<T>::type Q;
   constexpr int kDup = sizeof(T) == 4 ? 2 : 1;  // copies of each 64-source group
@@ -461,7 +477,7 @@ __device__ __forceinline__ void sym_valu_pair(const double* const* __restrict__ 
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     const int k = w * 2 * kWave + h * kWave + lane;
-    part_i[p * kSymTile + k] = (P)accI[h];
+    rows_i[k] = (P)accI[h];
   }
   if (I != J) {
     __syncthreads();
@@ -469,21 +485,22 @@ __device__ __forceinline__ void sym_valu_pair(const double* const* __restrict__ 
       double s = (double)red[k];
 #pragma unroll
       for (int q = 1; q < kBlock / kWave; ++q) s += (double)red[q * kSymTile + k];
-      part_j[p * kSymTile + k] = (P)s;
+      rows_j[k] = (P)s;
     }
   }
 }
 
 // Symmetric product kernel on the vector unit: kSymExact / kSymScreened over
-// all tile pairs (workgroup p = pair p), kSymPlanned over the tile pairs
-// list[0 .. *count) of a screening plan (pairs with a kept block; the
-// partials of the others were zeroed when the plan was built).
+// all tile pairs (workgroup p = pair p, partial rows by pair),
+// kSymPlanned over the tile pairs list[0 .. *count) of a screening plan
+// (pairs with a kept block; rows in the plan's slotted layout, which holds
+// no row for the others).
 template <typename T, int KIND, int DIM, bool COINC, int MODE>
 __global__ __launch_bounds__(kBlock) void kmv_sym_kernel(
     const double* const* __restrict__ points, int64_t off, int64_t n, double scale, double coinc_thr,
     double coinc_shift, double drop_s, const double* __restrict__ x, double* __restrict__ part_i,
     double* __restrict__ part_j, const uint32_t* __restrict__ list, const int64_t* __restrict__ count,
-    const int32_t* __restrict__ skip) {
+    const int32_t* __restrict__ slots, int64_t pcap, const int32_t* __restrict__ skip) {
   if (skip != nullptr && *skip != 0) return;
   if (MODE == kSymPlanned && (int64_t)blockIdx.x >= *count) return;
   typedef typename Quad<T>::type Q;
@@ -498,8 +515,18 @@ __global__ __launch_bounds__(kBlock) void kmv_sym_kernel(
   __shared__ double exp_tab[Exp2Lds<T>::size];
   if (sizeof(T) == 8) fill_exp2_table(exp_tab);  // first read after the staging barrier
   const int64_t p = MODE == kSymPlanned ? (int64_t)list[blockIdx.x] : (int64_t)blockIdx.x;
+  // partial rows: by pair (part_i for tile I, part_j for tile J), or with a
+  // plan in its slotted layout (part_i)
+  double* rows_i = part_i + p * kSymTile;
+  double* rows_j = part_j + p * kSymTile;
+  if (MODE == kSymPlanned) {
+    int64_t I, J;
+    decode_pair(p, I, J);
+    rows_i = slot_row(part_i, I, J, slots[2 * p], pcap);
+    rows_j = I != J ? slot_row(part_i, J, I, slots[2 * p + 1], pcap) : rows_i;
+  }
   sym_valu_pair<T, KIND, DIM, COINC, MODE != kSymExact, false, double>(
-      points, off, n, scale, coinc_thr, coinc_shift, drop_s, x, part_i, part_j, p, 0u,
+      points, off, n, scale, coinc_thr, coinc_shift, drop_s, x, rows_i, rows_j, p, 0u,
       reinterpret_cast<SymSrc<T>*>(&src[0][0]), gbox, own_red, exp_tab);
 }
 
@@ -616,8 +643,8 @@ constexpr int kMfFloats = kMfScale + kSymTile;
 // take s = t + n_p with C = n_q.
 template <int KIND, int DIM, typename P>
 __device__ __forceinline__ void sym_mfma_pair(const double* const* __restrict__ points, int64_t off, int64_t n,
-                                              double scale, const double* __restrict__ x, P* __restrict__ part_i,
-                                              P* __restrict__ part_j, int64_t p, int64_t I, int64_t J,
+                                              double scale, const double* __restrict__ x, P* __restrict__ rows_i,
+                                              P* __restrict__ rows_j, int64_t p, int64_t I, int64_t J,
                                               float4 centre, const uint32_t* __restrict__ tmask, float* mf) {
   constexpr bool FACTOR = KIND == MPLAK_KERNEL_GAUSSIAN;
   const int tid = threadIdx.x, lane = tid & (kWave - 1);
@@ -798,7 +825,7 @@ __device__ __forceinline__ void sym_mfma_pair(const double* const* __restrict__ 
     for (int h = 0; h < 2; ++h) {
       const int k = 256 * h + 64 * (c16 >> 2) + 16 * w + 4 * r4 + (c16 & 3);
       const float v = FACTOR ? out[h] * mf[kMfScale + k] : out[h];
-      part_i[p * kSymTile + k] = (P)((double)v + cs[k]);
+      rows_i[k] = (P)((double)v + cs[k]);
     }
     return;
   }
@@ -806,7 +833,7 @@ __device__ __forceinline__ void sym_mfma_pair(const double* const* __restrict__ 
   for (int h = 0; h < 2; ++h) {
     const int k = 256 * h + 64 * (c16 >> 2) + 16 * w + 4 * r4 + (c16 & 3);
     const float v = FACTOR ? out[h] * mf[kMfScale + k] : out[h];
-    part_i[p * kSymTile + k] = (P)v;
+    rows_i[k] = (P)v;
   }
   if (I != J) {
     __syncthreads();  // every wave is done reading the staged tiles
@@ -817,7 +844,7 @@ __device__ __forceinline__ void sym_mfma_pair(const double* const* __restrict__ 
       double s = (double)mf[k];
 #pragma unroll
       for (int q = 1; q < kBlock / kWave; ++q) s += (double)mf[q * kSymTile + k];
-      part_j[p * kSymTile + k] = (P)s;
+      rows_j[k] = (P)s;
     }
   }
 }
@@ -827,7 +854,7 @@ __device__ __forceinline__ void sym_mfma_pair(const double* const* __restrict__ 
 // used while the value at `sel` is <= below[l] (thresholds decreasing), level
 // 0 otherwise (also for a NaN).  The kernel and its finish read the same
 // value, so they use the same level; every level writes the partial sums of
-// level 0's plan, whose partner lists of the chosen level are all the finish
+// level 0's plan in its own slotted layout (below), which is all the finish
 // reads.  A single plan is the case nlev = 1.
 constexpr int kMaxLevels = 3;
 struct PlanLevels {
@@ -838,10 +865,13 @@ struct PlanLevels {
   const uint32_t* tmask[kMaxLevels];
   const uint32_t* partners[kMaxLevels];
   const int32_t* pcount[kMaxLevels];
+  const int32_t* slots[kMaxLevels];
   double below[kMaxLevels];
   const double* sel;
+  int64_t pcap;
   int nlev;
 };
+
 
 // Element l of a kernel-argument array by selects (no dynamic indexing, so
 // the array stays in the argument segment).
@@ -862,7 +892,7 @@ __device__ __forceinline__ int plan_level(const PlanLevels& lv) {
 template <int KIND, int DIM>
 __global__ __launch_bounds__(kBlock) void kmv_sym_planned_kernel(
     const double* const* __restrict__ points, int64_t off, int64_t n, double scale, const double* __restrict__ x,
-    float* __restrict__ part_i, float* __restrict__ part_j, const PlanLevels lv, const int32_t* __restrict__ skip) {
+    float* __restrict__ part, const PlanLevels lv, const int32_t* __restrict__ skip) {
   if (skip != nullptr && *skip != 0) return;
   const int lev = plan_level(lv);
   if ((int64_t)blockIdx.x >= *level_pick(lv.count, lev)) return;
@@ -873,14 +903,17 @@ __global__ __launch_bounds__(kBlock) void kmv_sym_planned_kernel(
   __shared__ __attribute__((aligned(16))) float mf[kMfFloats];
   static_assert((kBlock / kWave) * kSymTile <= kMfFloats, "column sums reuse the staging buffer");
   static_assert(sizeof(SymSrc<float>) * kSymGroups <= sizeof(mf), "vector-unit staging fits");
+  const int32_t* __restrict__ slots = level_pick(lv.slots, lev);
   const int64_t p = (int64_t)list[blockIdx.x];
   int64_t I, J;
   decode_pair(p, I, J);
+  float* rows_i = slot_row(part, I, J, slots[2 * p], lv.pcap);
+  float* rows_j = I != J ? slot_row(part, J, I, slots[2 * p + 1], lv.pcap) : rows_i;
   const float4 centre = tinfo[I];
   if (centre.w != 0.0f) {
-    sym_mfma_pair<KIND, DIM, float>(points, off, n, scale, x, part_i, part_j, p, I, J, centre, tmask, mf);
+    sym_mfma_pair<KIND, DIM, float>(points, off, n, scale, x, rows_i, rows_j, p, I, J, centre, tmask, mf);
   } else {
-    sym_valu_pair<float, KIND, DIM, false, true, true, float>(points, off, n, scale, 0.0, 0.0, 0.0, x, part_i, part_j, p,
+    sym_valu_pair<float, KIND, DIM, false, true, true, float>(points, off, n, scale, 0.0, 0.0, 0.0, x, rows_i, rows_j, p,
                                                        bmask[p], reinterpret_cast<SymSrc<float>*>(mf), nullptr,
                                                        nullptr, nullptr);
   }
@@ -940,7 +973,7 @@ __device__ __forceinline__ double rows4_reduce_scatter_f64(const double (&v)[4],
 template <int KIND, int DIM>
 __device__ __forceinline__ void sym_tile64_pair(const double* const* __restrict__ points, int64_t off, int64_t n,
                                                 double scale, const double* __restrict__ x,
-                                                double* __restrict__ part_i, double* __restrict__ part_j, int64_t p,
+                                                double* __restrict__ rows_i, double* __restrict__ rows_j, int64_t p,
                                                 int64_t I, int64_t J, const uint32_t* __restrict__ tmask,
                                                 double* lds) {
   const int tid = threadIdx.x, lane = tid & (kWave - 1);
@@ -1067,7 +1100,7 @@ __device__ __forceinline__ void sym_tile64_pair(const double* const* __restrict_
       for (int v = 0; v < 4; ++v) rv[4 * a + v] = accR[a][v];
     const double out = row16_reduce_scatter_f64(rv, c16);
     const int k = 16 * (4 * (4 * h + (c16 >> 2)) + w) + 4 * r4 + (c16 & 3);
-    part_i[p * kSymTile + k] = out;
+    rows_i[k] = out;
   }
   if (I != J) {
     __syncthreads();  // every wave is done reading the staged tiles
@@ -1078,7 +1111,7 @@ __device__ __forceinline__ void sym_tile64_pair(const double* const* __restrict_
       double sum = lds[k];
 #pragma unroll
       for (int q2 = 1; q2 < kBlock / kWave; ++q2) sum += lds[q2 * kSymTile + k];
-      part_j[p * kSymTile + k] = sum;
+      rows_j[k] = sum;
     }
   }
 }
@@ -1086,31 +1119,31 @@ __device__ __forceinline__ void sym_tile64_pair(const double* const* __restrict_
 template <int KIND, int DIM>
 __global__ __launch_bounds__(kBlock) void kmv_sym_planned64_kernel(
     const double* const* __restrict__ points, int64_t off, int64_t n, double scale, const double* __restrict__ x,
-    double* __restrict__ part_i, double* __restrict__ part_j, const uint32_t* __restrict__ list,
-    const int64_t* __restrict__ count, const uint32_t* __restrict__ tmask, const int32_t* __restrict__ skip) {
+    double* __restrict__ part, const uint32_t* __restrict__ list, const int64_t* __restrict__ count,
+    const uint32_t* __restrict__ tmask, const int32_t* __restrict__ slots, int64_t pcap,
+    const int32_t* __restrict__ skip) {
   if (skip != nullptr && *skip != 0) return;
   if ((int64_t)blockIdx.x >= *count) return;
   __shared__ __attribute__((aligned(16))) double buf[kT64Doubles];
   const int64_t p = (int64_t)list[blockIdx.x];
   int64_t I, J;
   decode_pair(p, I, J);
-  sym_tile64_pair<KIND, DIM>(points, off, n, scale, x, part_i, part_j, p, I, J, tmask, buf);
+  double* rows_i = slot_row(part, I, J, slots[2 * p], pcap);
+  double* rows_j = I != J ? slot_row(part, J, I, slots[2 * p + 1], pcap) : rows_i;
+  sym_tile64_pair<KIND, DIM>(points, off, n, scale, x, rows_i, rows_j, p, I, J, tmask, buf);
 }
 
 // y_i = sum over partner tiles P of row i's tile X of that pair's partial,
 // plus the nugget term.  A workgroup takes 64 consecutive rows (one tile) and
 // four partner slices: wave k sums the partners P = k (mod 4) in ascending
-// order, then the four slice sums are added in slice order.  With a partner
-// list (planned product) a wave visits only the listed partners of its slice,
-// in the same order; the partials it leaves out are zero, so the result is
-// the same.  The wave reads its list with one load per lane and takes the
-// partners from it with readlane, kFinishBatch independent partial loads at a
+// order, then the four slice sums are added in slice order.  With a plan
+// (planned product) a wave adds only the listed partners of its slice, in the
+// same order; the partials it leaves out are zero, so the result is the same.
+// Their rows are contiguous in the plan's slotted layout, so the wave needs
+// only the list's length; kFinishBatch independent partial loads go out at a
 // time.  With `dots` the finish also leaves, per workgroup, the partial sums
 // of <x,x> and <x,y> over its rows (wave butterfly, so a fixed order):
 // dots[b] and dots[gridDim.x + b].
-constexpr int kFinishRows = kWave;
-constexpr int kFinishSlices = kBlock / kWave;
-constexpr int kFinishBatch = 8;  // independent partial loads in flight per wave
 
 template <typename PT>
 __device__ __forceinline__ double sym_partial(const PT* __restrict__ part_i, const PT* __restrict__ part_j,
@@ -1127,7 +1160,6 @@ __global__ __launch_bounds__(kBlock) void kmv_sym_finish_kernel(
     const int32_t* __restrict__ skip, double* __restrict__ dots) {
   if (skip != nullptr && *skip != 0) return;
   const int lev = LIST ? plan_level(lv) : 0;
-  const uint32_t* __restrict__ partners = level_pick(lv.partners, lev);
   const int32_t* __restrict__ pcount = level_pick(lv.pcount, lev);
   __shared__ double red[kFinishSlices][kFinishRows];
   const int r = threadIdx.x % kFinishRows, k = threadIdx.x / kFinishRows;
@@ -1135,35 +1167,35 @@ __global__ __launch_bounds__(kBlock) void kmv_sym_finish_kernel(
   for (int64_t row0 = (int64_t)blockIdx.x * kFinishRows; row0 < n; row0 += (int64_t)gridDim.x * kFinishRows) {
     const int64_t i = row0 + r;
     const int64_t X = row0 / kSymTile, t = i % kSymTile;  // rows past n read their tile's padding partials
-    // The slice's partner count and its first 64 partners (clamped index,
-    // no dependence on the count), then x of the row: all in flight
-    // together, and before the partials.
-    const uint32_t* lst = LIST ? partners + (X * kFinishSlices + k) * pcap : nullptr;
+    // The slice's partner count, and its first kFinishBatch partial rows
+    // (slotted layout: the partners' rows in list order; clamped to the
+    // slice's pcap rows, no dependence on the count) with x of the row: all in
+    // flight together.  A batch past the count adds +0.0, which leaves a sum
+    // that started at +0.0 unchanged, so the bits are those of adding one by
+    // one in list order.
+    const PT* rows = LIST ? part_i + (X * kFinishSlices + k) * pcap * kSymTile + t : nullptr;
     const int craw = LIST ? pcount[X * kFinishSlices + k] : 0;
-    const int head = LIST ? (int)lst[r < pcap ? r : pcap - 1] : 0;
+    double v0[kFinishBatch];
+    if (LIST) {
+#pragma unroll
+      for (int u = 0; u < kFinishBatch; ++u) v0[u] = (double)rows[(u < pcap ? u : pcap - 1) * kSymTile];
+    }
     const double xi = (k == 0 && i < n) ? x[i] : 0.0;
     double s = 0.0;
     if (LIST) {
-      // keeps the list load here instead of after the count is known
-      asm volatile("" ::"v"(head));
-      // The partials go out kFinishBatch at a time, a batch past the count
-      // adding +0.0 (which leaves a sum that started at +0.0 unchanged), so
-      // the bits are those of adding one by one.
       const int cnt = __builtin_amdgcn_readfirstlane(craw);
-      for (int q0 = 0; q0 < cnt; q0 += kWave) {
-        const int m = cnt - q0 < kWave ? cnt - q0 : kWave;
-        const int mine = q0 == 0 ? head : (int)lst[q0 + r < pcap ? q0 + r : pcap - 1];
-        for (int q = 0; q < m; q += kFinishBatch) {
-          double v[kFinishBatch];
 #pragma unroll
-          for (int u = 0; u < kFinishBatch; ++u) {
-            const int qu = q + u < m ? q + u : m - 1;
-            const double pv = sym_partial(part_i, part_j, X, (int64_t)__builtin_amdgcn_readlane(mine, qu), t);
-            v[u] = q + u < m ? pv : 0.0;
-          }
+      for (int u = 0; u < kFinishBatch; ++u) s += u < cnt ? v0[u] : 0.0;
+      for (int q = kFinishBatch; q < cnt; q += kFinishBatch) {
+        double v[kFinishBatch];
 #pragma unroll
-          for (int u = 0; u < kFinishBatch; ++u) s += v[u];
+        for (int u = 0; u < kFinishBatch; ++u) {
+          const int qu = q + u < cnt ? q + u : cnt - 1;
+          const double pv = (double)rows[qu * kSymTile];
+          v[u] = q + u < cnt ? pv : 0.0;
         }
+#pragma unroll
+        for (int u = 0; u < kFinishBatch; ++u) s += v[u];
       }
     } else {
       for (int64_t P = k; P < ntiles; P += kFinishSlices) s += sym_partial(part_i, part_j, X, P, t);
@@ -1214,6 +1246,8 @@ struct SymArgs {
   double *pi, *pj;
   const uint32_t* list;
   const int64_t* count;
+  const int32_t* slots;  // planned: the plan's slotted partial rows (level 0)
+  int64_t pcap;
   // planned FP32 without coincidence shift: the plans' tile data, for the
   // planned kernel (matrix-core form on compact target tiles); null otherwise
   const PlanLevels* lv;
@@ -1227,28 +1261,26 @@ void launch_sym_dim(hipStream_t st, bool coinc, const mplak_kernel_desc* d, cons
   if constexpr (MODE == kSymPlanned && sizeof(T) == 8) {
     if (a.lv != nullptr && !coinc) {
       hipLaunchKernelGGL((kmv_sym_planned64_kernel<KIND, DIM>), grid, dim3(kBlock), 0, st, d->points, d->row_offset,
-                         d->row_count, a.scale, a.x, a.pi, a.pj, a.lv->list[0], a.lv->count[0], a.lv->tmask[0],
-                         a.skip);
+                         d->row_count, a.scale, a.x, a.pi, a.lv->list[0], a.lv->count[0], a.lv->tmask[0],
+                         a.lv->slots[0], a.lv->pcap, a.skip);
       return;
     }
   }
   if constexpr (MODE == kSymPlanned && sizeof(T) == 4) {
     if (a.lv != nullptr && !coinc) {
-      float* fi = reinterpret_cast<float*>(a.pi);
-      float* fj = fi + (size_t)(nt * (nt + 1) / 2) * kSymTile;
       hipLaunchKernelGGL((kmv_sym_planned_kernel<KIND, DIM>), grid, dim3(kBlock), 0, st, d->points, d->row_offset,
-                         d->row_count, a.scale, a.x, fi, fj, *a.lv, a.skip);
+                         d->row_count, a.scale, a.x, reinterpret_cast<float*>(a.pi), *a.lv, a.skip);
       return;
     }
   }
   if (coinc)
     hipLaunchKernelGGL((kmv_sym_kernel<T, KIND, DIM, true, MODE>), grid, dim3(kBlock), 0, st, d->points,
                        d->row_offset, d->row_count, a.scale, a.thr, d->coincident_shift, a.drop_s, a.x, a.pi, a.pj,
-                       a.list, a.count, a.skip);
+                       a.list, a.count, a.slots, a.pcap, a.skip);
   else
     hipLaunchKernelGGL((kmv_sym_kernel<T, KIND, DIM, false, MODE>), grid, dim3(kBlock), 0, st, d->points,
                        d->row_offset, d->row_count, a.scale, a.thr, d->coincident_shift, a.drop_s, a.x, a.pi, a.pj,
-                       a.list, a.count, a.skip);
+                       a.list, a.count, a.slots, a.pcap, a.skip);
 }
 
 template <typename T, int KIND, int MODE>
@@ -1307,6 +1339,8 @@ int run_sym(const mplak_kernel_desc* d, int precision, int mode, double drop_s, 
   a.x = x;
   a.list = lv != nullptr ? lv->list[0] : nullptr;
   a.count = lv != nullptr ? lv->count[0] : nullptr;
+  a.slots = lv != nullptr ? lv->slots[0] : nullptr;
+  a.pcap = lv != nullptr ? lv->pcap : 0;
   a.skip = skip_if;
   const int64_t nt = sym_tiles(d->row_count);
   a.pi = static_cast<double*>(work);
@@ -1363,8 +1397,8 @@ constexpr int kPlanKeyBits = 7;         // key classes: 96 - cost (cost 1 .. 96)
 constexpr int kPlanCounts = 5;          // entries written by mplak_screen_plan_counts
 
 struct PlanLayout {
-  size_t count, tinfo, boxes, sboxes, list, bmask, tmask, keys_in, keys_out, tmp, tmp_bytes, pcount, partners, part,
-      total;
+  size_t count, tinfo, boxes, sboxes, list, bmask, tmask, keys_in, keys_out, tmp, tmp_bytes, pcount, partners, slots,
+      part, total;
   int64_t pcap;  // partner list capacity per (tile, slice)
 };
 
@@ -1390,8 +1424,11 @@ PlanLayout plan_layout(int64_t n) {
   L.pcap = (nt + kFinishSlices - 1) / kFinishSlices;
   L.pcount = L.tmp + align_up(tb, 256);
   L.partners = L.pcount + align_up(sizeof(int32_t) * kFinishSlices * (size_t)nt, 256);
-  L.part = L.partners + align_up(sizeof(uint32_t) * (size_t)nt * kFinishSlices * (size_t)L.pcap, 256);
-  L.total = L.part + 2 * np * kSymTile * sizeof(double);
+  L.slots = L.partners + align_up(sizeof(uint32_t) * (size_t)nt * kFinishSlices * (size_t)L.pcap, 256);
+  L.part = L.slots + align_up(sizeof(int32_t) * 2 * np, 256);
+  // partial rows: by pair (2 per pair) without a plan, slotted with one
+  const size_t rows = std::max((size_t)2 * np, (size_t)nt * kFinishSlices * (size_t)L.pcap);
+  L.total = L.part + rows * kSymTile * sizeof(double);
   return L;
 }
 
@@ -1404,6 +1441,8 @@ void plan_level_view(const PlanLayout& L, char* base, int l, PlanLevels& lv) {
   lv.tmask[l] = reinterpret_cast<const uint32_t*>(base + L.tmask);
   lv.partners[l] = reinterpret_cast<const uint32_t*>(base + L.partners);
   lv.pcount[l] = reinterpret_cast<const int32_t*>(base + L.pcount);
+  lv.slots[l] = reinterpret_cast<const int32_t*>(base + L.slots);
+  lv.pcap = L.pcap;
 }
 
 template <typename T>
@@ -1629,7 +1668,8 @@ __global__ __launch_bounds__(kBlock) void plan_count_kernel(const uint64_t* __re
 // slice); keys are the unsorted per-pair keys).
 __global__ __launch_bounds__(kBlock) void plan_partner_kernel(const uint64_t* __restrict__ keys, int64_t nt,
                                                               int64_t pcap, uint32_t* __restrict__ partners,
-                                                              int32_t* __restrict__ pcount) {
+                                                              int32_t* __restrict__ pcount,
+                                                              int32_t* __restrict__ slots) {
   // one wave per list (tile X, slice k): lane l tests P = k + 4 (q0 + l), and
   // the kept ones are written in P order (ballot, then each lane's rank)
   const int lane = threadIdx.x & (kWave - 1);
@@ -1646,7 +1686,13 @@ __global__ __launch_bounds__(kBlock) void plan_partner_kernel(const uint64_t* __
       kept = (keys[p] >> 32) != kPlanDropped;
     }
     const uint64_t m = __ballot(kept);
-    if (kept) lst[c + __popcll(m & ((1ull << lane) - 1ull))] = (uint32_t)P;
+    if (kept) {
+      // the pair's partial rows of X go to row `pos` of this list's slots
+      const int32_t pos = c + __popcll(m & ((1ull << lane) - 1ull));
+      lst[pos] = (uint32_t)P;
+      if (P >= X) slots[2 * sym_pair_index(X, P)] = pos;                 // X is tile I of the pair
+      if (P <= X) slots[2 * sym_pair_index(P, X) + 1] = P == X ? 0 : pos;  // X is tile J (none on the diagonal)
+    }
     c += __popcll(m);
   }
   if (lane == 0) pcount[id] = c;
@@ -1767,7 +1813,7 @@ int mplak_screen_plan_build(const mplak_kernel_desc* d, int precision, double dr
   if (st != MPLAK_OK) return st;
   hipLaunchKernelGGL(plan_partner_kernel, dim3((unsigned)((nt * kFinishSlices * kWave + kBlock - 1) / kBlock)), dim3(kBlock),
                      0, hs, kin, nt, L.pcap, reinterpret_cast<uint32_t*>(base + L.partners),
-                     reinterpret_cast<int32_t*>(base + L.pcount));
+                     reinterpret_cast<int32_t*>(base + L.pcount), reinterpret_cast<int32_t*>(base + L.slots));
   st = launch_status();
   if (st != MPLAK_OK) return st;
   size_t tb = L.tmp_bytes;
