@@ -540,6 +540,38 @@ def test_mixed_precision_reuses_its_graph_for_new_points(s, oracle):
     assert not np.array_equal(sols[0], sols[1])
 
 
+def test_mixed_precision_replays_its_setup_on_new_contents(s, oracle):
+    """Three mixed solves through the same point, right-hand-side and
+    solution buffers, whose contents are overwritten in place between the
+    solves (system 1, system 2, system 1 again): the later solves replay the
+    captured inner-iteration chunk with unchanged arguments, so everything
+    behind them (order, plans, vectors) must come from the new contents.
+    Each solve reaches the FP64 bar on the system it was given, and the third
+    repeats the first bitwise."""
+    from tests.dropin_util import to_device
+    n = 6000
+    systems = [gaussian_system(oracle, n, 0.05, 1.0, 0, seed=seed) for seed in (1311, 1312)]
+    P = Points(systems[0][0])
+    kd = mpla.mpla_kernel_matrix_data(P.ptr, 3, 0, 0, mpla.MPLA_PREC_F64, 0.05, 1.0, 0.0)
+    A = s.generic(n, n, C.addressof(kd))
+    b = s.vector(n)
+    x = s.vector(n)
+    sols = []
+    for which in (0, 1, 0):
+        pts, bh, k = systems[which]
+        to_device(P.buf.ptr, np.ascontiguousarray(pts, dtype=np.float64).ravel())
+        s.set(b, bh)
+        s.set(x, np.zeros(n))
+        s.lib.mpla_kernel_conjugate_gradient_mixed(C.byref(b), C.byref(A), C.byref(x), 5000, 1e-8,
+                                                   mpla.MPLA_PREC_F32, s.I)
+        assert s.inst.last_converged == 1
+        xs = s.get(x)
+        assert np.linalg.norm(bh - k.matvec(xs)) / np.linalg.norm(bh) < 1e-8 * 1.01, f"system {which}"
+        sols.append(xs)
+    assert np.array_equal(sols[0], sols[2])
+    assert not np.array_equal(sols[0], sols[1])
+
+
 def test_graph_replay_is_deterministic(s, oracle):
     n = 4096
     pts, bh, k = gaussian_system(oracle, n)
