@@ -1116,21 +1116,206 @@ __device__ __forceinline__ void sym_tile64_pair(const double* const* __restrict_
   }
 }
 
+// FP64 matrix-core form of the planned FP64 tile pair: Gaussian kernel,
+// compact target tile.  Coordinates are centred on tile I's box centre;
+// v_mfma_f64_16x16x4f64 gives t = |c_s|^2 - 2 c_t.c_s for a 16 x 16 tile
+// (A = [c_t, 1], B = [-2 c_s, |c_s|^2]; A's rows permuted, below) in the
+// lane layout of the difference form (lane 16 r + c: targets
+// 16 (4 a + w) + 4 r + v, v < 4, source 64 g + 16 b + c).  The entry 2^-(|c_t|^2 + t) = 2^-|c_t|^2 2^-t: the factor
+// 2^-|c_t|^2 goes into the target values (column sums) and into each row sum
+// once, so the loop takes 2^-t only (t >= -|c_t|^2 >= -kCenterRadius2).  The
+// rounding of t is at the level of |c|^2 ulps, which the compact target tile
+// bounds for every entry that matters.  Same 16 x 16 tile screen, passes and
+// reduction order as the difference form.
+// LDS layout (doubles): target operand [32 slabs][48] (lane 16 k + c of a
+// slab: coordinate k of its centred target c, k < 3), source operand
+// [32 slabs][64] (k < 3: -2 x coordinate k; k = 3: |c_s|^2), target values
+// times 2^-|c_t|^2, source values, the exp2 table; the column sums of the
+// four waves reuse the operands.
+constexpr int kM64Tgt = 0;
+constexpr int kM64Src = kSlabs * 48;
+constexpr int kM64Xt = kM64Src + kSlabs * kWave;
+constexpr int kM64Xs = kM64Xt + kSymTile;
+constexpr int kM64Tab = kM64Xs + kSymTile;
+constexpr int kM64Doubles = kM64Tab + kExp2Table;
+static_assert((kBlock / kWave) * kSymTile <= kM64Xt, "column sums reuse the operands");
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+template <int DIM>
+__device__ __forceinline__ void sym_mfma64_pair(const double* const* __restrict__ points, int64_t off, int64_t n,
+                                                double scale, const double* __restrict__ x,
+                                                double* __restrict__ rows_i, double* __restrict__ rows_j, int64_t p,
+                                                int64_t I, int64_t J, float4 centre,
+                                                const uint32_t* __restrict__ tmask, double* lds) {
+  const int tid = threadIdx.x, lane = tid & (kWave - 1);
+  const int w = __builtin_amdgcn_readfirstlane(tid / kWave);
+  const int c16 = lane & 15, r4 = lane >> 4;
+  const double ctr[3] = {(double)centre.x, (double)centre.y, (double)centre.z};
+  double* tgt = lds + kM64Tgt;
+  double* src = lds + kM64Src;
+  double* xt = lds + kM64Xt;
+  double* xs = lds + kM64Xs;
+  double* tab = lds + kM64Tab;
+  // staging: thread t takes points t and t + 256 of both tiles; every global
+  // load is issued before the LDS writes (clamped indices, no branches)
+  constexpr int kPer = kSymTile / kBlock;
+  double pc[2][kPer][3], pv[2][kPer];
+#pragma unroll
+  for (int side = 0; side < 2; ++side) {  // 0: sources (tile J), 1: targets (tile I)
+#pragma unroll
+    for (int it = 0; it < kPer; ++it) {
+      const int64_t j = (side == 0 ? J : I) * kSymTile + tid + it * kBlock;
+      const int64_t jc = j < n ? j : n - 1;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) pc[side][it][d] = global_load(points[d] + off + jc);
+      pv[side][it] = global_load(x + jc);
+    }
+  }
+#pragma unroll
+  for (int side = 0; side < 2; ++side) {
+#pragma unroll
+    for (int it = 0; it < kPer; ++it) {
+      const int k = tid + it * kBlock;
+      const int64_t j = (side == 0 ? J : I) * kSymTile + k;
+      double c[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) c[d] = pc[side][it][d] * scale - ctr[d];
+      const double nc = fma(c[2], c[2], fma(c[1], c[1], c[0] * c[0]));
+      const double xv = j < n ? pv[side][it] : 0.0;
+      const int slab = k >> 4, cc = k & 15;
+      if (side == 0) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) src[slab * kWave + 16 * d + cc] = -2.0 * c[d];
+        src[slab * kWave + 48 + cc] = nc;
+        xs[k] = xv;
+      } else {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) tgt[slab * 48 + 16 * d + cc] = c[d];
+        xt[k] = xv * exp2(-nc);
+      }
+    }
+  }
+  fill_exp2_table(tab);
+  uint32_t gmask[kSymGroups];
+  const uint32_t* tm = tmask + p * kTmaskWords + w * kSymGroups;
+#pragma unroll
+  for (int g = 0; g < kSymGroups; ++g) gmask[g] = (uint32_t)__builtin_amdgcn_readfirstlane((int)tm[g]);
+  __syncthreads();
+
+  double tg[kSymGroups];
+#pragma unroll
+  for (int g = 0; g < kSymGroups; ++g) tg[g] = 0.0;
+  auto group_mask = [&](int g) -> uint32_t {
+    uint32_t m = 0;
+#pragma unroll
+    for (int q = 0; q < kSymGroups; ++q)
+      if (q == g) m = gmask[q];
+    return m;
+  };
+  const f64x4 zero4 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 1
+  for (int h = 0; h < 2; ++h) {
+    double accR[4][4], Aop[4];
+#pragma unroll
+    for (int a2 = 0; a2 < 4; ++a2) {
+      // Result i of the FP64 MFMA in lane 16 r + c is row 4 i + r of the
+      // tile (column c), so row 4 v + r of A takes target 4 r + v of the
+      // slab: lane 16 k + c holds coordinate k of target 4 (c mod 4) + c / 4
+      // of slab 4 (4 h + a2) + w (1 at k = 3), and result v of lane 16 r + c
+      // is target 4 r + v, as in the difference form.
+      const int st = 4 * (4 * h + a2) + w;
+      const int tc = 4 * (c16 & 3) + (c16 >> 2);
+      const double av = tgt[st * 48 + (r4 < 3 ? 16 * r4 + tc : 0)];
+      Aop[a2] = r4 < 3 ? av : 1.0;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) accR[a2][v] = 0.0;
+    }
+    const uint32_t half_bits = 0x0f0f0f0fu << (4 * h);
+#pragma unroll 1
+    for (int g = 0; g < kSymGroups; ++g) {
+      const uint32_t gm = group_mask(g);
+      if ((gm & half_bits) == 0) continue;
+      double accC[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int a2 = 0; a2 < 4; ++a2) {
+        const int a = 4 * h + a2;
+        const uint32_t cm =
+            ((gm >> a) & 1u) | ((gm >> (7 + a)) & 2u) | ((gm >> (14 + a)) & 4u) | ((gm >> (21 + a)) & 8u);
+        if (cm == 0) continue;
+        const int t0 = 16 * (4 * a + w) + 4 * r4;
+        double xp[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) xp[v] = xt[t0 + v];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          if (((cm >> b) & 1u) == 0) continue;
+          const double bq = src[(4 * g + b) * kWave + lane];
+          const double xq = xs[g * kWave + 16 * b + c16];
+          const f64x4 t = __builtin_amdgcn_mfma_f64_16x16x4f64(Aop[a2], bq, zero4, 0, 0, 0);
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const double e = exp2_neg(t[v], tab);
+            accR[a2][v] = fma(e, xq, accR[a2][v]);
+            accC[b] = fma(e, xp[v], accC[b]);
+          }
+        }
+      }
+      const double cs = rows4_reduce_scatter_f64(accC, r4);  // source 16 r + c of the group
+#pragma unroll
+      for (int q2 = 0; q2 < kSymGroups; ++q2)
+        if (q2 == g) tg[q2] += cs;
+    }
+    double rv[16];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) rv[4 * a + v] = accR[a][v];
+    const double out = row16_reduce_scatter_f64(rv, c16);
+    const int k = 16 * (4 * (4 * h + (c16 >> 2)) + w) + 4 * r4 + (c16 & 3);
+    // the row's factor 2^-|c_t|^2, from the staged coordinates as in staging
+    const int slab = k >> 4, cc = k & 15;
+    const double t0c = tgt[slab * 48 + cc], t1c = tgt[slab * 48 + 16 + cc], t2c = tgt[slab * 48 + 32 + cc];
+    const double nt = fma(t2c, t2c, fma(t1c, t1c, t0c * t0c));
+    rows_i[k] = out * exp2(-nt);
+  }
+  if (I != J) {
+    __syncthreads();  // every wave is done reading the staged tiles
+#pragma unroll
+    for (int g = 0; g < kSymGroups; ++g) lds[w * kSymTile + g * kWave + lane] = tg[g];
+    __syncthreads();
+    for (int k = tid; k < kSymTile; k += kBlock) {
+      double sum = lds[k];
+#pragma unroll
+      for (int q2 = 1; q2 < kBlock / kWave; ++q2) sum += lds[q2 * kSymTile + k];
+      rows_j[k] = sum;
+    }
+  }
+}
+
+// Four waves per SIMD (128 registers): left to itself the compiler gives the
+// Gaussian kernel, which holds both tile forms, 146 and three waves.
 template <int KIND, int DIM>
-__global__ __launch_bounds__(kBlock) void kmv_sym_planned64_kernel(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) void kmv_sym_planned64_kernel(
     const double* const* __restrict__ points, int64_t off, int64_t n, double scale, const double* __restrict__ x,
     double* __restrict__ part, const uint32_t* __restrict__ list, const int64_t* __restrict__ count,
-    const uint32_t* __restrict__ tmask, const int32_t* __restrict__ slots, int64_t pcap,
-    const int32_t* __restrict__ skip) {
+    const uint32_t* __restrict__ tmask, const float4* __restrict__ tinfo, const int32_t* __restrict__ slots,
+    int64_t pcap, const int32_t* __restrict__ skip) {
   if (skip != nullptr && *skip != 0) return;
   if ((int64_t)blockIdx.x >= *count) return;
-  __shared__ __attribute__((aligned(16))) double buf[kT64Doubles];
+  constexpr int kBufDoubles = kM64Doubles > kT64Doubles ? kM64Doubles : kT64Doubles;
+  __shared__ __attribute__((aligned(16))) double buf[kBufDoubles];
   const int64_t p = (int64_t)list[blockIdx.x];
   int64_t I, J;
   decode_pair(p, I, J);
   double* rows_i = slot_row(part, I, J, slots[2 * p], pcap);
   double* rows_j = I != J ? slot_row(part, J, I, slots[2 * p + 1], pcap) : rows_i;
-  sym_tile64_pair<KIND, DIM>(points, off, n, scale, x, rows_i, rows_j, p, I, J, tmask, buf);
+  // FP64 plans mark compact tiles 2, the others 1 (both take the tile screen)
+  const float4 centre = tinfo[I];
+  if (KIND == MPLAK_KERNEL_GAUSSIAN && centre.w == 2.0f)
+    sym_mfma64_pair<DIM>(points, off, n, scale, x, rows_i, rows_j, p, I, J, centre, tmask, buf);
+  else
+    sym_tile64_pair<KIND, DIM>(points, off, n, scale, x, rows_i, rows_j, p, I, J, tmask, buf);
 }
 
 // y_i = sum over partner tiles P of row i's tile X of that pair's partial,
@@ -1262,7 +1447,7 @@ void launch_sym_dim(hipStream_t st, bool coinc, const mplak_kernel_desc* d, cons
     if (a.lv != nullptr && !coinc) {
       hipLaunchKernelGGL((kmv_sym_planned64_kernel<KIND, DIM>), grid, dim3(kBlock), 0, st, d->points, d->row_offset,
                          d->row_count, a.scale, a.x, a.pi, a.lv->list[0], a.lv->count[0], a.lv->tmask[0],
-                         a.lv->slots[0], a.lv->pcap, a.skip);
+                         a.lv->tinfo[0], a.lv->slots[0], a.lv->pcap, a.skip);
       return;
     }
   }
@@ -1502,7 +1687,7 @@ __device__ __forceinline__ void load_box(const double* b, T* out) {
   for (int d = 0; d < 6; ++d) out[d] = (T)b[d];
 }
 
-// tinfo[t] = (centre of tile t's box, 1 if the tile is compact else 0).
+// tinfo[t] = (centre of tile t's box, compactness flag: see below).
 template <typename T, int DIM>
 __global__ __launch_bounds__(kBlock) void plan_tile_kernel(const double* __restrict__ boxes, int64_t nt,
                                                            float4* __restrict__ tinfo) {
@@ -1527,11 +1712,13 @@ __global__ __launch_bounds__(kBlock) void plan_tile_kernel(const double* __restr
     const T half = T(0.5) * (hi[d] - lo[d]);
     r2 = fma(half, half, r2);
   }
-  // FP32: the matrix-core form's centring bound; FP64: the tile form has no
-  // such bound and takes every tile
-  const bool compact = sizeof(T) == 8 || r2 <= (T)kCenterRadius2;
+  // the matrix-core forms' centring bound.  FP32 plans: 1 compact, 0 not
+  // (vector form, block screen); FP64 plans: 2 compact, 1 not (every tile
+  // takes the tile screen; the compact ones the matrix-core form)
+  const bool compact = r2 <= (T)kCenterRadius2;
+  const float flag = sizeof(T) == 8 ? (compact ? 2.0f : 1.0f) : (compact ? 1.0f : 0.0f);
   tinfo[t] = make_float4((float)(T(0.5) * (lo[0] + hi[0])), (float)(T(0.5) * (lo[1] + hi[1])),
-                         (float)(T(0.5) * (lo[2] + hi[2])), compact ? 1.0f : 0.0f);
+                         (float)(T(0.5) * (lo[2] + hi[2])), flag);
 }
 
 // tmask[p][w][g] (one thread each).
