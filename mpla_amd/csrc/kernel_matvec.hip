@@ -264,11 +264,12 @@ constexpr int kFinishRows = kWave;
 constexpr int kFinishSlices = kBlock / kWave;
 constexpr int kFinishBatch = 8;  // independent partial loads in flight per wave
 
-// Partial rows of a plan's listed pairs ("slotted" layout): tile X's
-// partner lists in slices k = P mod kFinishSlices, pcap rows of 512 each,
-// hold the rows of X of its listed pairs (X, P) in list order, so the finish
-// reads a list's partials without the list.  slots[2 p] is the row of pair
-// p = (I, J) in I's list, slots[2 p + 1] its row in J's list (I < J).
+// Partial rows of a plan's listed pairs ("slotted" layout): tile X has one
+// block of pcap rows of 512 per slice k = P mod kFinishSlices, holding the
+// rows of X of its listed pairs (X, P) of that slice in ascending P, so the
+// finish reads a slice's partials knowing only their count.  slots[2 p] is
+// the row of pair p = (I, J) in I's block, slots[2 p + 1] its row in J's
+// block (I < J).
 template <typename P>
 __device__ __forceinline__ P* slot_row(P* part, int64_t X, int64_t partner, int32_t slot, int64_t pcap) {
   return part + ((X * kFinishSlices + (partner & (kFinishSlices - 1))) * pcap + slot) * kSymTile;
@@ -863,7 +864,6 @@ struct PlanLevels {
   const float4* tinfo[kMaxLevels];
   const uint32_t* bmask[kMaxLevels];
   const uint32_t* tmask[kMaxLevels];
-  const uint32_t* partners[kMaxLevels];
   const int32_t* pcount[kMaxLevels];
   const int32_t* slots[kMaxLevels];
   double below[kMaxLevels];
@@ -1325,7 +1325,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) 
 // (planned product) a wave adds only the listed partners of its slice, in the
 // same order; the partials it leaves out are zero, so the result is the same.
 // Their rows are contiguous in the plan's slotted layout, so the wave needs
-// only the list's length; kFinishBatch independent partial loads go out at a
+// only their count; kFinishBatch independent partial loads go out at a
 // time.  With `dots` the finish also leaves, per workgroup, the partial sums
 // of <x,x> and <x,y> over its rows (wave butterfly, so a fixed order):
 // dots[b] and dots[gridDim.x + b].
@@ -1582,9 +1582,9 @@ constexpr int kPlanKeyBits = 7;         // key classes: 96 - cost (cost 1 .. 96)
 constexpr int kPlanCounts = 5;          // entries written by mplak_screen_plan_counts
 
 struct PlanLayout {
-  size_t count, tinfo, boxes, sboxes, list, bmask, tmask, keys_in, keys_out, tmp, tmp_bytes, pcount, partners, slots,
+  size_t count, tinfo, boxes, sboxes, list, bmask, tmask, keys_in, keys_out, tmp, tmp_bytes, pcount, slots,
       part, total;
-  int64_t pcap;  // partner list capacity per (tile, slice)
+  int64_t pcap;  // partner slots per (tile, slice)
 };
 
 PlanLayout plan_layout(int64_t n) {
@@ -1608,8 +1608,7 @@ PlanLayout plan_layout(int64_t n) {
   L.tmp_bytes = tb;
   L.pcap = (nt + kFinishSlices - 1) / kFinishSlices;
   L.pcount = L.tmp + align_up(tb, 256);
-  L.partners = L.pcount + align_up(sizeof(int32_t) * kFinishSlices * (size_t)nt, 256);
-  L.slots = L.partners + align_up(sizeof(uint32_t) * (size_t)nt * kFinishSlices * (size_t)L.pcap, 256);
+  L.slots = L.pcount + align_up(sizeof(int32_t) * kFinishSlices * (size_t)nt, 256);
   L.part = L.slots + align_up(sizeof(int32_t) * 2 * np, 256);
   // partial rows: by pair (2 per pair) without a plan, slotted with one
   const size_t rows = std::max((size_t)2 * np, (size_t)nt * kFinishSlices * (size_t)L.pcap);
@@ -1624,7 +1623,6 @@ void plan_level_view(const PlanLayout& L, char* base, int l, PlanLevels& lv) {
   lv.tinfo[l] = reinterpret_cast<const float4*>(base + L.tinfo);
   lv.bmask[l] = reinterpret_cast<const uint32_t*>(base + L.bmask);
   lv.tmask[l] = reinterpret_cast<const uint32_t*>(base + L.tmask);
-  lv.partners[l] = reinterpret_cast<const uint32_t*>(base + L.partners);
   lv.pcount[l] = reinterpret_cast<const int32_t*>(base + L.pcount);
   lv.slots[l] = reinterpret_cast<const int32_t*>(base + L.slots);
   lv.pcap = L.pcap;
@@ -1850,20 +1848,20 @@ __global__ __launch_bounds__(kBlock) void plan_count_kernel(const uint64_t* __re
   }
 }
 
-// Partner lists of the finish: for tile X and slice k, the tiles P = k (mod 4)
-// in ascending order whose pair with X is listed (one wave per (tile,
-// slice); keys are the unsorted per-pair keys).
+// Partner slots of the finish: for tile X and slice k, the tiles P = k (mod 4)
+// in ascending order whose pair with X is listed take slots 0, 1, ... of that
+// (tile, slice): the slot of each kept pair's two sides and the count per
+// (tile, slice) (one wave per (tile, slice); keys are the unsorted per-pair
+// keys).
 __global__ __launch_bounds__(kBlock) void plan_partner_kernel(const uint64_t* __restrict__ keys, int64_t nt,
-                                                              int64_t pcap, uint32_t* __restrict__ partners,
                                                               int32_t* __restrict__ pcount,
                                                               int32_t* __restrict__ slots) {
-  // one wave per list (tile X, slice k): lane l tests P = k + 4 (q0 + l), and
-  // the kept ones are written in P order (ballot, then each lane's rank)
+  // one wave per (tile X, slice k): lane l tests P = k + 4 (q0 + l), and the
+  // kept ones are ranked in P order (ballot, then each lane's rank)
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t id = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kWave;
   if (id >= nt * kFinishSlices) return;
   const int64_t X = id / kFinishSlices, k = id % kFinishSlices;
-  uint32_t* lst = partners + id * pcap;
   int32_t c = 0;
   for (int64_t P0 = k; P0 < nt; P0 += kFinishSlices * kWave) {
     const int64_t P = P0 + kFinishSlices * lane;
@@ -1876,7 +1874,6 @@ __global__ __launch_bounds__(kBlock) void plan_partner_kernel(const uint64_t* __
     if (kept) {
       // the pair's partial rows of X go to row `pos` of this list's slots
       const int32_t pos = c + __popcll(m & ((1ull << lane) - 1ull));
-      lst[pos] = (uint32_t)P;
       if (P >= X) slots[2 * sym_pair_index(X, P)] = pos;                 // X is tile I of the pair
       if (P <= X) slots[2 * sym_pair_index(P, X) + 1] = P == X ? 0 : pos;  // X is tile J (none on the diagonal)
     }
@@ -1999,7 +1996,7 @@ int mplak_screen_plan_build(const mplak_kernel_desc* d, int precision, double dr
   st = launch_status();
   if (st != MPLAK_OK) return st;
   hipLaunchKernelGGL(plan_partner_kernel, dim3((unsigned)((nt * kFinishSlices * kWave + kBlock - 1) / kBlock)), dim3(kBlock),
-                     0, hs, kin, nt, L.pcap, reinterpret_cast<uint32_t*>(base + L.partners),
+                     0, hs, kin, nt,
                      reinterpret_cast<int32_t*>(base + L.pcount), reinterpret_cast<int32_t*>(base + L.slots));
   st = launch_status();
   if (st != MPLAK_OK) return st;
