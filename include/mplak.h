@@ -179,9 +179,10 @@ int mplak_kernel_matvec_screened(const mplak_kernel_desc* desc, int precision, d
  * arguments, without launching work for tile pairs whose entries are all
  * dropped.  FP64 with an FP64 plan and no coincidence shift: the screen is
  * applied per 16 x 16 block of points, with the entries from FP64
- * coordinate differences, so it drops only entries below 2^-drop_log2 and
- * agrees with the screened product to FP64 accuracy plus those entries
- * (deterministic).  FP32 with an FP32 plan and no coincidence shift: on target tiles
+ * coordinate differences (Gaussian kernel on compact target tiles: FP64
+ * squared distances from the matrix cores in tile-centred form), so it
+ * drops only entries below 2^-drop_log2 and agrees with the screened
+ * product to FP64 accuracy plus those entries (deterministic).  FP32 with an FP32 plan and no coincidence shift: on target tiles
  * whose bounding box is compact (half-diagonal <= 7 in the scaled
  * coordinates, the usual case after mplak_spatial_order) the squared
  * distances come from the matrix cores in tile-centred form and the screen
